@@ -1,0 +1,210 @@
+"""Benchmark: plain GPU decode + develop of a ProRes RAW frame on one MI355X.
+
+Workload (one step): the 4096x2160 BGGR frame of the plain-GPU-at-scale
+configuration (bitstream v1, tile alignment 3, qscale 45, seeded quant
+matrix and 8-point curve, 2% of tiles corrupted) is decoded with the Apple
+transform, linearized in the same kernel, and developed to Rec.2020 planes.
+The packet is resident in device memory before timing starts; the step is
+captured once in a HIP graph and replayed.
+
+Prints one JSON line. `value` is Gsensels/s (visible width x height per
+second). `roofline` covers the develop kernel (the dominant one) against
+HBM bandwidth; `cpu_baseline` times the oracle (a C restatement of the
+crate's CPU path, threaded like its rayon code) on a bounded sample.
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+
+def parse_args():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--width", type=int, default=4096)
+    ap.add_argument("--height", type=int, default=2160)
+    ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    return ap.parse_args()
+
+
+def main():
+    args = parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are out of scope "
+                                   "(frames shard across GPUs with no exchange)"}))
+        sys.exit(2)
+    import torch
+
+    import proresraw_rs_amd as P
+    from proresraw_rs_amd import _native as N
+    from proresraw_rs_amd import synth as S
+    from proresraw_rs_amd.workloads import config1
+
+    dev = torch.device("cuda:0")
+    lib = N.lib()
+    ctx = P.context()
+    sc = config1(args.width, args.height)
+    packet = S.encode_scene(sc)
+    hdr, info = P.parse_frame_info(packet)
+    raw_hdr = hdr._raw
+    w, h = hdr.width, hdr.height
+    cw, ch = info.coded_width, info.coded_height
+    npix = w * h
+
+    padded = ((len(packet) + 3) & ~3) + 16
+    d_packet = torch.zeros(padded, dtype=torch.uint8, device=dev)
+    d_packet[:len(packet)] = torch.frombuffer(bytearray(packet), dtype=torch.uint8).to(dev)
+    d_samples = torch.empty(cw * ch, dtype=torch.int16, device=dev)
+    d_linear = torch.empty(npix, dtype=torch.int16, device=dev)
+    d_flags = torch.empty(max(info.nb_tiles, 1), dtype=torch.uint8, device=dev)
+    d_status = torch.zeros(4, dtype=torch.int32, device=dev)
+    planes = [torch.empty(npix, dtype=torch.float32, device=dev) for _ in range(3)]
+    err = N.PrrError()
+
+    # Params need max_qscale: one decode, read the status back.
+    def decode(stream):
+        rc = lib.prr_decode_frame_device(
+            ctx.handle, ctypes.byref(raw_hdr), ctypes.byref(info), d_packet.data_ptr(),
+            len(packet), 0, d_samples.data_ptr(), d_linear.data_ptr(), d_flags.data_ptr(),
+            d_status.data_ptr(), stream, ctypes.byref(err))
+        if rc:
+            raise P.Error(err)
+
+    stream = torch.cuda.current_stream().cuda_stream
+    decode(stream)
+    torch.cuda.synchronize()
+    st = d_status.cpu().numpy().view(np.uint32)
+    if st[0] != 0xFFFFFFFF or st[3]:
+        raise RuntimeError(f"device status {st}")
+    max_qscale = int(st[1])
+    prm = P.DemosaicParams.new(hdr, hdr.quant_matrix[0] * max_qscale)._raw()
+
+    def develop(stream):
+        rc = lib.prr_develop_device(ctx.handle, ctypes.byref(raw_hdr), d_samples.data_ptr(),
+                                    ctypes.byref(prm), 2, planes[0].data_ptr(),
+                                    planes[1].data_ptr(), planes[2].data_ptr(), stream,
+                                    ctypes.byref(err))
+        if rc:
+            raise P.Error(err)
+
+    def step(stream):
+        decode(stream)
+        develop(stream)
+
+    # Per-kernel durations on the launch stream, eager.
+    def timed(fn, n=20):
+        s = torch.cuda.current_stream()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        fn(s.cuda_stream)
+        e0.record(s)
+        for _ in range(n):
+            fn(s.cuda_stream)
+        e1.record(s)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / n
+
+    t_decode_ms = timed(decode)
+    t_develop_ms = timed(develop)
+
+    graph = None
+    if not args.no_graph:
+        graph = torch.cuda.CUDAGraph()
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):
+            step(s.cuda_stream)  # warm: scratch allocated outside capture
+            torch.cuda.synchronize()
+            with torch.cuda.graph(graph, stream=s):
+                step(s.cuda_stream)
+        torch.cuda.current_stream().wait_stream(s)
+        torch.cuda.synchronize()
+
+    def run_step():
+        if graph is not None:
+            graph.replay()
+        else:
+            step(torch.cuda.current_stream().cuda_stream)
+
+    for _ in range(args.warmup):
+        run_step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        run_step()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms_per_step = 1e3 * elapsed / args.steps
+    value = npix * args.steps / elapsed / 1e9
+
+    # Develop roofline: algorithmic bytes = code values read (2 B per visible
+    # sensel) + three f32 planes written (12 B per visible sensel).
+    dev_bytes = 14.0 * npix
+    achieved = dev_bytes / (t_develop_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": 8000.0, "unit": "GB/s",
+                "frac": round(achieved / 8000.0, 4), "traffic": None,
+                "kernel": "k_develop", "kernel_ms": round(t_develop_ms, 4),
+                "decode_kernels_ms": round(t_decode_ms, 4)}
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(packet, args.cpu_seconds)
+
+    line = {
+        "metric": "Gsensels/s decode+develop, plain vs batched+learned demosaic at >=45 dB PSNR",
+        "value": round(value, 3),
+        "unit": "Gsensels/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u16+f32",
+        "data": "synthetic",
+        "config": {"workload": f"plain GPU decode(Apple)+linearize+develop(Rec2020), {w}x{h} BGGR "
+                               "v1 a3 q45, seeded qmat+curve, 2% corrupt tiles",
+                   "packet_bytes": len(packet), "graph": graph is not None},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    if cpu:
+        line["gpu_over_cpu"] = round(value / cpu["value"], 1)
+    print(json.dumps(line), flush=True)
+
+
+def cpu_baseline(packet, seconds):
+    """The oracle's decode + linearize + develop(Rec2020) of the same packet,
+    repeated for about `seconds`, on every host core (capped at 16)."""
+    from oracle import oracle as O
+    threads = O.THREADS
+    t0 = time.perf_counter()
+    n = 0
+    while True:
+        h, s, sk, mq = O.decode_frame(packet, threads=threads)
+        O.linearize(h, s, threads=threads)
+        prm = O.params(h, h.quant_matrix[0] * mq)
+        O.develop_planar(h, s, prm, 2, threads=threads)
+        n += 1
+        if time.perf_counter() - t0 >= seconds:
+            break
+    el = time.perf_counter() - t0
+    npix = h.width * h.height
+    return {"value": round(npix * n / el / 1e9, 5), "unit": "Gsensels/s", "cores": threads,
+            "kind": "port", "sample": f"{n} frames of the same {h.width}x{h.height} packet, "
+                                      f"{el:.1f} s, median-free mean"}
+
+
+if __name__ == "__main__":
+    main()

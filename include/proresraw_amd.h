@@ -1,0 +1,208 @@
+/* proresraw_amd: ProRes RAW decode and develop on AMD Instinct MI355X
+ * (gfx950) with hand-written HIP kernels, behind a C ABI.
+ *
+ * The entry points replace the public API of the Rust crate proresraw_rs
+ * (re-exported at src/lib.rs:50-66); each declaration names the reference
+ * function it stands in for. Conventions:
+ *   - every function returns 0 (PRR_OK) or a PRR_E_* status and, when `err`
+ *     is not NULL, fills it with the matching error payload;
+ *   - callers own every buffer; sizes are passed explicitly, and a size that
+ *     does not match returns PRR_E_OUTPUT_SIZE where the crate panics
+ *     ("output size", "plane size");
+ *   - malformed tiles are not errors: they are reported as skipped tiles;
+ *   - a prr_context owns one HIP stream and the device scratch of one host
+ *     thread; use one context per thread. Host-buffer entry points are
+ *     synchronous; *_device entry points enqueue on the given stream (NULL =
+ *     the context's stream) and take device pointers.
+ */
+#ifndef PRORESRAW_AMD_H
+#define PRORESRAW_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. 1..12 mirror the variants of the crate's Error enum
+ * (src/error.rs:7-63) in declaration order. */
+enum prr_status {
+    PRR_OK = 0,
+    PRR_E_TRUNCATED = 1,            /* Error::Truncated {what, needed=a, available=b} */
+    PRR_E_FRAME_SIZE_MISMATCH = 2,  /* Error::FrameSizeMismatch {declared=a, actual=b} */
+    PRR_E_BAD_QUANT_MATRIX = 3,     /* Error::BadQuantMatrix {index=a, value=b} */
+    PRR_E_BAD_CURVE = 4,            /* Error::BadCurve(points) */
+    PRR_E_UNEXPECTED_CURVE = 5,     /* Error::UnexpectedCurve */
+    PRR_E_BAD_TAG = 6,              /* Error::BadTag(tag) */
+    PRR_E_HEADER_TOO_SHORT = 7,     /* Error::HeaderTooShort(a) */
+    PRR_E_UNSUPPORTED_VERSION = 8,  /* Error::UnsupportedVersion(a) */
+    PRR_E_BAD_DIMENSIONS = 9,       /* Error::BadDimensions {width=a, height=b} */
+    PRR_E_BAD_ALIGNMENT = 10,       /* Error::BadAlignment(a) */
+    PRR_E_TILE_OUT_OF_BOUNDS = 11,  /* Error::TileOutOfBounds {tile=a} */
+    PRR_E_BAD_TILE_HEADER = 12,     /* Error::BadTileHeader {tile=a} */
+    PRR_E_OUTPUT_SIZE = 100,        /* a caller buffer has the wrong size (crate: panic) */
+    PRR_E_INVALID_ARGUMENT = 101,   /* NULL pointer, bad enum value, misaligned buffer */
+    PRR_E_DEVICE = 102,             /* a HIP runtime call failed; `what` names it */
+    PRR_E_DECODE_OVERRUN = 103,     /* a stream exceeded the per-stream symbol bound */
+};
+
+typedef struct {
+    int32_t code;
+    char what[28];
+    uint64_t a;
+    uint64_t b;
+    uint16_t points[8];
+    uint8_t tag[4];
+} prr_error;
+
+/* FrameHeader (src/header.rs:112-146). */
+typedef struct {
+    uint8_t version;
+    uint8_t vendor[4];
+    uint8_t bayer;            /* BayerPattern code: 0 RGGB, 1 GRBG, 2 GBRG, 3 BGGR */
+    uint16_t width;
+    uint16_t height;
+    uint8_t crop[4];          /* left, right, top, bottom */
+    uint16_t white_level;
+    uint16_t wb_cct;
+    uint16_t flags;
+    uint8_t alignment;
+    uint8_t quant_matrix[64];
+    uint16_t curve[8];
+    float wb_red;
+    float wb_blue;
+    float color_matrix[9];    /* row-major camera RGB to XYZ */
+    float gain;
+} prr_header;
+
+/* Sizes derived from a header (header.rs:249-256, tile.rs:547-561,
+ * reduced.rs:99-111). */
+typedef struct {
+    uint32_t coded_width;
+    uint32_t coded_height;
+    uint32_t tiles_per_row;
+    uint32_t tile_rows;
+    uint32_t nb_tiles;
+    size_t header_end;        /* bytes from the packet start to the size table */
+} prr_frame_info;
+
+/* DemosaicParams (src/demosaic.rs:57-74). */
+typedef struct {
+    int32_t cfa_phase;
+    int32_t black;
+    float one_over_range;
+    float noise[3];
+    float clip[2];
+    float black_over_range;
+    float wb[2];
+} prr_demosaic_params;
+
+enum prr_idct { PRR_IDCT_APPLE = 0, PRR_IDCT_FFMPEG = 1 };               /* idct.rs:19-26 */
+enum prr_scale { PRR_SCALE_HALF = 1, PRR_SCALE_QUARTER = 2, PRR_SCALE_EIGHTH = 3 }; /* reduced.rs:25-32 */
+enum prr_greens { PRR_GREENS_FIRST = 0, PRR_GREENS_AVERAGE = 1 };          /* reduced.rs:64-72 */
+enum prr_stage { PRR_STAGE_DEMOSAICED = 0, PRR_STAGE_WHITE_BALANCED = 1,
+                 PRR_STAGE_REC2020 = 2 };                                    /* demosaic.rs:112-123 */
+enum prr_exposure { PRR_EXPOSURE_NORMALIZED = 0, PRR_EXPOSURE_REFLECTION = 1 }; /* demosaic.rs:127-133 */
+
+/* Per-frame result the device path writes: the first out-of-bounds tile
+ * (UINT32_MAX when none; parse_frame's TileOutOfBounds), the largest tile
+ * qscale byte (ParsedFrame::max_qscale, decoder.rs:104-116), the number of
+ * skipped tiles and a non-zero `overrun` if a stream hit the symbol bound. */
+typedef struct {
+    uint32_t oob_tile;
+    uint32_t max_qscale;
+    uint32_t n_skipped;
+    uint32_t overrun;
+} prr_device_status;
+
+typedef struct prr_context prr_context;
+
+/* ---- context ---------------------------------------------------------- */
+int prr_context_create(int device, prr_context **out, prr_error *err);
+void prr_context_destroy(prr_context *ctx);
+/* The context's HIP stream (hipStream_t). */
+void *prr_context_stream(prr_context *ctx);
+/* Version string of this library. */
+const char *prr_version(void);
+/* Decode launch shape (no counterpart in the crate, which has no launch to
+ * shape): lanes decoding streams per workgroup (16, 32 or 64) and the bytes
+ * of the packet staged in LDS per workgroup; 0 keeps the current value. */
+int prr_set_decode_config(prr_context *ctx, uint32_t streams_per_wg, uint32_t byte_cap,
+                          prr_error *err);
+
+/* ---- host-side frame structure (no device work) ----------------------- */
+/* header_len (decoder.rs:58-73). */
+int prr_header_len(const uint8_t *packet, size_t len, size_t *out, prr_error *err);
+/* parse_header (decoder.rs:45-53). */
+int prr_parse_header(const uint8_t *packet, size_t len, prr_header *out, prr_error *err);
+/* Prefix checks of parse_frame (decoder.rs:76-103) plus the grid sizes. */
+int prr_parse_frame_info(const uint8_t *packet, size_t len, prr_header *hdr, prr_frame_info *info,
+                         prr_error *err);
+/* quantizer / Frame::quantizer (decoder.rs:33-35, 150-152). */
+uint32_t prr_quantizer(const prr_header *hdr, uint8_t max_qscale);
+/* DemosaicParams::new / with_white_balance (demosaic.rs:80-106). */
+void prr_demosaic_params_new(const prr_header *hdr, uint32_t quantizer,
+                             prr_demosaic_params *out);
+void prr_demosaic_params_with_white_balance(const prr_header *hdr, uint32_t quantizer, float red,
+                                            float blue, prr_demosaic_params *out);
+/* output_matrix (demosaic.rs:138-153). */
+void prr_output_matrix(const prr_header *hdr, int exposure, float m[9]);
+
+/* ---- host-buffer entry points (drop-in; synchronous) ------------------ */
+/* decode_frame / decode_frame_with (decoder.rs:221-267): `samples` holds
+ * coded_width*coded_height code values; skipped tile indices ascend. */
+int prr_decode_frame(prr_context *ctx, const uint8_t *packet, size_t len, int idct,
+                     prr_header *hdr, uint16_t *samples, size_t samples_len, uint32_t *skipped,
+                     size_t skipped_cap, size_t *n_skipped, uint8_t *max_qscale,
+                     prr_error *err);
+/* Frame::linearize_into (decoder.rs:197-216): `samples` at stride
+ * coded_width, `out` width*height linear values. */
+int prr_linearize_into(prr_context *ctx, const prr_header *hdr, const uint16_t *samples,
+                       size_t samples_len, uint16_t *out, size_t out_len, prr_error *err);
+/* decode_frame_reduced / decode_frame_reduced_with (reduced.rs:142-201):
+ * `rgb` holds 3*(coded_width>>scale)*(coded_height>>scale) values. */
+int prr_decode_frame_reduced(prr_context *ctx, const uint8_t *packet, size_t len, int scale,
+                             int greens, prr_header *hdr, uint16_t *rgb, size_t rgb_len,
+                             uint32_t *skipped, size_t skipped_cap, size_t *n_skipped,
+                             uint8_t *max_qscale, prr_error *err);
+/* develop_planar_into (demosaic.rs:250-261): planes of width*height. */
+int prr_develop_planar_into(prr_context *ctx, const prr_header *hdr, const uint16_t *samples,
+                            size_t samples_len, const prr_demosaic_params *params, int stage,
+                            float *r, float *g, float *b, size_t plane_len, prr_error *err);
+/* develop_into (demosaic.rs:223-227): interleaved RGB, 3*width*height. */
+int prr_develop_into(prr_context *ctx, const prr_header *hdr, const uint16_t *samples,
+                     size_t samples_len, const prr_demosaic_params *params, int stage,
+                     float *rgb, size_t rgb_len, prr_error *err);
+
+/* ---- device-resident twins (enqueue only) ------------------------------ */
+/* Decode a packet already in device memory. `hdr` and `info` come from
+ * prr_parse_frame_info on the packet's first bytes. `d_packet` must be 4-byte
+ * aligned and readable for ((len + 3) & ~3) + 8 bytes. `d_samples` holds
+ * coded_width*coded_height values; `d_linear`, when not NULL, receives the
+ * linearized visible picture (width*height) in the same pass;
+ * `d_tile_flags`, when not NULL, gets one byte per tile (1 = skipped);
+ * `d_status` receives the frame's prr_device_status. */
+int prr_decode_frame_device(prr_context *ctx, const prr_header *hdr, const prr_frame_info *info,
+                            const uint8_t *d_packet, size_t len, int idct, uint16_t *d_samples,
+                            uint16_t *d_linear, uint8_t *d_tile_flags,
+                            prr_device_status *d_status, void *stream, prr_error *err);
+/* Frame::linearize_into on device buffers. */
+int prr_linearize_device(prr_context *ctx, const prr_header *hdr, const uint16_t *d_samples,
+                         uint16_t *d_out, void *stream, prr_error *err);
+/* Reduced decode of a packet in device memory (same packet rules). */
+int prr_decode_frame_reduced_device(prr_context *ctx, const prr_header *hdr,
+                                    const prr_frame_info *info, const uint8_t *d_packet,
+                                    size_t len, int scale, int greens, uint16_t *d_rgb,
+                                    uint8_t *d_tile_flags, prr_device_status *d_status,
+                                    void *stream, prr_error *err);
+/* develop_planar_into on device buffers; d_g == d_b == NULL selects the
+ * interleaved layout of develop_into with 3*width*height values in d_r. */
+int prr_develop_device(prr_context *ctx, const prr_header *hdr, const uint16_t *d_samples,
+                       const prr_demosaic_params *params, int stage, float *d_r, float *d_g,
+                       float *d_b, void *stream, prr_error *err);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,135 @@
+"""ctypes declarations of the C ABI in include/proresraw_amd.h.
+
+Loading fails loudly when the native library has not been built: there is
+no fallback implementation behind this module.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_DIR = os.path.join(_HERE, "_lib")
+LIB_PATH = os.path.join(LIB_DIR, "libproresraw_amd.so")
+SYNTH_PATH = os.path.join(LIB_DIR, "libprr_synth.so")
+
+c_u8p = ctypes.POINTER(ctypes.c_uint8)
+c_vp = ctypes.c_void_p
+
+
+class PrrError(ctypes.Structure):
+    _fields_ = [
+        ("code", ctypes.c_int32),
+        ("what", ctypes.c_char * 28),
+        ("a", ctypes.c_uint64),
+        ("b", ctypes.c_uint64),
+        ("points", ctypes.c_uint16 * 8),
+        ("tag", ctypes.c_uint8 * 4),
+    ]
+
+
+class PrrHeader(ctypes.Structure):
+    _fields_ = [
+        ("version", ctypes.c_uint8),
+        ("vendor", ctypes.c_uint8 * 4),
+        ("bayer", ctypes.c_uint8),
+        ("width", ctypes.c_uint16),
+        ("height", ctypes.c_uint16),
+        ("crop", ctypes.c_uint8 * 4),
+        ("white_level", ctypes.c_uint16),
+        ("wb_cct", ctypes.c_uint16),
+        ("flags", ctypes.c_uint16),
+        ("alignment", ctypes.c_uint8),
+        ("quant_matrix", ctypes.c_uint8 * 64),
+        ("curve", ctypes.c_uint16 * 8),
+        ("wb_red", ctypes.c_float),
+        ("wb_blue", ctypes.c_float),
+        ("color_matrix", ctypes.c_float * 9),
+        ("gain", ctypes.c_float),
+    ]
+
+
+class PrrFrameInfo(ctypes.Structure):
+    _fields_ = [
+        ("coded_width", ctypes.c_uint32),
+        ("coded_height", ctypes.c_uint32),
+        ("tiles_per_row", ctypes.c_uint32),
+        ("tile_rows", ctypes.c_uint32),
+        ("nb_tiles", ctypes.c_uint32),
+        ("header_end", ctypes.c_size_t),
+    ]
+
+
+class PrrDemosaicParams(ctypes.Structure):
+    _fields_ = [
+        ("cfa_phase", ctypes.c_int32),
+        ("black", ctypes.c_int32),
+        ("one_over_range", ctypes.c_float),
+        ("noise", ctypes.c_float * 3),
+        ("clip", ctypes.c_float * 2),
+        ("black_over_range", ctypes.c_float),
+        ("wb", ctypes.c_float * 2),
+    ]
+
+
+class PrrDeviceStatus(ctypes.Structure):
+    _fields_ = [
+        ("oob_tile", ctypes.c_uint32),
+        ("max_qscale", ctypes.c_uint32),
+        ("n_skipped", ctypes.c_uint32),
+        ("overrun", ctypes.c_uint32),
+    ]
+
+
+E = ctypes.POINTER(PrrError)
+H = ctypes.POINTER(PrrHeader)
+FI = ctypes.POINTER(PrrFrameInfo)
+DP = ctypes.POINTER(PrrDemosaicParams)
+sz = ctypes.c_size_t
+i32 = ctypes.c_int
+u32 = ctypes.c_uint32
+
+# name -> (restype, argtypes); every symbol include/proresraw_amd.h declares.
+SIGNATURES = {
+    "prr_context_create": (i32, [i32, ctypes.POINTER(c_vp), E]),
+    "prr_context_destroy": (None, [c_vp]),
+    "prr_context_stream": (c_vp, [c_vp]),
+    "prr_version": (ctypes.c_char_p, []),
+    "prr_set_decode_config": (i32, [c_vp, u32, u32, E]),
+    "prr_header_len": (i32, [c_vp, sz, ctypes.POINTER(sz), E]),
+    "prr_parse_header": (i32, [c_vp, sz, H, E]),
+    "prr_parse_frame_info": (i32, [c_vp, sz, H, FI, E]),
+    "prr_quantizer": (u32, [H, ctypes.c_uint8]),
+    "prr_demosaic_params_new": (None, [H, u32, DP]),
+    "prr_demosaic_params_with_white_balance": (None, [H, u32, ctypes.c_float, ctypes.c_float, DP]),
+    "prr_output_matrix": (None, [H, i32, c_vp]),
+    "prr_decode_frame": (i32, [c_vp, c_vp, sz, i32, H, c_vp, sz, c_vp, sz,
+                               ctypes.POINTER(sz), c_u8p, E]),
+    "prr_linearize_into": (i32, [c_vp, H, c_vp, sz, c_vp, sz, E]),
+    "prr_decode_frame_reduced": (i32, [c_vp, c_vp, sz, i32, i32, H, c_vp, sz, c_vp, sz,
+                                       ctypes.POINTER(sz), c_u8p, E]),
+    "prr_develop_planar_into": (i32, [c_vp, H, c_vp, sz, DP, i32, c_vp, c_vp, c_vp, sz, E]),
+    "prr_develop_into": (i32, [c_vp, H, c_vp, sz, DP, i32, c_vp, sz, E]),
+    "prr_decode_frame_device": (i32, [c_vp, H, FI, c_vp, sz, i32, c_vp, c_vp, c_vp, c_vp, c_vp, E]),
+    "prr_linearize_device": (i32, [c_vp, H, c_vp, c_vp, c_vp, E]),
+    "prr_decode_frame_reduced_device": (i32, [c_vp, H, FI, c_vp, sz, i32, i32, c_vp, c_vp, c_vp,
+                                              c_vp, E]),
+    "prr_develop_device": (i32, [c_vp, H, c_vp, DP, i32, c_vp, c_vp, c_vp, c_vp, E]),
+}
+
+_lib = None
+
+
+def lib():
+    """The loaded C-ABI library (raises OSError when it is not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise OSError(
+                f"{LIB_PATH} is missing: build it with `python -m proresraw_rs_amd.build` "
+                "(hipcc for gfx950); there is no CPU fallback")
+        handle = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(handle, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = handle
+    return _lib

@@ -1,0 +1,705 @@
+/* Host layer of proresraw_amd: frame structure parsing, the device context
+ * and the C-ABI entry points declared in include/proresraw_amd.h.
+ *
+ * Parsing follows the crate's host-side rules (decoder.rs:45-123,
+ * header.rs:156-246, tile.rs:36-61); everything per tile or per sensel runs
+ * in the HIP kernels of prr_kernels.hip. No path here computes a decode or
+ * develop on the CPU: without a usable HIP device the entry points fail
+ * with PRR_E_DEVICE.
+ */
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "prr_internal.h"
+
+using namespace prr;
+
+/* ------------------------------------------------------------------ */
+/* Errors. */
+
+static int fail(prr_error *err, int code, const char *what = nullptr, uint64_t a = 0,
+                uint64_t b = 0)
+{
+    if (err) {
+        std::memset(err, 0, sizeof *err);
+        err->code = code;
+        if (what)
+            std::strncpy(err->what, what, sizeof err->what - 1);
+        err->a = a;
+        err->b = b;
+    }
+    return code;
+}
+
+static int device_fail(prr_error *err, hipError_t e, const char *what)
+{
+    return fail(err, PRR_E_DEVICE, what, (uint64_t)e, 0);
+}
+
+#define HIP_TRY(call, what)                                                                        \
+    do {                                                                                           \
+        hipError_t e_ = (call);                                                                    \
+        if (e_ != hipSuccess)                                                                      \
+            return device_fail(err, e_, what);                                                     \
+    } while (0)
+
+/* ------------------------------------------------------------------ */
+/* Frame structure. */
+
+namespace {
+
+uint16_t rd16(const uint8_t *p) { return (uint16_t)((p[0] << 8) | p[1]); }
+uint32_t rd32(const uint8_t *p)
+{
+    return ((uint32_t)p[0] << 24) | ((uint32_t)p[1] << 16) | ((uint32_t)p[2] << 8) | p[3];
+}
+float rdf32(const uint8_t *p)
+{
+    uint32_t u = rd32(p);
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+
+constexpr size_t kPrefix = 10;    /* decoder.rs:13 */
+constexpr size_t kFixedBody = 70; /* header.rs:11 */
+constexpr uint16_t kDefaultCurve[8] = {0, 512, 1024, 2048, 4096, 8192, 16384, 32768};
+
+/* FrameHeader::parse (header.rs:156-246). */
+int parse_body(const uint8_t *b, size_t n, prr_header *h, prr_error *err)
+{
+    if (n < kFixedBody)
+        return fail(err, PRR_E_TRUNCATED, "frame header", kFixedBody, n);
+    std::memset(h, 0, sizeof *h);
+    h->version = b[1];
+    if (h->version > 1)
+        return fail(err, PRR_E_UNSUPPORTED_VERSION, nullptr, h->version);
+    std::memcpy(h->vendor, b + 2, 4);
+    h->width = rd16(b + 6);
+    h->height = rd16(b + 8);
+    if (!h->width || !h->height || (h->width & 1) || (h->height & 1))
+        return fail(err, PRR_E_BAD_DIMENSIONS, nullptr, h->width, h->height);
+    std::memcpy(h->crop, b + 10, 4);
+    h->bayer = (uint8_t)(rd16(b + 14) & 3);
+    h->white_level = (uint16_t)(rd16(b + 16) + 256u);
+    h->wb_red = rdf32(b + 18);
+    h->wb_blue = rdf32(b + 22);
+    for (int i = 0; i < 9; i++)
+        h->color_matrix[i] = rdf32(b + 26 + 4 * i);
+    h->gain = rdf32(b + 62);
+    h->wb_cct = rd16(b + 66);
+    h->flags = rd16(b + 68);
+    h->alignment = (uint8_t)((h->flags >> 1) & 7);
+    if (h->alignment > 4)
+        return fail(err, PRR_E_BAD_ALIGNMENT, nullptr, h->alignment);
+    size_t at = kFixedBody;
+    std::memset(h->quant_matrix, 1, sizeof h->quant_matrix);
+    if (h->flags & 1) {
+        if (at + 64 > n)
+            return fail(err, PRR_E_TRUNCATED, "quantization matrix", at + 64, n);
+        std::memcpy(h->quant_matrix, b + at, 64);
+        at += 64;
+        for (int i = 0; i < 64; i++)
+            if (h->quant_matrix[i] < 1 || h->quant_matrix[i] > 63)
+                return fail(err, PRR_E_BAD_QUANT_MATRIX, nullptr, (uint64_t)i, h->quant_matrix[i]);
+    }
+    if ((h->flags >> 4) & 1) {
+        if (at + 16 > n)
+            return fail(err, PRR_E_TRUNCATED, "linearization curve", at + 16, n);
+        uint16_t pts[8];
+        for (int i = 0; i < 8; i++)
+            pts[i] = rd16(b + at + 2 * i);
+        if (h->version == 0)
+            return fail(err, PRR_E_UNEXPECTED_CURVE);
+        for (int i = 0; i + 1 < 8; i++)
+            if (pts[i] >= pts[i + 1]) {
+                fail(err, PRR_E_BAD_CURVE);
+                if (err)
+                    std::memcpy(err->points, pts, sizeof pts);
+                return PRR_E_BAD_CURVE;
+            }
+        std::memcpy(h->curve, pts, sizeof pts);
+    } else {
+        std::memcpy(h->curve, kDefaultCurve, sizeof kDefaultCurve);
+    }
+    return PRR_OK;
+}
+
+uint32_t tiles_per_row_of(uint32_t width, uint32_t alignment)
+{
+    uint32_t columns = (width + 15) / 16;
+    return (columns >> alignment) + (uint32_t)__builtin_popcount(columns & ((1u << alignment) - 1));
+}
+
+} // namespace
+
+extern "C" int prr_header_len(const uint8_t *packet, size_t len, size_t *out, prr_error *err)
+{
+    if (!packet && len)
+        return fail(err, PRR_E_INVALID_ARGUMENT, "packet");
+    if (len < kPrefix)
+        return fail(err, PRR_E_TRUNCATED, "frame prefix", kPrefix, len);
+    if (std::memcmp(packet + 4, "prrf", 4) != 0) {
+        fail(err, PRR_E_BAD_TAG);
+        if (err)
+            std::memcpy(err->tag, packet + 4, 4);
+        return PRR_E_BAD_TAG;
+    }
+    uint16_t hs = rd16(packet + 8);
+    if (hs < 72)
+        return fail(err, PRR_E_HEADER_TOO_SHORT, nullptr, hs);
+    *out = kPrefix + hs - 2;
+    return PRR_OK;
+}
+
+extern "C" int prr_parse_header(const uint8_t *packet, size_t len, prr_header *out,
+                                prr_error *err)
+{
+    size_t end;
+    int rc = prr_header_len(packet, len, &end, err);
+    if (rc)
+        return rc;
+    if (end > len)
+        return fail(err, PRR_E_TRUNCATED, "frame header", end, len);
+    return parse_body(packet + kPrefix, end - kPrefix, out, err);
+}
+
+extern "C" int prr_parse_frame_info(const uint8_t *packet, size_t len, prr_header *hdr,
+                                    prr_frame_info *info, prr_error *err)
+{
+    if (!packet && len)
+        return fail(err, PRR_E_INVALID_ARGUMENT, "packet");
+    if (len < kPrefix)
+        return fail(err, PRR_E_TRUNCATED, "frame prefix", kPrefix, len);
+    uint32_t declared = rd32(packet);
+    if (declared == 0 || (size_t)declared > len)
+        return fail(err, PRR_E_FRAME_SIZE_MISMATCH, nullptr, declared, len);
+    size_t end;
+    int rc = prr_header_len(packet, len, &end, err);
+    if (rc)
+        return rc;
+    rc = prr_parse_header(packet, len, hdr, err);
+    if (rc)
+        return rc;
+    info->coded_width = ((uint32_t)hdr->width + 15) & ~15u;
+    info->coded_height = ((uint32_t)hdr->height + 15) & ~15u;
+    info->tiles_per_row = tiles_per_row_of(hdr->width, hdr->alignment);
+    info->tile_rows = info->coded_height / 16;
+    info->nb_tiles = info->tiles_per_row * info->tile_rows;
+    info->header_end = end;
+    size_t table_end = end + 2 * (size_t)info->nb_tiles;
+    if (table_end > len)
+        return fail(err, PRR_E_TRUNCATED, "tile size table", table_end, len);
+    return PRR_OK;
+}
+
+extern "C" uint32_t prr_quantizer(const prr_header *hdr, uint8_t max_qscale)
+{
+    return (uint32_t)hdr->quant_matrix[0] * (uint32_t)max_qscale;
+}
+
+/* DemosaicParams::with_white_balance (demosaic.rs:86-106). */
+extern "C" void prr_demosaic_params_with_white_balance(const prr_header *hdr, uint32_t quantizer,
+                                                       float red, float blue,
+                                                       prr_demosaic_params *p)
+{
+    const uint32_t range = (uint16_t)(hdr->white_level - 256u);
+    const double q = (double)quantizer;
+    const double base = q * q * 5.0 + 64.0 / 3.0;
+    const double range2 = (double)range * (double)range;
+    const float rangef = (float)range;
+    p->cfa_phase = hdr->bayer;
+    p->black = 256;
+    p->one_over_range = 1.0f / rangef;
+    p->noise[0] = (float)(base / 150994944.0);
+    p->noise[1] = (float)(base / (range2 * 256.0));
+    p->noise[2] = (float)(64.0 / 3.0 / range2);
+    p->clip[0] = red == 0.0f ? 1.0f : 1.0f / red;
+    p->clip[1] = blue == 0.0f ? 1.0f : 1.0f / blue;
+    p->black_over_range = 256.0f / rangef;
+    p->wb[0] = red;
+    p->wb[1] = blue;
+}
+
+extern "C" void prr_demosaic_params_new(const prr_header *hdr, uint32_t quantizer,
+                                        prr_demosaic_params *p)
+{
+    prr_demosaic_params_with_white_balance(hdr, quantizer, hdr->wb_red, hdr->wb_blue, p);
+}
+
+/* output_matrix (demosaic.rs:138-153) with XYZ_TO_REC2020 (demosaic.rs:37-41). */
+extern "C" void prr_output_matrix(const prr_header *hdr, int exposure, float m[9])
+{
+    static const double xyz_to_rec2020[3][3] = {
+        {1.716651187971268, -0.355670783776393, -0.25336628137366},
+        {-0.666684351832489, 1.616481236634939, 0.015768545813911},
+        {0.017639857445311, -0.042770613257809, 0.942103121235474}};
+    double gain = (double)hdr->gain;
+    if (exposure == PRR_EXPOSURE_NORMALIZED)
+        gain = gain * 10.0 / 9.0;
+    for (int i = 0; i < 9; i++) {
+        const int row = i / 3, col = i % 3;
+        const double v = xyz_to_rec2020[row][0] * (double)hdr->color_matrix[col] +
+                         xyz_to_rec2020[row][1] * (double)hdr->color_matrix[3 + col] +
+                         xyz_to_rec2020[row][2] * (double)hdr->color_matrix[6 + col];
+        m[i] = (float)(v * gain);
+    }
+}
+
+extern "C" const char *prr_version(void) { return "proresraw_amd 0.1.0 (gfx950)"; }
+
+/* ------------------------------------------------------------------ */
+/* Context: a stream and grow-only device scratch. */
+
+struct DevBuf {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+};
+
+struct prr_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    DecodeLaunch launch{32, 16384, 65536};
+    DevBuf packet, offsets, status, flags, samples, out0, out1, out2;
+    std::vector<uint8_t> host_flags;
+};
+
+static int ensure(prr_context *ctx, DevBuf &b, size_t bytes, prr_error *err)
+{
+    if (b.bytes >= bytes && b.ptr)
+        return PRR_OK;
+    if (b.ptr) {
+        (void)hipStreamSynchronize(ctx->stream);
+        (void)hipFree(b.ptr);
+        b.ptr = nullptr;
+        b.bytes = 0;
+    }
+    size_t want = std::max<size_t>(bytes, 256);
+    HIP_TRY(hipMalloc(&b.ptr, want), "hipMalloc");
+    b.bytes = want;
+    return PRR_OK;
+}
+
+extern "C" int prr_context_create(int device, prr_context **out, prr_error *err)
+{
+    if (!out)
+        return fail(err, PRR_E_INVALID_ARGUMENT, "out");
+    *out = nullptr;
+    int count = 0;
+    HIP_TRY(hipGetDeviceCount(&count), "hipGetDeviceCount");
+    if (device < 0 || device >= count)
+        return fail(err, PRR_E_DEVICE, "no such device", (uint64_t)device, (uint64_t)count);
+    HIP_TRY(hipSetDevice(device), "hipSetDevice");
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
+    prr_context *ctx = new prr_context();
+    ctx->device = device;
+    hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete ctx;
+        return device_fail(err, e, "hipStreamCreate");
+    }
+    ctx->launch.lds_limit = (uint32_t)std::min<size_t>(prop.sharedMemPerBlock, 65536);
+    if (const char *v = std::getenv("PRR_STREAMS_PER_WG"))
+        ctx->launch.streams_per_wg = (uint32_t)std::atoi(v);
+    if (const char *v = std::getenv("PRR_BYTE_CAP"))
+        ctx->launch.byte_cap = (uint32_t)std::atoi(v);
+    *out = ctx;
+    return PRR_OK;
+}
+
+extern "C" void prr_context_destroy(prr_context *ctx)
+{
+    if (!ctx)
+        return;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    for (DevBuf *b : {&ctx->packet, &ctx->offsets, &ctx->status, &ctx->flags, &ctx->samples,
+                      &ctx->out0, &ctx->out1, &ctx->out2})
+        if (b->ptr)
+            (void)hipFree(b->ptr);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+extern "C" void *prr_context_stream(prr_context *ctx) { return ctx ? ctx->stream : nullptr; }
+
+/* Decode launch parameters: streams per workgroup (16, 32 or 64) and the
+ * LDS bytes staged per workgroup; 0 keeps the current value. */
+extern "C" int prr_set_decode_config(prr_context *ctx, uint32_t streams_per_wg,
+                                     uint32_t byte_cap, prr_error *err)
+{
+    if (!ctx)
+        return fail(err, PRR_E_INVALID_ARGUMENT, "ctx");
+    if (streams_per_wg) {
+        if (streams_per_wg != 16 && streams_per_wg != 32 && streams_per_wg != 64)
+            return fail(err, PRR_E_INVALID_ARGUMENT, "streams_per_wg", streams_per_wg);
+        ctx->launch.streams_per_wg = streams_per_wg;
+    }
+    if (byte_cap) {
+        if (byte_cap < 256 || (byte_cap & 15))
+            return fail(err, PRR_E_INVALID_ARGUMENT, "byte_cap", byte_cap);
+        ctx->launch.byte_cap = byte_cap;
+    }
+    return PRR_OK;
+}
+
+/* ------------------------------------------------------------------ */
+/* Device-resident entry points. */
+
+static FrameDesc frame_desc(const prr_header *hdr, const prr_frame_info *info,
+                            const uint8_t *d_packet, size_t len, uint64_t *d_offsets)
+{
+    FrameDesc f;
+    std::memset(&f, 0, sizeof f);
+    f.packet = d_packet;
+    f.len = len;
+    f.offsets = d_offsets;
+    f.table_off = (uint32_t)info->header_end;
+    f.nb_tiles = info->nb_tiles;
+    f.tiles_per_row = info->tiles_per_row;
+    f.columns = ((uint32_t)hdr->width + 15) / 16;
+    f.alignment = hdr->alignment;
+    f.coded_width = info->coded_width;
+    f.coded_height = info->coded_height;
+    f.width = hdr->width;
+    f.height = hdr->height;
+    f.bayer = hdr->bayer;
+    std::memcpy(f.qmat, hdr->quant_matrix, 64);
+    std::memcpy(f.curve, hdr->curve, 16);
+    return f;
+}
+
+static int check_device_args(prr_context *ctx, const prr_header *hdr, const prr_frame_info *info,
+                             const uint8_t *d_packet, prr_device_status *d_status,
+                             prr_error *err)
+{
+    if (!ctx || !hdr || !info || !d_packet || !d_status)
+        return fail(err, PRR_E_INVALID_ARGUMENT, "null argument");
+    if ((uintptr_t)d_packet & 3)
+        return fail(err, PRR_E_INVALID_ARGUMENT, "packet alignment");
+    return PRR_OK;
+}
+
+extern "C" int prr_decode_frame_device(prr_context *ctx, const prr_header *hdr,
+                                       const prr_frame_info *info, const uint8_t *d_packet,
+                                       size_t len, int idct, uint16_t *d_samples,
+                                       uint16_t *d_linear, uint8_t *d_tile_flags,
+                                       prr_device_status *d_status, void *stream,
+                                       prr_error *err)
+{
+    int rc = check_device_args(ctx, hdr, info, d_packet, d_status, err);
+    if (rc)
+        return rc;
+    if (!d_samples || (idct != PRR_IDCT_APPLE && idct != PRR_IDCT_FFMPEG))
+        return fail(err, PRR_E_INVALID_ARGUMENT, "samples or idct");
+    hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
+    HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+    rc = ensure(ctx, ctx->offsets, sizeof(uint64_t) * ((size_t)info->nb_tiles + 1), err);
+    if (rc)
+        return rc;
+    FrameDesc f = frame_desc(hdr, info, d_packet, len, (uint64_t *)ctx->offsets.ptr);
+    HIP_TRY(launch_status_init(d_status, s), "status kernel");
+    HIP_TRY(launch_scan(f, d_status, s), "scan kernel");
+    HIP_TRY(launch_decode(f, idct, d_samples, d_linear, d_tile_flags, d_status, ctx->launch, s),
+            "decode kernel");
+    return PRR_OK;
+}
+
+extern "C" int prr_decode_frame_reduced_device(prr_context *ctx, const prr_header *hdr,
+                                               const prr_frame_info *info,
+                                               const uint8_t *d_packet, size_t len, int scale,
+                                               int greens, uint16_t *d_rgb,
+                                               uint8_t *d_tile_flags,
+                                               prr_device_status *d_status, void *stream,
+                                               prr_error *err)
+{
+    int rc = check_device_args(ctx, hdr, info, d_packet, d_status, err);
+    if (rc)
+        return rc;
+    if (!d_rgb || scale < PRR_SCALE_HALF || scale > PRR_SCALE_EIGHTH ||
+        (greens != PRR_GREENS_FIRST && greens != PRR_GREENS_AVERAGE))
+        return fail(err, PRR_E_INVALID_ARGUMENT, "rgb, scale or greens");
+    hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
+    HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+    rc = ensure(ctx, ctx->offsets, sizeof(uint64_t) * ((size_t)info->nb_tiles + 1), err);
+    if (rc)
+        return rc;
+    FrameDesc f = frame_desc(hdr, info, d_packet, len, (uint64_t *)ctx->offsets.ptr);
+    HIP_TRY(launch_status_init(d_status, s), "status kernel");
+    HIP_TRY(launch_scan(f, d_status, s), "scan kernel");
+    HIP_TRY(launch_decode_reduced(f, scale, greens, d_rgb, d_tile_flags, d_status, ctx->launch, s),
+            "reduced kernel");
+    return PRR_OK;
+}
+
+extern "C" int prr_linearize_device(prr_context *ctx, const prr_header *hdr,
+                                    const uint16_t *d_samples, uint16_t *d_out, void *stream,
+                                    prr_error *err)
+{
+    if (!ctx || !hdr || !d_samples || !d_out)
+        return fail(err, PRR_E_INVALID_ARGUMENT, "null argument");
+    hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
+    HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+    uint32_t stride = ((uint32_t)hdr->width + 15) & ~15u;
+    HIP_TRY(launch_linearize(d_samples, stride, hdr->width, hdr->height, hdr->curve, d_out, s),
+            "linearize kernel");
+    return PRR_OK;
+}
+
+/* Conversion::new (demosaic.rs:286-304): the stage folded into the params. */
+static DevelopDesc develop_desc(const prr_header *hdr, const uint16_t *d_samples,
+                                const prr_demosaic_params *p, int stage)
+{
+    DevelopDesc d;
+    std::memset(&d, 0, sizeof d);
+    d.samples = d_samples;
+    d.stride = ((uint32_t)hdr->width + 15) & ~15u;
+    d.width = hdr->width;
+    d.height = hdr->height;
+    d.cfa_phase = p->cfa_phase;
+    d.black = p->black;
+    d.one_over_range = p->one_over_range;
+    for (int i = 0; i < 3; i++)
+        d.noise[i] = p->noise[i];
+    d.clip[0] = p->clip[0];
+    d.clip[1] = p->clip[1];
+    d.black_over_range = p->black_over_range;
+    d.wb[0] = p->wb[0];
+    d.wb[1] = p->wb[1];
+    if (stage == PRR_STAGE_DEMOSAICED) {
+        d.wb[0] = d.wb[1] = 1.0f;
+        d.clip[0] = d.clip[1] = 1.0f;
+    }
+    if (stage == PRR_STAGE_REC2020) {
+        d.has_matrix = 1;
+        prr_output_matrix(hdr, PRR_EXPOSURE_NORMALIZED, d.m);
+    }
+    std::memcpy(d.curve, hdr->curve, 16);
+    return d;
+}
+
+extern "C" int prr_develop_device(prr_context *ctx, const prr_header *hdr,
+                                  const uint16_t *d_samples, const prr_demosaic_params *params,
+                                  int stage, float *d_r, float *d_g, float *d_b, void *stream,
+                                  prr_error *err)
+{
+    if (!ctx || !hdr || !d_samples || !params || !d_r)
+        return fail(err, PRR_E_INVALID_ARGUMENT, "null argument");
+    if ((d_g == nullptr) != (d_b == nullptr))
+        return fail(err, PRR_E_INVALID_ARGUMENT, "planes");
+    if (stage < PRR_STAGE_DEMOSAICED || stage > PRR_STAGE_REC2020)
+        return fail(err, PRR_E_INVALID_ARGUMENT, "stage", (uint64_t)stage);
+    if (((uintptr_t)d_r | (uintptr_t)d_g | (uintptr_t)d_b) & 7)
+        return fail(err, PRR_E_INVALID_ARGUMENT, "plane alignment");
+    hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
+    HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+    DevelopDesc d = develop_desc(hdr, d_samples, params, stage);
+    HIP_TRY(launch_develop(d, d_r, d_g, d_b, s), "develop kernel");
+    return PRR_OK;
+}
+
+/* ------------------------------------------------------------------ */
+/* Host-buffer entry points: copy in, run the device path, copy out. */
+
+static int upload_packet(prr_context *ctx, const uint8_t *packet, size_t len, prr_error *err)
+{
+    size_t padded = ((len + 3) & ~(size_t)3) + 16;
+    int rc = ensure(ctx, ctx->packet, padded, err);
+    if (rc)
+        return rc;
+    HIP_TRY(hipMemcpyAsync(ctx->packet.ptr, packet, len, hipMemcpyHostToDevice, ctx->stream),
+            "hipMemcpy packet");
+    return PRR_OK;
+}
+
+/* Turn the device status into the crate's errors and the skipped list. */
+static int finish_frame(prr_context *ctx, const prr_frame_info *info, uint32_t *skipped,
+                        size_t skipped_cap, size_t *n_skipped, uint8_t *max_qscale,
+                        prr_error *err)
+{
+    prr_device_status st;
+    ctx->host_flags.resize(info->nb_tiles);
+    HIP_TRY(hipMemcpyAsync(&st, ctx->status.ptr, sizeof st, hipMemcpyDeviceToHost, ctx->stream),
+            "hipMemcpy status");
+    if (info->nb_tiles)
+        HIP_TRY(hipMemcpyAsync(ctx->host_flags.data(), ctx->flags.ptr, info->nb_tiles,
+                               hipMemcpyDeviceToHost, ctx->stream),
+                "hipMemcpy flags");
+    HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    if (st.oob_tile != 0xFFFFFFFFu)
+        return fail(err, PRR_E_TILE_OUT_OF_BOUNDS, nullptr, st.oob_tile);
+    if (st.overrun)
+        return fail(err, PRR_E_DECODE_OVERRUN, "symbol bound");
+    size_t k = 0;
+    for (uint32_t t = 0; t < info->nb_tiles; t++)
+        if (ctx->host_flags[t]) {
+            if (k >= skipped_cap)
+                return fail(err, PRR_E_OUTPUT_SIZE, "skipped capacity", k + 1, skipped_cap);
+            skipped[k++] = t;
+        }
+    if (n_skipped)
+        *n_skipped = k;
+    if (max_qscale)
+        *max_qscale = (uint8_t)st.max_qscale;
+    return PRR_OK;
+}
+
+extern "C" int prr_decode_frame(prr_context *ctx, const uint8_t *packet, size_t len, int idct,
+                                prr_header *hdr, uint16_t *samples, size_t samples_len,
+                                uint32_t *skipped, size_t skipped_cap, size_t *n_skipped,
+                                uint8_t *max_qscale, prr_error *err)
+{
+    if (!ctx || !hdr || (!samples && samples_len))
+        return fail(err, PRR_E_INVALID_ARGUMENT, "null argument");
+    prr_frame_info info;
+    int rc = prr_parse_frame_info(packet, len, hdr, &info, err);
+    if (rc)
+        return rc;
+    size_t n = (size_t)info.coded_width * info.coded_height;
+    if (samples_len != n)
+        return fail(err, PRR_E_OUTPUT_SIZE, "output size", n, samples_len);
+    HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+    if ((rc = upload_packet(ctx, packet, len, err)) ||
+        (rc = ensure(ctx, ctx->status, sizeof(prr_device_status), err)) ||
+        (rc = ensure(ctx, ctx->flags, info.nb_tiles + 1, err)) ||
+        (rc = ensure(ctx, ctx->samples, n * 2 + 16, err)))
+        return rc;
+    rc = prr_decode_frame_device(ctx, hdr, &info, (const uint8_t *)ctx->packet.ptr, len, idct,
+                                 (uint16_t *)ctx->samples.ptr, nullptr,
+                                 (uint8_t *)ctx->flags.ptr,
+                                 (prr_device_status *)ctx->status.ptr, nullptr, err);
+    if (rc)
+        return rc;
+    rc = finish_frame(ctx, &info, skipped, skipped_cap, n_skipped, max_qscale, err);
+    if (rc)
+        return rc;
+    HIP_TRY(hipMemcpy(samples, ctx->samples.ptr, n * 2, hipMemcpyDeviceToHost), "hipMemcpy samples");
+    return PRR_OK;
+}
+
+extern "C" int prr_decode_frame_reduced(prr_context *ctx, const uint8_t *packet, size_t len,
+                                        int scale, int greens, prr_header *hdr, uint16_t *rgb,
+                                        size_t rgb_len, uint32_t *skipped, size_t skipped_cap,
+                                        size_t *n_skipped, uint8_t *max_qscale, prr_error *err)
+{
+    if (!ctx || !hdr || (!rgb && rgb_len))
+        return fail(err, PRR_E_INVALID_ARGUMENT, "null argument");
+    if (scale < PRR_SCALE_HALF || scale > PRR_SCALE_EIGHTH)
+        return fail(err, PRR_E_INVALID_ARGUMENT, "scale", (uint64_t)scale);
+    prr_frame_info info;
+    int rc = prr_parse_frame_info(packet, len, hdr, &info, err);
+    if (rc)
+        return rc;
+    size_t n = 3 * (size_t)(info.coded_width >> scale) * (info.coded_height >> scale);
+    if (rgb_len != n)
+        return fail(err, PRR_E_OUTPUT_SIZE, "output size", n, rgb_len);
+    HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+    if ((rc = upload_packet(ctx, packet, len, err)) ||
+        (rc = ensure(ctx, ctx->status, sizeof(prr_device_status), err)) ||
+        (rc = ensure(ctx, ctx->flags, info.nb_tiles + 1, err)) ||
+        (rc = ensure(ctx, ctx->out0, n * 2 + 16, err)))
+        return rc;
+    rc = prr_decode_frame_reduced_device(ctx, hdr, &info, (const uint8_t *)ctx->packet.ptr, len,
+                                         scale, greens, (uint16_t *)ctx->out0.ptr,
+                                         (uint8_t *)ctx->flags.ptr,
+                                         (prr_device_status *)ctx->status.ptr, nullptr, err);
+    if (rc)
+        return rc;
+    rc = finish_frame(ctx, &info, skipped, skipped_cap, n_skipped, max_qscale, err);
+    if (rc)
+        return rc;
+    HIP_TRY(hipMemcpy(rgb, ctx->out0.ptr, n * 2, hipMemcpyDeviceToHost), "hipMemcpy rgb");
+    return PRR_OK;
+}
+
+extern "C" int prr_linearize_into(prr_context *ctx, const prr_header *hdr,
+                                  const uint16_t *samples, size_t samples_len, uint16_t *out,
+                                  size_t out_len, prr_error *err)
+{
+    if (!ctx || !hdr || !samples || (!out && out_len))
+        return fail(err, PRR_E_INVALID_ARGUMENT, "null argument");
+    size_t cw = ((size_t)hdr->width + 15) & ~(size_t)15, ch = ((size_t)hdr->height + 15) & ~(size_t)15;
+    if (samples_len != cw * ch)
+        return fail(err, PRR_E_INVALID_ARGUMENT, "samples size", cw * ch, samples_len);
+    size_t n = (size_t)hdr->width * hdr->height;
+    if (out_len != n)
+        return fail(err, PRR_E_OUTPUT_SIZE, "output size", n, out_len);
+    HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+    int rc;
+    if ((rc = ensure(ctx, ctx->samples, cw * ch * 2 + 16, err)) ||
+        (rc = ensure(ctx, ctx->out0, n * 2 + 16, err)))
+        return rc;
+    HIP_TRY(hipMemcpyAsync(ctx->samples.ptr, samples, cw * ch * 2, hipMemcpyHostToDevice,
+                           ctx->stream),
+            "hipMemcpy samples");
+    rc = prr_linearize_device(ctx, hdr, (const uint16_t *)ctx->samples.ptr,
+                              (uint16_t *)ctx->out0.ptr, nullptr, err);
+    if (rc)
+        return rc;
+    HIP_TRY(hipMemcpyAsync(out, ctx->out0.ptr, n * 2, hipMemcpyDeviceToHost, ctx->stream),
+            "hipMemcpy linear");
+    HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    return PRR_OK;
+}
+
+static int develop_host(prr_context *ctx, const prr_header *hdr, const uint16_t *samples,
+                        size_t samples_len, const prr_demosaic_params *params, int stage,
+                        float *r, float *g, float *b, size_t out_len, int planar,
+                        prr_error *err)
+{
+    if (!ctx || !hdr || !samples || !params || !r || (planar && (!g || !b)))
+        return fail(err, PRR_E_INVALID_ARGUMENT, "null argument");
+    size_t cw = ((size_t)hdr->width + 15) & ~(size_t)15, ch = ((size_t)hdr->height + 15) & ~(size_t)15;
+    if (samples_len != cw * ch)
+        return fail(err, PRR_E_INVALID_ARGUMENT, "samples size", cw * ch, samples_len);
+    size_t n = (size_t)hdr->width * hdr->height;
+    if (out_len != (planar ? n : 3 * n))
+        return fail(err, PRR_E_OUTPUT_SIZE, planar ? "plane size" : "output size",
+                    planar ? n : 3 * n, out_len);
+    HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+    int rc;
+    if ((rc = ensure(ctx, ctx->samples, cw * ch * 2 + 16, err)) ||
+        (rc = ensure(ctx, ctx->out0, (planar ? n : 3 * n) * 4 + 16, err)) ||
+        (planar && (rc = ensure(ctx, ctx->out1, n * 4 + 16, err))) ||
+        (planar && (rc = ensure(ctx, ctx->out2, n * 4 + 16, err))))
+        return rc;
+    HIP_TRY(hipMemcpyAsync(ctx->samples.ptr, samples, cw * ch * 2, hipMemcpyHostToDevice,
+                           ctx->stream),
+            "hipMemcpy samples");
+    rc = prr_develop_device(ctx, hdr, (const uint16_t *)ctx->samples.ptr, params, stage,
+                            (float *)ctx->out0.ptr, planar ? (float *)ctx->out1.ptr : nullptr,
+                            planar ? (float *)ctx->out2.ptr : nullptr, nullptr, err);
+    if (rc)
+        return rc;
+    if (planar) {
+        HIP_TRY(hipMemcpyAsync(r, ctx->out0.ptr, n * 4, hipMemcpyDeviceToHost, ctx->stream), "hipMemcpy r");
+        HIP_TRY(hipMemcpyAsync(g, ctx->out1.ptr, n * 4, hipMemcpyDeviceToHost, ctx->stream), "hipMemcpy g");
+        HIP_TRY(hipMemcpyAsync(b, ctx->out2.ptr, n * 4, hipMemcpyDeviceToHost, ctx->stream), "hipMemcpy b");
+    } else {
+        HIP_TRY(hipMemcpyAsync(r, ctx->out0.ptr, 3 * n * 4, hipMemcpyDeviceToHost, ctx->stream),
+                "hipMemcpy rgb");
+    }
+    HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    return PRR_OK;
+}
+
+extern "C" int prr_develop_planar_into(prr_context *ctx, const prr_header *hdr,
+                                       const uint16_t *samples, size_t samples_len,
+                                       const prr_demosaic_params *params, int stage, float *r,
+                                       float *g, float *b, size_t plane_len, prr_error *err)
+{
+    return develop_host(ctx, hdr, samples, samples_len, params, stage, r, g, b, plane_len, 1, err);
+}
+
+extern "C" int prr_develop_into(prr_context *ctx, const prr_header *hdr, const uint16_t *samples,
+                                size_t samples_len, const prr_demosaic_params *params, int stage,
+                                float *rgb, size_t rgb_len, prr_error *err)
+{
+    return develop_host(ctx, hdr, samples, samples_len, params, stage, rgb, nullptr, nullptr,
+                        rgb_len, 0, err);
+}

@@ -1,0 +1,72 @@
+/* Internal interface between the host layer (prr_host.cpp) and the HIP
+ * kernels (prr_kernels.hip). */
+#ifndef PRR_INTERNAL_H
+#define PRR_INTERNAL_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/proresraw_amd.h"
+
+namespace prr {
+
+/* Everything a decode kernel needs to know about one frame. */
+struct FrameDesc {
+    const uint8_t *packet;   /* device, 4-byte aligned, padded */
+    uint64_t len;            /* packet length in bytes */
+    uint64_t *offsets;       /* device scratch, nb_tiles + 1 tile starts */
+    uint32_t table_off;      /* byte offset of the tile size table */
+    uint32_t nb_tiles;
+    uint32_t tiles_per_row;
+    uint32_t columns;        /* 16-sensel columns */
+    uint32_t alignment;
+    uint32_t coded_width;
+    uint32_t coded_height;
+    uint32_t width;
+    uint32_t height;
+    uint32_t bayer;
+    uint8_t qmat[64];
+    uint16_t curve[8];
+};
+
+/* Develop parameters with the stage applied (demosaic.rs:286-304). */
+struct DevelopDesc {
+    const uint16_t *samples; /* code values at stride `stride` */
+    uint32_t stride;
+    uint32_t width;
+    uint32_t height;
+    int32_t cfa_phase;
+    int32_t black;
+    float one_over_range;
+    float noise[3];
+    float clip[2];
+    float black_over_range;
+    float wb[2];
+    int32_t has_matrix;
+    float m[9];
+    uint16_t curve[8];
+};
+
+/* Launch configuration of the full and reduced decode kernels. */
+struct DecodeLaunch {
+    uint32_t streams_per_wg; /* 16, 32 or 64 lanes decoding streams */
+    uint32_t byte_cap;       /* LDS bytes staged per workgroup */
+    uint32_t lds_limit;      /* bytes of LDS a workgroup may use */
+};
+
+hipError_t launch_scan(const FrameDesc &f, prr_device_status *status, hipStream_t s);
+hipError_t launch_decode(const FrameDesc &f, int ffmpeg, uint16_t *samples, uint16_t *linear,
+                         uint8_t *flags, prr_device_status *status, const DecodeLaunch &cfg,
+                         hipStream_t s);
+hipError_t launch_decode_reduced(const FrameDesc &f, int shift, int average, uint16_t *rgb,
+                                 uint8_t *flags, prr_device_status *status,
+                                 const DecodeLaunch &cfg, hipStream_t s);
+hipError_t launch_linearize(const uint16_t *samples, uint32_t stride, uint32_t width,
+                            uint32_t height, const uint16_t *curve, uint16_t *out,
+                            hipStream_t s);
+hipError_t launch_develop(const DevelopDesc &d, float *r, float *g, float *b, hipStream_t s);
+hipError_t launch_status_init(prr_device_status *status, hipStream_t s);
+
+} // namespace prr
+
+#endif

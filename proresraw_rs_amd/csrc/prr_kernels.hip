@@ -1,0 +1,1256 @@
+/* HIP kernels for gfx950 (MI355X): ProRes RAW decode and develop.
+ *
+ *  k_status_init  per-frame status word (no hipMemsetAsync: graph safe)
+ *  k_scan         size table -> tile offsets, max_qscale, first OOB tile
+ *                 (parse_frame, decoder.rs:104-116)
+ *  k_decode<F>    one wave per group of tiles: stage the group's bytes in
+ *                 LDS, one lane per component stream runs the entropy
+ *                 decoder (bits.rs, coefficients.rs) into LDS coefficient
+ *                 blocks, then the wave runs the 8x8 inverse transform
+ *                 (idct.rs Apple or FFmpeg), places each stream on its
+ *                 Bayer phase (decoder.rs:272-307) through an LDS tile and
+ *                 writes code values and, fused, linear values (curve.rs)
+ *                 with coalesced stores
+ *  k_reduced      the same front end with 8x8 / 4x4 / 2x2 transforms to
+ *                 linear interleaved RGB (reduced.rs:217-273)
+ *  k_linearize    Frame::linearize_into (decoder.rs:197-216)
+ *  k_develop      green pass + chroma pass + white balance + output matrix
+ *                 in one kernel over 64x32 pixel tiles with LDS halos
+ *                 (demosaic.rs:356-720)
+ *
+ * Floating point follows the crate operation for operation: this file is
+ * compiled with -ffp-contract=off, fmaf appears exactly where the crate
+ * calls mul_add, and division and square root are the correctly rounded
+ * defaults.
+ */
+#include "prr_internal.h"
+
+namespace prr {
+
+/* ------------------------------------------------------------------ */
+/* Tables (codebook.rs:26-110, idct.rs:51-87). */
+
+/* Codebook table in LDS: AC 0..94, RN 96..123, LN 128..142, DC 144..156. */
+enum { TBL_AC = 0, TBL_RN = 96, TBL_LN = 128, TBL_DC = 144, TBL_N = 160 };
+/* Scans in LDS after the codebooks: 8x8 at 0, 4x4 at 64, 2x2 at 80. */
+enum { SCAN8 = 0, SCAN4 = 64, SCAN2 = 80, SCAN_N = 96 };
+/* Bytes before the coefficient area: codebooks, scans, curve. */
+enum { LDS_HEAD = 512 };
+
+__device__ const uint16_t g_codebooks[TBL_N] = {
+    /* AC_CB: runs 1x000, 1x211, 2x111, 3x222, 3x122, 8x233, 2x133, 22x244, 52x355, 1x166 */
+    0x000, 0x211, 0x111, 0x111, 0x222, 0x222, 0x222, 0x122, 0x122, 0x122,
+    0x233, 0x233, 0x233, 0x233, 0x233, 0x233, 0x233, 0x233, 0x133, 0x133,
+    0x244, 0x244, 0x244, 0x244, 0x244, 0x244, 0x244, 0x244, 0x244, 0x244,
+    0x244, 0x244, 0x244, 0x244, 0x244, 0x244, 0x244, 0x244, 0x244, 0x244,
+    0x244, 0x244, 0x355, 0x355, 0x355, 0x355, 0x355, 0x355, 0x355, 0x355,
+    0x355, 0x355, 0x355, 0x355, 0x355, 0x355, 0x355, 0x355, 0x355, 0x355,
+    0x355, 0x355, 0x355, 0x355, 0x355, 0x355, 0x355, 0x355, 0x355, 0x355,
+    0x355, 0x355, 0x355, 0x355, 0x355, 0x355, 0x355, 0x355, 0x355, 0x355,
+    0x355, 0x355, 0x355, 0x355, 0x355, 0x355, 0x355, 0x355, 0x355, 0x355,
+    0x355, 0x355, 0x355, 0x355, 0x166, 0, /* 96: RN_CB */
+    0x200, 0x100, 0x000, 0x000, 0x211, 0x211, 0x111, 0x111, 0x011, 0x011,
+    0x021, 0x021, 0x222, 0x022, 0x022, 0x022, 0x022, 0x022, 0x022, 0x022,
+    0x022, 0x022, 0x022, 0x022, 0x022, 0x032, 0x032, 0x044, 0, 0, 0, 0, /* 128: LN_CB */
+    0x100, 0x111, 0x222, 0x222, 0x122, 0x122, 0x433, 0x433, 0x233, 0x233,
+    0x233, 0x233, 0x233, 0x233, 0x033, 0, /* 144: DC_CB */
+    0x010, 0x021, 0x032, 0x033, 0x033, 0x033, 0x044, 0x044, 0x044, 0x044,
+    0x044, 0x044, 0x076, 0, 0, 0};
+
+__device__ const uint8_t g_scans[SCAN_N] = {
+    0,  8,  1,  9,  16, 24, 17, 25, 2,  10, 3,  11, 18, 26, 19, 27, 32, 40, 33, 34, 41, 48,
+    56, 49, 42, 35, 43, 50, 57, 58, 51, 59, 4,  12, 5,  6,  13, 20, 28, 21, 14, 7,  15, 22,
+    29, 36, 44, 37, 30, 23, 31, 38, 45, 52, 60, 53, 46, 39, 47, 54, 61, 62, 55, 63,
+    0,  4,  1,  5,  8,  12, 9,  13, 2,  6,  3,  7,  10, 14, 11, 15, /* 4x4 */
+    0,  2,  1,  3,  0,  0,  0,  0,  0,  0,  0,  0,  0,  0,  0,  0}; /* 2x2 */
+
+__device__ constexpr int32_t kRowTables[4][4][8] = {
+    {{4096, 5352, 4096, 2217, 4096, -2217, 4096, -5352},
+     {4096, 2217, -4096, -5352, -4096, 5352, 4096, -2217},
+     {5681, 4816, 4816, -1130, 3218, -5681, 1130, -3218},
+     {3218, 1130, -5681, -3218, 1130, 4816, 4816, -5681}},
+    {{5681, 7423, 5681, 3075, 5681, -3075, 5681, -7423},
+     {5681, 3075, -5681, -7423, -5681, 7423, 5681, -3075},
+     {7880, 6681, 6681, -1567, 4464, -7880, 1567, -4464},
+     {4464, 1567, -7880, -4464, 1567, 6681, 6681, -7880}},
+    {{5352, 6992, 5352, 2896, 5352, -2896, 5352, -6992},
+     {5352, 2896, -5352, -6992, -5352, 6992, 5352, -2896},
+     {7423, 6293, 6293, -1477, 4205, -7423, 1477, -4205},
+     {4205, 1477, -7423, -4205, 1477, 6293, 6293, -7423}},
+    {{4816, 6293, 4816, 2607, 4816, -2607, 4816, -6293},
+     {4816, 2607, -4816, -6293, -4816, 6293, 4816, -2607},
+     {6681, 5663, 5663, -1329, 3784, -6681, 1329, -3784},
+     {3784, 1329, -6681, -3784, 1329, 5663, 5663, -6681}}};
+__device__ constexpr int kRowSet[8] = {0, 1, 2, 3, 0, 3, 2, 1};
+
+__device__ inline float bits_f32(uint32_t u) { return __uint_as_float(u); }
+
+/* ------------------------------------------------------------------ */
+/* Small helpers. */
+
+__device__ inline uint32_t be16_at(const uint8_t *p)
+{
+    return ((uint32_t)p[0] << 8) | p[1];
+}
+
+/* Curve::linearize (curve.rs:14-24) from the 9-point curve in LDS
+ * (c[8] = 65536). */
+__device__ inline uint32_t linearize(const uint32_t *c, uint32_t code)
+{
+    uint32_t seg = code >> 13, frac = code & 0x1FFFu;
+    uint32_t c0 = c[seg], c1 = c[seg + 1];
+    uint32_t v = (c0 * 8192u + ((c1 - c0) & 0xFFFFu) * frac + 4096u) >> 13;
+    return v < 65535u ? v : 65535u;
+}
+
+/* Tile geometry from its frame-order index (TileGrid::row, tile.rs:569-591). */
+__device__ inline void tile_geometry(const FrameDesc &f, uint32_t t, uint32_t &x, uint32_t &y,
+                                     uint32_t &log2)
+{
+    uint32_t row = t / f.tiles_per_row;
+    uint32_t k = t - row * f.tiles_per_row;
+    uint32_t a = f.alignment;
+    uint32_t full = f.columns >> a;
+    y = row * 16;
+    if (k < full) {
+        x = (k << a) * 16;
+        log2 = a;
+        return;
+    }
+    uint32_t j = k - full, rem = f.columns & ((1u << a) - 1), col = full << a;
+    log2 = 0;
+    x = col * 16;
+    for (int e = (int)a - 1; e >= 0; e--) {
+        if (rem & (1u << e)) {
+            if (j == 0) {
+                log2 = (uint32_t)e;
+                x = col * 16;
+                return;
+            }
+            j--;
+            col += 1u << e;
+        }
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* Bit reader over a dword-addressed byte source (bits.rs:17-111). The
+ * source is the group's bytes staged in LDS, or the packet in global
+ * memory when they do not fit; positions are byte offsets into it. */
+
+__device__ inline uint32_t load_be(const uint32_t *buf, uint32_t pos)
+{
+    uint32_t i = pos >> 2;
+    uint32_t lo = buf[i], hi = buf[i + 1];
+    uint32_t w = __builtin_amdgcn_alignbyte(hi, lo, pos & 3u);
+    return __builtin_bswap32(w);
+}
+
+/* The top `n` bytes of w, zero below (n clamped to 0..4). */
+__device__ inline uint32_t keep_bytes(uint32_t w, int32_t n)
+{
+    if (n >= 4)
+        return w;
+    if (n <= 0)
+        return 0;
+    return w & (0xFFFFFFFFu << (32 - 8 * n));
+}
+
+struct Reader {
+    const uint32_t *buf;
+    uint32_t pos;
+    uint32_t end;
+    uint64_t window;
+    int32_t extra;
+
+    /* BitReader::new (bits.rs:44-76). */
+    __device__ inline void init(const uint32_t *b, uint32_t start, uint32_t stop, uint32_t align)
+    {
+        buf = b;
+        end = stop;
+        uint32_t len = stop - start, remaining = len, address = align;
+        uint32_t first;
+        if (remaining < 4) {
+            first = keep_bytes(load_be(b, start), (int32_t)remaining);
+            pos = start + remaining;
+            address += remaining;
+            remaining = 0;
+        } else {
+            first = load_be(b, start);
+            pos = start + 4;
+            address += 4;
+            remaining -= 4;
+        }
+        uint32_t low = 0;
+        extra = 0;
+        uint32_t to_boundary = (4u - (address & 3u)) & 3u;
+        if (to_boundary != 0 && to_boundary <= remaining) {
+            low = keep_bytes(load_be(b, pos), (int32_t)to_boundary);
+            pos += to_boundary;
+            extra = 8 * (int32_t)to_boundary;
+        }
+        window = ((uint64_t)first << 32) | low;
+    }
+
+    /* next_word (bits.rs:79-89): four bytes zero-filled past the end. */
+    __device__ inline uint32_t next_word()
+    {
+        int32_t n = (int32_t)(end - pos);
+        uint32_t w = n > 0 ? keep_bytes(load_be(buf, pos), n) : 0u;
+        pos = n >= 4 ? pos + 4 : end;
+        return w;
+    }
+
+    __device__ inline uint32_t top() const { return (uint32_t)(window >> 32); }
+
+    /* consume (bits.rs:97-105). */
+    __device__ inline void consume(uint32_t len)
+    {
+        window <<= (len & 63u);
+        extra = (int32_t)((uint32_t)extra - len);
+        if (extra < 0) {
+            uint32_t shift = (uint32_t)(-(int32_t)(int8_t)(uint8_t)(uint32_t)extra) & 0x3fu;
+            window |= (uint64_t)next_word() << shift;
+            extra += 32;
+        }
+    }
+};
+
+/* codeword (bits.rs:116-134) of a non-zero view. The first branch of the
+ * reference (top bit set) is the Rice branch with zero leading zeros. */
+__device__ inline void codeword(uint32_t p, uint32_t cb, uint32_t &value, uint32_t &len)
+{
+    uint32_t lz = (uint32_t)__builtin_clz(p);
+    uint32_t s = cb >> 8, e = (cb >> 4) & 15u, r = cb & 15u;
+    if (lz <= s) {
+        uint32_t tail = ((p << ((lz + 1) & 31u)) >> 1) >> ((31u - r) & 31u);
+        value = (lz << r) + tail;
+        len = r + 1 + lz;
+    } else {
+        uint32_t l = 2 * lz - s + e;
+        value = ((s + 1) << r) + (p >> ((32u - l) & 31u)) - (1u << e);
+        len = l;
+    }
+}
+
+enum { ST_FIRST_DC, ST_DC, ST_RUN, ST_LEVEL, ST_ZEROS, ST_AFTER_ZEROS };
+
+/* decode_component_reduced (coefficients.rs:37-146) as a one-symbol-per-
+ * iteration state machine, so lanes decoding different streams stay in
+ * step. Coefficient (raster k, block b) goes to coefs[(k << kshift) + b],
+ * kshift being the frame's largest log2_blocks: the frequency-major layout
+ * the transform phase reads without bank conflicts. Returns true if the symbol bound was hit. */
+__device__ inline bool decode_stream(Reader &r, uint32_t log2_blocks, uint32_t log2_coefs,
+                                     uint32_t kshift, const uint8_t *scan, const uint16_t *tbl,
+                                     int16_t *coefs, uint32_t budget)
+{
+    const uint32_t blocks = 1u << log2_blocks;
+    const uint32_t codes = 1u << (log2_blocks + log2_coefs);
+    const uint32_t mask = blocks - 1;
+    uint32_t state = ST_FIRST_DC, n = 0, left = 0, sign = 0;
+    int32_t dc = 0;
+    uint32_t dc_cb = 0x064, ac_cb = 0x031, rn_cb = 0x000, ln_cb = 0x042;
+    for (uint32_t it = 0;; ++it) {
+        uint32_t top = r.top();
+        if (top == 0)
+            return false;
+        if (it >= budget)
+            return true;
+        uint32_t cb = state == ST_FIRST_DC ? 0x2BCu
+                      : state == ST_DC     ? dc_cb
+                      : state == ST_RUN    ? ln_cb
+                      : state == ST_ZEROS  ? rn_cb
+                                           : ac_cb;
+        uint32_t sym, len;
+        codeword(top, cb, sym, len);
+        bool level = state == ST_LEVEL || state == ST_AFTER_ZEROS;
+        r.consume(level ? len + 1 : len);
+        if (level) {
+            bool negative = ((top << (len & 31u)) >> 31) != 0;
+            uint16_t magnitude = (uint16_t)((uint16_t)sym + 1u);
+            int16_t value = (int16_t)(negative ? (uint16_t)(0u - magnitude) : magnitude);
+            uint32_t k = scan[n >> log2_blocks];
+            coefs[(k << kshift) + (n & mask)] = value;
+            ac_cb = tbl[TBL_AC + (sym < 94u ? sym : 94u)];
+            n += 1;
+            if (state == ST_AFTER_ZEROS) {
+                ln_cb = tbl[TBL_LN + (sym < 14u ? sym : 14u)];
+                if (n >= codes)
+                    return false;
+                state = ST_RUN;
+            } else {
+                if (n == codes)
+                    return false;
+                left -= 1;
+                state = left ? ST_LEVEL : ST_ZEROS;
+            }
+        } else if (state == ST_RUN) {
+            left = sym;
+            state = left ? ST_LEVEL : ST_ZEROS;
+        } else if (state == ST_ZEROS) {
+            rn_cb = tbl[TBL_RN + (sym < 27u ? sym : 27u)];
+            n = n + 1 + sym;
+            if (n >= codes)
+                return false;
+            state = ST_AFTER_ZEROS;
+        } else {
+            if (state == ST_FIRST_DC) {
+                uint32_t odd = sym & 1u;
+                dc = (int32_t)((((odd + sym) >> 1) ^ (0u - odd)) + odd);
+                state = ST_DC;
+            } else {
+                sign ^= sym & 1u;
+                uint32_t magnitude = ((sym & 1u) + sym) >> 1;
+                int32_t delta = (int32_t)(((0u - sign) ^ magnitude) + sign);
+                dc_cb = tbl[TBL_DC + (magnitude < 12u ? magnitude : 12u)];
+                sign = delta < 0 ? 1u : 0u;
+                dc = (int32_t)((uint32_t)dc + (uint32_t)delta);
+            }
+            coefs[n] = (int16_t)dc; /* raster 0 of block n */
+            n += 1;
+            if (n == blocks) {
+                if (n >= codes)
+                    return false;
+                state = ST_RUN;
+            }
+        }
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* Inverse transforms (idct.rs). d holds dequantized coefficients. */
+
+__device__ inline int32_t dequantize(int32_t c, int32_t q)
+{
+    int32_t v = c * q;
+    return v < -32768 ? -32768 : v > 32767 ? 32767 : v;
+}
+
+/* trunc_to_i32 + output_code (idct.rs:108-123). */
+__device__ inline uint32_t output_code(float v)
+{
+    int32_t t = (v >= -2147483648.0f && v < 2147483648.0f) ? (int32_t)v : INT32_MIN;
+    int32_t centred = (int32_t)((uint32_t)t - 32768u);
+    centred = centred < -32768 ? -32768 : centred > 32767 ? 32767 : centred;
+    return ((uint32_t)(uint16_t)(int16_t)centred) ^ 0x8000u;
+}
+
+/* apple_block (idct.rs:126-174): integer rows, unfused f32 columns. */
+__device__ inline void apple_8x8(const int32_t *d, uint32_t *out)
+{
+    float rows[64];
+#pragma unroll
+    for (int r = 0; r < 8; r++) {
+        const int set = kRowSet[r];
+        const int32_t *x = d + 8 * r;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            int32_t even = (x[0] * kRowTables[set][0][2 * k] + x[2] * kRowTables[set][0][2 * k + 1]) +
+                           (x[4] * kRowTables[set][1][2 * k] + x[6] * kRowTables[set][1][2 * k + 1]);
+            int32_t odd = (x[1] * kRowTables[set][2][2 * k] + x[3] * kRowTables[set][2][2 * k + 1]) +
+                          (x[5] * kRowTables[set][3][2 * k] + x[7] * kRowTables[set][3][2 * k + 1]);
+            rows[8 * r + k] = (float)(even + odd) * bits_f32(0x39800000u);
+            rows[8 * r + 7 - k] = (float)(even - odd) * bits_f32(0x39800000u);
+        }
+    }
+    const float BIAS = bits_f32(0x47000080u), TAN1 = bits_f32(0x3e4bafafu),
+                TAN2 = bits_f32(0x3ed413cdu), TAN3 = bits_f32(0x3f2b0dc1u),
+                COS4 = bits_f32(0x3f3504f3u);
+#pragma unroll
+    for (int c = 0; c < 8; c++) {
+        float r0 = rows[c], r1 = rows[8 + c], r2 = rows[16 + c], r3 = rows[24 + c];
+        float r4 = rows[32 + c], r5 = rows[40 + c], r6 = rows[48 + c], r7 = rows[56 + c];
+        float p = r0 + BIAS;
+        float ea = p + r4, eb = p - r4;
+        float f = r6 * TAN2 + r2, g = r2 * TAN2 - r6;
+        float e0 = ea + f, e3 = ea - f, e1 = eb + g, e2 = eb - g;
+        float a = r7 * TAN1 + r1, b = r1 * TAN1 - r7;
+        float cc = r5 * TAN3 + r3, dd = r5 - r3 * TAN3;
+        float o0 = a + cc, t = a - cc, o3 = b + dd, u = b - dd;
+        float o1 = (t + u) * COS4, o2 = (t - u) * COS4;
+        out[c] = output_code(e0 + o0);
+        out[8 + c] = output_code(e1 + o1);
+        out[16 + c] = output_code(e2 + o2);
+        out[24 + c] = output_code(e3 + o3);
+        out[32 + c] = output_code(e3 - o3);
+        out[40 + c] = output_code(e2 - o2);
+        out[48 + c] = output_code(e1 - o1);
+        out[56 + c] = output_code(e0 - o0);
+    }
+}
+
+/* butterfly + outputs (idct.rs:281-332), wrapping 32-bit arithmetic. */
+__device__ inline void ff_pass(const int32_t *x, int32_t round, int shift, int32_t *y)
+{
+    auto w = [](int32_t k, int32_t v) { return (uint32_t)k * (uint32_t)v; };
+    uint32_t base = w(16384, x[0]) + (uint32_t)round;
+    uint32_t a0 = base + w(21407, x[2]) + w(16384, x[4]) + w(8867, x[6]);
+    uint32_t a1 = base + w(8867, x[2]) - w(16384, x[4]) - w(21407, x[6]);
+    uint32_t a2 = base - w(8867, x[2]) - w(16384, x[4]) + w(21407, x[6]);
+    uint32_t a3 = base - w(21407, x[2]) + w(16384, x[4]) - w(8867, x[6]);
+    uint32_t b0 = w(22725, x[1]) + w(19265, x[3]) + w(12873, x[5]) + w(4520, x[7]);
+    uint32_t b1 = w(19265, x[1]) - w(4520, x[3]) - w(22725, x[5]) - w(12873, x[7]);
+    uint32_t b2 = w(12873, x[1]) - w(22725, x[3]) + w(4520, x[5]) + w(19265, x[7]);
+    uint32_t b3 = w(4520, x[1]) - w(12873, x[3]) + w(19265, x[5]) - w(22725, x[7]);
+    y[0] = (int32_t)(a0 + b0) >> shift;
+    y[1] = (int32_t)(a1 + b1) >> shift;
+    y[2] = (int32_t)(a2 + b2) >> shift;
+    y[3] = (int32_t)(a3 + b3) >> shift;
+    y[4] = (int32_t)(a3 - b3) >> shift;
+    y[5] = (int32_t)(a2 - b2) >> shift;
+    y[6] = (int32_t)(a1 - b1) >> shift;
+    y[7] = (int32_t)(a0 - b0) >> shift;
+}
+
+/* ffmpeg_block (idct.rs:268-351) with code_value (idct.rs:263-265). */
+__device__ inline void ffmpeg_8x8(const int32_t *d, uint32_t *out)
+{
+    int32_t blk[64];
+#pragma unroll
+    for (int r = 0; r < 8; r++)
+        ff_pass(d + 8 * r, 1 << 12, 13, blk + 8 * r);
+#pragma unroll
+    for (int c = 0; c < 8; c++) {
+        int32_t x[8], y[8];
+#pragma unroll
+        for (int i = 0; i < 8; i++)
+            x[i] = blk[8 * i + c];
+        x[0] = (int32_t)((uint32_t)x[0] + 1u);
+        ff_pass(x, 0, 15, y);
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            int64_t v = (int64_t)y[i] + 32768;
+            out[8 * i + c] = (uint32_t)(v < 0 ? 0 : v > 65535 ? 65535 : v);
+        }
+    }
+}
+
+/* apple_block_4x4 (idct.rs:179-226); d holds the 4x4 corner. */
+__device__ inline void apple_4x4(const int32_t *d, uint32_t *out)
+{
+    const float SCALE4 = bits_f32(0x39000000u), BIAS = bits_f32(0x47000080u),
+                TAN2 = bits_f32(0x3ed413cdu);
+    float rows[16];
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        const int32_t k = (r & 1) ? 10703 : 8192;
+        const int32_t t0 = (r & 1) ? 13985 : 10703, t1 = (r & 1) ? 5793 : 4433;
+        const int32_t t2 = (r & 1) ? 5793 : 4433, t3 = (r & 1) ? -13985 : -10703;
+        const int32_t *x = d + 4 * r;
+        int32_t e0 = k * x[0] + k * x[2], e1 = k * x[0] - k * x[2];
+        int32_t o0 = t0 * x[1] + t1 * x[3], o1 = t2 * x[1] + t3 * x[3];
+        rows[4 * r + 0] = (float)(e0 + o0) * SCALE4;
+        rows[4 * r + 1] = (float)(e1 + o1) * SCALE4;
+        rows[4 * r + 2] = (float)(e1 - o1) * SCALE4;
+        rows[4 * r + 3] = (float)(e0 - o0) * SCALE4;
+    }
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        float r0 = rows[c] + BIAS;
+        float r1 = rows[4 + c], r2 = rows[8 + c], r3 = rows[12 + c];
+        float e0 = r2 + r0, e1 = r0 - r2;
+        float f = r3 * TAN2 + r1, g = r1 * TAN2 - r3;
+        out[c] = output_code(f + e0);
+        out[4 + c] = output_code(g + e1);
+        out[8 + c] = output_code(e1 - g);
+        out[12 + c] = output_code(e0 - f);
+    }
+}
+
+__device__ inline uint32_t saturate_code(int32_t v)
+{
+    v = v < -32768 ? -32768 : v > 32767 ? 32767 : v;
+    return ((uint32_t)(uint16_t)(int16_t)v) ^ 0x8000u;
+}
+
+/* apple_block_2x2 (idct.rs:232-247). */
+__device__ inline void apple_2x2(const int32_t *d, uint32_t *out)
+{
+    int32_t ts = d[0] + d[1], td = d[0] - d[1], bs = d[2] + d[3], bd = d[2] - d[3];
+    out[0] = saturate_code(ts + bs);
+    out[1] = saturate_code(td + bd);
+    out[2] = saturate_code(ts - bs);
+    out[3] = saturate_code(td - bd);
+}
+
+/* ------------------------------------------------------------------ */
+/* Shared front end of the decode kernels: LDS layout, staging, tile
+ * headers and the entropy decode of every stream of the group. */
+
+struct TileInfo {
+    uint32_t x, y, log2, qscale, valid;
+};
+
+/* Padding (in int16) after each stream's coefficients so that the lanes
+ * of one transform chunk read distinct LDS banks. */
+__device__ __host__ inline uint32_t stream_stride(uint32_t per_block, uint32_t amax)
+{
+    uint32_t blocks = 1u << amax;
+    return (per_block << amax) + (blocks < 2 ? 2 : blocks);
+}
+
+struct Group {
+    uint16_t *tbl;
+    uint8_t *scan;
+    uint32_t *curve;     /* 9 entries */
+    TileInfo *tiles;     /* up to 16 */
+    int16_t *coefs;
+    uint16_t *stage;     /* 4096 entries */
+    uint32_t *bytes;
+    uint32_t t0, t_end, T;
+    uint64_t base4;
+    bool staged;
+};
+
+/* Carve the dynamic LDS. Layout: [head 512 B][tiles 16 x 20 B][coefs][stage 8 KB][bytes]. */
+__device__ inline Group carve(uint32_t *smem, uint32_t spw, uint32_t stride)
+{
+    Group g;
+    uint8_t *base = (uint8_t *)smem;
+    g.tbl = (uint16_t *)base;                                /* 320 B */
+    g.scan = base + TBL_N * 2;                               /* 96 B */
+    g.curve = (uint32_t *)(base + TBL_N * 2 + SCAN_N);       /* 36 B, ends at 452 */
+    g.tiles = (TileInfo *)(base + LDS_HEAD);                 /* 320 B */
+    g.coefs = (int16_t *)(base + LDS_HEAD + 16 * sizeof(TileInfo));
+    size_t coef_bytes = ((size_t)spw * stride * 2 + 15) & ~(size_t)15;
+    g.stage = (uint16_t *)((uint8_t *)g.coefs + coef_bytes);
+    g.bytes = (uint32_t *)((uint8_t *)g.stage + 8192);
+    return g;
+}
+
+__host__ inline size_t group_lds_bytes(uint32_t spw, uint32_t stride, uint32_t byte_cap)
+{
+    size_t coef_bytes = ((size_t)spw * stride * 2 + 15) & ~(size_t)15;
+    return LDS_HEAD + 16 * 20 + coef_bytes + 8192 + byte_cap + 16;
+}
+
+/* Everything up to and including the entropy decode. Each lane `l` below
+ * 4T decodes stream l & 3 of the group's tile l >> 2 whose streams are
+ * listed in `want` (bit s set = decode stream s). Returns the lane's
+ * overrun flag. */
+__device__ inline bool front_end(const FrameDesc &f, Group &g, uint32_t spw, uint32_t stride,
+                                 uint32_t byte_cap, uint32_t log2_coefs, uint32_t want,
+                                 uint8_t *flags, prr_device_status *status)
+{
+    const uint32_t lane = threadIdx.x;
+    g.T = spw >> 2;
+    g.t0 = blockIdx.x * g.T;
+    g.t_end = min(g.t0 + g.T, f.nb_tiles);
+
+    for (uint32_t i = lane; i < TBL_N; i += 64)
+        g.tbl[i] = g_codebooks[i];
+    for (uint32_t i = lane; i < SCAN_N; i += 64)
+        g.scan[i] = g_scans[i];
+    if (lane < 9)
+        g.curve[lane] = lane < 8 ? f.curve[lane] : 65536u;
+    uint32_t zero_words = (spw * stride) / 2;
+    for (uint32_t i = lane; i < zero_words; i += 64)
+        ((uint32_t *)g.coefs)[i] = 0;
+
+    uint64_t r0 = f.offsets[g.t0], r1 = f.offsets[g.t_end];
+    if (r1 > f.len)
+        r1 = f.len;
+    if (r0 > r1)
+        r0 = r1;
+    g.base4 = r0 & ~(uint64_t)3;
+    uint64_t nbytes = r1 - g.base4;
+    g.staged = nbytes + 8 <= byte_cap;
+    if (g.staged) {
+        const uint32_t *src = (const uint32_t *)(f.packet + g.base4);
+        uint32_t nd = (uint32_t)((nbytes + 3) >> 2) + 2;
+        for (uint32_t i = lane; i < nd; i += 64)
+            g.bytes[i] = src[i];
+    }
+
+    /* Tile::parse (tile.rs:126-154), by the four lanes of each tile. */
+    const uint32_t tl = lane >> 2, s = lane & 3;
+    const uint32_t t = g.t0 + tl;
+    bool have = lane < 4 * g.T && t < g.t_end;
+    uint64_t toff = 0, tend = 0;
+    bool valid = false;
+    uint32_t hl = 0, sz[3] = {0, 0, 0}, qs = 0;
+    if (have) {
+        toff = f.offsets[t];
+        tend = f.offsets[t + 1];
+        if (tend <= f.len && tend - toff >= 8) {
+            const uint8_t *h = f.packet + toff;
+            hl = h[0] >> 3;
+            qs = h[1];
+            sz[0] = be16_at(h + 2);
+            sz[1] = be16_at(h + 4);
+            sz[2] = be16_at(h + 6);
+            valid = hl >= 8 && qs != 0 && sz[0] && sz[1] && sz[2] &&
+                    (uint64_t)(hl + sz[0] + sz[1] + sz[2]) < tend - toff;
+        }
+    }
+    uint32_t tx = 0, ty = 0, tlog2 = 0;
+    if (have)
+        tile_geometry(f, t, tx, ty, tlog2);
+    if (have && s == 0) {
+        TileInfo info = {tx, ty, tlog2, qs, (uint32_t)(valid ? 1 : 0)};
+        if (tend > f.len)
+            info.valid = 2; /* out of bounds: the frame is an error, write nothing */
+        g.tiles[tl] = info;
+        if (flags)
+            flags[t] = valid ? 0 : 1;
+        if (!valid && tend <= f.len)
+            atomicAdd(&status->n_skipped, 1u);
+    }
+    __syncthreads();
+
+    bool overrun = false;
+    if (valid && ((want >> s) & 1u)) {
+        uint64_t start = toff + hl;
+        for (uint32_t k = 0; k < s; k++)
+            start += sz[k];
+        uint64_t stop = s < 3 ? start + sz[s] : tend;
+        uint32_t align = (uint32_t)(start & 3);
+        uint32_t codes = 1u << (tlog2 + log2_coefs);
+        uint32_t budget = 16u * (8u * (uint32_t)(stop - start) + codes) + 1024u;
+        const uint8_t *scan = g.scan + (log2_coefs == 6 ? SCAN8 : log2_coefs == 4 ? SCAN4 : SCAN2);
+        int16_t *mine = g.coefs + lane * stride;
+        Reader rd;
+        if (g.staged) {
+            rd.init(g.bytes, (uint32_t)(start - g.base4), (uint32_t)(stop - g.base4), align);
+        } else {
+            rd.init((const uint32_t *)(f.packet + g.base4), (uint32_t)(start - g.base4),
+                    (uint32_t)(stop - g.base4), align);
+        }
+        overrun = decode_stream(rd, tlog2, log2_coefs, f.alignment, scan, g.tbl, mine, budget);
+    }
+    if (overrun)
+        atomicOr(&status->overrun, 1u);
+    __syncthreads();
+    return overrun;
+}
+
+/* ------------------------------------------------------------------ */
+
+__global__ void k_status_init(prr_device_status *status)
+{
+    if (threadIdx.x == 0) {
+        status->oob_tile = 0xFFFFFFFFu;
+        status->max_qscale = 0;
+        status->n_skipped = 0;
+        status->overrun = 0;
+    }
+}
+
+/* Size-table walk of parse_frame (decoder.rs:104-116) as a block scan. */
+__global__ __launch_bounds__(1024) void k_scan(FrameDesc f, prr_device_status *status)
+{
+    __shared__ uint64_t sums[1024];
+    const uint32_t n = f.nb_tiles, t = threadIdx.x;
+    const uint32_t per = (n + 1023) / 1024;
+    const uint32_t lo = min(t * per, n), hi = min(lo + per, n);
+    const uint8_t *tab = f.packet + f.table_off;
+    uint64_t sum = 0;
+    for (uint32_t i = lo; i < hi; i++)
+        sum += be16_at(tab + 2 * (size_t)i);
+    sums[t] = sum;
+    __syncthreads();
+    for (uint32_t d = 1; d < 1024; d <<= 1) {
+        uint64_t v = t >= d ? sums[t - d] : 0;
+        __syncthreads();
+        sums[t] += v;
+        __syncthreads();
+    }
+    uint64_t off = (uint64_t)f.table_off + 2ull * n + (t ? sums[t - 1] : 0);
+    uint32_t mq = 0, oob = 0xFFFFFFFFu;
+    for (uint32_t i = lo; i < hi; i++) {
+        uint32_t size = be16_at(tab + 2 * (size_t)i);
+        uint32_t q = off + 1 < f.len ? f.packet[off + 1] : 0u;
+        mq = q > mq ? q : mq;
+        if (off + size > f.len && oob == 0xFFFFFFFFu)
+            oob = i;
+        f.offsets[i] = off;
+        off += size;
+    }
+    if (t == 1023)
+        f.offsets[n] = (uint64_t)f.table_off + 2ull * n + sums[1023];
+    if (oob != 0xFFFFFFFFu)
+        atomicMin(&status->oob_tile, oob);
+    if (mq)
+        atomicMax(&status->max_qscale, mq);
+}
+
+/* Full decode (decoder.rs:226-307). One 64-lane workgroup per group of
+ * spw/4 tiles. */
+template <int FFMPEG>
+__global__ __launch_bounds__(64) void k_decode(FrameDesc f, uint16_t *samples, uint16_t *linear,
+                                                uint8_t *flags, prr_device_status *status,
+                                                uint32_t spw, uint32_t byte_cap)
+{
+    extern __shared__ uint32_t smem[];
+    const uint32_t amax = f.alignment;
+    const uint32_t stride = stream_stride(64, amax);
+    Group g = carve(smem, spw, stride);
+    front_end(f, g, spw, stride, byte_cap, 6, 0xFu, flags, status);
+
+    const uint32_t lane = threadIdx.x;
+    const uint32_t blocks_max = 1u << amax;
+    const uint32_t W = 16u << amax;                 /* stage tile width */
+    const uint32_t tiles_per_chunk = 64u / (4u << amax) ? 64u / (4u << amax) : 1u;
+    /* Bayer phases of the streams (header.rs:79-82). */
+    const uint32_t rr = f.bayer >> 1, rc = f.bayer & 1;
+    const uint32_t slots = (4u * g.T) << amax;
+    for (uint32_t c0 = 0; c0 < slots; c0 += 64) {
+        const uint32_t slot = c0 + lane;
+        const uint32_t ls = slot >> amax, b = slot & (blocks_max - 1);
+        const uint32_t tl = ls >> 2, s = ls & 3;
+        const uint32_t tc = tl - (c0 >> amax) / 4;
+        if (g.t0 + tl < g.t_end) {
+            TileInfo ti = g.tiles[tl];
+            if (ti.valid == 1 && b < (1u << ti.log2)) {
+                const int16_t *src = g.coefs + ls * stride + b;
+                int32_t d[64];
+#pragma unroll
+                for (int k = 0; k < 64; k++)
+                    d[k] = dequantize(src[k << amax], (int32_t)f.qmat[k] * (int32_t)ti.qscale);
+                uint32_t v[64];
+                if (FFMPEG)
+                    ffmpeg_8x8(d, v);
+                else
+                    apple_8x8(d, v);
+                const uint32_t pr = (s == 0 || s == 2) ? 1 - rr : rr;
+                const uint32_t pc = (s == 1 || s == 2) ? 1 - rc : rc;
+                uint16_t *dst = g.stage + tc * 16 * W + pr * W + 16 * b + pc;
+#pragma unroll
+                for (int i = 0; i < 8; i++)
+#pragma unroll
+                    for (int j = 0; j < 8; j++)
+                        dst[2 * i * W + 2 * j] = (uint16_t)v[8 * i + j];
+            }
+        }
+        __syncthreads();
+        /* Coalesced copy of the chunk's tiles: two sensels per lane and
+         * store, code values and (fused) linear values. */
+        const uint32_t first_tile = (c0 >> amax) / 4;
+        const uint32_t dw_per_row = W / 2;
+        const uint32_t total = tiles_per_chunk * 16 * dw_per_row;
+        for (uint32_t i = lane; i < total; i += 64) {
+            const uint32_t tcc = i / (16 * dw_per_row);
+            const uint32_t rem = i - tcc * 16 * dw_per_row;
+            const uint32_t row = rem / dw_per_row, dcol = rem - row * dw_per_row;
+            const uint32_t tli = first_tile + tcc;
+            if (tli >= g.T || g.t0 + tli >= g.t_end)
+                continue;
+            TileInfo ti = g.tiles[tli];
+            if (ti.valid == 2 || dcol >= (8u << ti.log2))
+                continue;
+            uint32_t pair = 0;
+            if (ti.valid == 1)
+                pair = *(const uint32_t *)(g.stage + tcc * 16 * W + row * W + 2 * dcol);
+            const uint32_t y = ti.y + row, x = ti.x + 2 * dcol;
+            *(uint32_t *)(samples + (size_t)y * f.coded_width + x) = pair;
+            if (linear && y < f.height && x < f.width) {
+                uint32_t l0 = linearize(g.curve, pair & 0xFFFFu);
+                uint32_t l1 = linearize(g.curve, pair >> 16);
+                *(uint32_t *)(linear + (size_t)y * f.width + x) = l0 | (l1 << 16);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+/* Reduced decode (reduced.rs:147-273): streams 3, 0, (1,) 2 as R, G, (G,) B,
+ * shift 1/2/3 for Half/Quarter/Eighth. */
+template <uint32_t shift>
+__global__ __launch_bounds__(64) void k_reduced(FrameDesc f, uint16_t *rgb, uint8_t *flags,
+                                                 prr_device_status *status, uint32_t spw,
+                                                 uint32_t byte_cap, uint32_t average)
+{
+    extern __shared__ uint32_t smem[];
+    const uint32_t amax = f.alignment;
+    constexpr uint32_t log2_coefs = shift == 1 ? 6u : shift == 2 ? 4u : 2u;
+    constexpr uint32_t per_block = 1u << log2_coefs;
+    const uint32_t stride = stream_stride(per_block, amax);
+    Group g = carve(smem, spw, stride);
+    const uint32_t want = average ? 0xFu : 0xDu;
+    front_end(f, g, spw, stride, byte_cap, log2_coefs, want, flags, status);
+
+    const uint32_t lane = threadIdx.x;
+    const uint32_t blocks_max = 1u << amax;
+    constexpr uint32_t side = 16u >> shift;         /* pixels per block side */
+    const uint32_t W = side << amax;                /* stage row width in pixels */
+    const uint32_t tiles_per_chunk = 64u / (4u << amax) ? 64u / (4u << amax) : 1u;
+    const uint32_t out_stride = f.coded_width >> shift; /* pixels */
+    const uint32_t slots = (4u * g.T) << amax;
+    for (uint32_t c0 = 0; c0 < slots; c0 += 64) {
+        const uint32_t slot = c0 + lane;
+        const uint32_t ls = slot >> amax, b = slot & (blocks_max - 1);
+        const uint32_t tl = ls >> 2, s = ls & 3;
+        const uint32_t tc = tl - (c0 >> amax) / 4;
+        if (g.t0 + tl < g.t_end && ((want >> s) & 1u)) {
+            TileInfo ti = g.tiles[tl];
+            if (ti.valid == 1 && b < (1u << ti.log2)) {
+                const int16_t *src = g.coefs + ls * stride + b;
+                uint32_t v[64];
+                if constexpr (shift == 1) {
+                    int32_t d[64];
+#pragma unroll
+                    for (int k = 0; k < 64; k++)
+                        d[k] = dequantize(src[k << amax], (int32_t)f.qmat[k] * (int32_t)ti.qscale);
+                    apple_8x8(d, v);
+                } else if constexpr (shift == 2) {
+                    int32_t d[16];
+#pragma unroll
+                    for (int k = 0; k < 16; k++)
+                        d[k] = dequantize(src[k << amax],
+                                          (int32_t)f.qmat[8 * (k >> 2) + (k & 3)] * (int32_t)ti.qscale);
+                    apple_4x4(d, v);
+                } else {
+                    int32_t d[4];
+                    const int qi[4] = {0, 1, 8, 9};
+#pragma unroll
+                    for (int k = 0; k < 4; k++)
+                        d[k] = dequantize(src[k << amax], (int32_t)f.qmat[qi[k]] * (int32_t)ti.qscale);
+                    apple_2x2(d, v);
+                }
+                uint16_t *dst = g.stage + ((tc * 4 + s) * side) * W + b * side;
+#pragma unroll
+                for (uint32_t i = 0; i < side; i++)
+#pragma unroll
+                    for (uint32_t j = 0; j < side; j++)
+                        dst[i * W + j] = (uint16_t)linearize(g.curve, v[i * side + j]);
+            }
+        }
+        __syncthreads();
+        /* Interleaved RGB rows: two values per dword store. */
+        const uint32_t first_tile = (c0 >> amax) / 4;
+        const uint32_t dw_per_row = (3 * W) / 2;
+        const uint32_t total = tiles_per_chunk * side * dw_per_row;
+        for (uint32_t i = lane; i < total; i += 64) {
+            const uint32_t tcc = i / (side * dw_per_row);
+            const uint32_t rem = i - tcc * side * dw_per_row;
+            const uint32_t row = rem / dw_per_row, dcol = rem - row * dw_per_row;
+            const uint32_t tli = first_tile + tcc;
+            if (tli >= g.T || g.t0 + tli >= g.t_end)
+                continue;
+            TileInfo ti = g.tiles[tli];
+            const uint32_t tw = side << ti.log2;
+            if (ti.valid == 2 || dcol >= (3 * tw) / 2)
+                continue;
+            uint32_t pair = 0;
+            if (ti.valid == 1) {
+                uint32_t vals[2];
+                for (int h = 0; h < 2; h++) {
+                    uint32_t e = 2 * dcol + h, px = e / 3, ch = e - px * 3;
+                    const uint16_t *base = g.stage + (tcc * 4 * side + row) * W + px;
+                    uint32_t v;
+                    if (ch == 0) {
+                        v = base[3 * side * W];
+                    } else if (ch == 2) {
+                        v = base[2 * side * W];
+                    } else {
+                        v = base[0];
+                        if (average)
+                            v = (v + base[side * W] + 1) >> 1;
+                    }
+                    vals[h] = v;
+                }
+                pair = vals[0] | (vals[1] << 16);
+            }
+            const uint32_t y = (ti.y >> shift) + row;
+            const size_t at = 3 * ((size_t)y * out_stride + (ti.x >> shift)) + 2 * dcol;
+            *(uint32_t *)(rgb + at) = pair;
+        }
+        __syncthreads();
+    }
+}
+
+/* Frame::linearize_into (decoder.rs:197-216), two values per lane. */
+__global__ __launch_bounds__(256) void k_linearize(const uint16_t *samples, uint32_t stride,
+                                                   uint32_t width, uint32_t height,
+                                                   uint4 curve_lo, uint4 curve_hi, uint16_t *out)
+{
+    __shared__ uint32_t c[9];
+    if (threadIdx.x < 9) {
+        uint32_t w[8] = {curve_lo.x, curve_lo.y, curve_lo.z, curve_lo.w,
+                         curve_hi.x, curve_hi.y, curve_hi.z, curve_hi.w};
+        c[threadIdx.x] = threadIdx.x < 8 ? w[threadIdx.x] : 65536u;
+    }
+    __syncthreads();
+    const uint32_t pairs_per_row = width / 2;
+    const size_t total = (size_t)pairs_per_row * height;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+         i += (size_t)gridDim.x * blockDim.x) {
+        uint32_t y = (uint32_t)(i / pairs_per_row), xp = (uint32_t)(i - (size_t)y * pairs_per_row);
+        uint32_t pair = *(const uint32_t *)(samples + (size_t)y * stride + 2 * xp);
+        uint32_t l0 = linearize(c, pair & 0xFFFFu), l1 = linearize(c, pair >> 16);
+        *(uint32_t *)(out + (size_t)y * width + 2 * xp) = l0 | (l1 << 16);
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* Develop (demosaic.rs). */
+
+/* variance (demosaic.rs:423-426). */
+__device__ inline float variance(float v, float off, float n0, float n1, float n2)
+{
+    float t = off + v;
+    return n2 + fmaxf(n1, n0 * (t * t));
+}
+
+/* dot4 (demosaic.rs:431-433). */
+__device__ inline float dot4(float a0, float a1, float a2, float a3, float b0, float b1, float b2,
+                             float b3)
+{
+    return fmaf(a3, b3, fmaf(a2, b2, fmaf(a0, b0, a1 * b1)));
+}
+
+/* mirror (demosaic.rs:547-557). */
+__device__ inline void mirror4(float *d, uint32_t pos, uint32_t len, float c)
+{
+    if (pos == 0) {
+        float a = d[1], b = d[3];
+        d[0] = a; d[1] = a; d[2] = b; d[3] = b;
+    } else if (pos == 1) {
+        d[2] = c;
+    } else if (pos == len - 1) {
+        float a = d[0], b = d[2];
+        d[0] = a; d[1] = a; d[2] = b; d[3] = b;
+    } else if (pos == len - 2) {
+        d[3] = c;
+    }
+}
+
+/* estimate (demosaic.rs:572-604). */
+__device__ inline float estimate(const float *xx, const float *gg, float cg, float cs, float off,
+                                 float n0, float n1, float n2)
+{
+    float mean_x = dot4(xx[0], xx[1], xx[2], xx[3], 0.25f, 0.25f, 0.25f, 0.25f);
+    if (!(cs < cg))
+        return mean_x;
+    float var_x[4], sx[4], sg[4];
+    bool above = true;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        var_x[i] = variance(xx[i], off, n0, n1, n2);
+        sx[i] = sqrtf(var_x[i]);
+        sg[i] = sqrtf(variance(gg[i], off, n0, n1, n2));
+        above = above & (sx[i] < xx[i]) & (sg[i] < gg[i]);
+    }
+    if (!above)
+        return mean_x;
+    float lo[4], mid[4], hi[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        lo[i] = ((cg - cs) * (xx[i] - sx[i])) / (sg[i] + gg[i]);
+        mid[i] = (cg * xx[i]) / gg[i];
+        hi[i] = ((cs + cg) * (sx[i] + xx[i])) / (gg[i] - sg[i]);
+    }
+    float spread = dot4(mid[0] - lo[0], mid[1] - lo[1], mid[2] - lo[2], mid[3] - lo[3],
+                        hi[0] - mid[0], hi[1] - mid[1], hi[2] - mid[2], hi[3] - mid[3]);
+    float mean_mid = dot4(mid[0], mid[1], mid[2], mid[3], 0.25f, 0.25f, 0.25f, 0.25f);
+    float dm0 = mid[0] - mean_mid, dm1 = mid[1] - mean_mid, dm2 = mid[2] - mean_mid,
+          dm3 = mid[3] - mean_mid;
+    float b = fmaf(spread, 0.75f, dot4(dm0, dm1, dm2, dm3, dm0, dm1, dm2, dm3));
+    float dx0 = xx[0] - mean_x, dx1 = xx[1] - mean_x, dx2 = xx[2] - mean_x, dx3 = xx[3] - mean_x;
+    float a = fmaf(dot4(var_x[0], var_x[1], var_x[2], var_x[3], 1.0f, 1.0f, 1.0f, 1.0f), 0.75f,
+                   dot4(dx0, dx1, dx2, dx3, dx0, dx1, dx2, dx3));
+    float weight = a / (b + a);
+    return fmaf(mean_mid - mean_x, weight, mean_x);
+}
+
+enum { DTW = 64, DTH = 32, NORM_W = DTW + 6, NORM_H = DTH + 6, GRN_W = DTW + 4, GRN_H = DTH + 2 };
+
+/* One 64x32 tile of output per 256-lane workgroup: normalized code values
+ * with a 3-sensel halo and green-pass values with a 1-sensel halo live in
+ * LDS; each lane handles a red/blue + green pair of sensels so every lane
+ * does the same work. */
+__global__ __launch_bounds__(256) void k_develop(DevelopDesc d, float *out_r, float *out_g,
+                                                 float *out_b)
+{
+    __shared__ float norm[NORM_H][NORM_W];
+    __shared__ float2 green[GRN_H][GRN_W];
+    __shared__ uint32_t curve[9];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t w = d.width, h = d.height;
+    const int32_t x0 = (int32_t)(blockIdx.x * DTW), y0 = (int32_t)(blockIdx.y * DTH);
+    if (tid < 9)
+        curve[tid] = tid < 8 ? d.curve[tid] : 65536u;
+    __syncthreads();
+
+    /* Normalized values (Conversion::new folded table, demosaic.rs:293-298),
+     * rows and columns clamped to the picture. */
+    for (uint32_t i = tid; i < NORM_H * NORM_W; i += 256) {
+        int32_t ly = (int32_t)(i / NORM_W), lx = (int32_t)(i - (uint32_t)ly * NORM_W);
+        int32_t y = y0 - 3 + ly, x = x0 - 3 + lx;
+        y = y < 0 ? 0 : y > (int32_t)h - 1 ? (int32_t)h - 1 : y;
+        x = x < 0 ? 0 : x > (int32_t)w - 1 ? (int32_t)w - 1 : x;
+        uint32_t code = d.samples[(size_t)y * d.stride + (uint32_t)x];
+        int32_t lin = (int32_t)linearize(curve, code);
+        norm[ly][lx] = d.one_over_range * (float)(lin - d.black);
+    }
+    __syncthreads();
+
+    /* Green pass (green_row_impl, demosaic.rs:481-541) over pairs. */
+    const float n0 = d.noise[0], n1 = d.noise[1], n2 = d.noise[2];
+    const float bs = (float)d.black * d.one_over_range;
+    const uint32_t red_row = (uint32_t)(d.cfa_phase >> 1), red_col = (uint32_t)(d.cfa_phase & 1);
+    for (uint32_t i = tid; i < GRN_H * (GRN_W / 2); i += 256) {
+        const int32_t ly = (int32_t)(i / (GRN_W / 2));
+        const int32_t lp = (int32_t)(i - (uint32_t)ly * (GRN_W / 2));
+        const int32_t gy = y0 - 1 + ly;
+        if (gy < 0 || gy >= (int32_t)h)
+            continue;
+        const bool phase = ((uint32_t)gy & 1u) == red_row;
+        const uint32_t first = phase ? red_col : 1 - red_col;
+        const float clip = phase ? d.clip[0] : d.clip[1];
+        const int32_t ny = gy - (y0 - 3);
+#pragma unroll
+        for (uint32_t k = 0; k < 2; k++) {
+            const int32_t gx = x0 - 2 + 2 * lp + (int32_t)k;
+            if (gx < x0 - 1 || gx > x0 + DTW || gx < 0 || gx >= (int32_t)w)
+                continue;
+            const int32_t nx = gx - (x0 - 3);
+            const float centre = norm[ny][nx];
+            if (((uint32_t)gx & 1u) != first) {
+                green[ly][2 * lp + k] = make_float2(0.0f, centre);
+                continue;
+            }
+            const float c = fminf(centre, clip);
+            float hh[4] = {norm[ny][nx - 1], norm[ny][nx + 1], norm[ny][nx - 2], norm[ny][nx + 2]};
+            float vv[4] = {norm[ny - 1][nx], norm[ny + 1][nx], norm[ny - 2][nx], norm[ny + 2][nx]};
+            hh[0] = fminf(hh[0], 1.0f);
+            hh[1] = fminf(hh[1], 1.0f);
+            hh[2] = fminf(hh[2], clip);
+            hh[3] = fminf(hh[3], clip);
+            vv[0] = fminf(vv[0], 1.0f);
+            vv[1] = fminf(vv[1], 1.0f);
+            vv[2] = fminf(vv[2], clip);
+            vv[3] = fminf(vv[3], clip);
+            mirror4(hh, (uint32_t)gx, w, c);
+            mirror4(vv, (uint32_t)gy, h, c);
+            float var_h[4], var_v[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                var_h[q] = variance(hh[q], bs, n0, n1, n2);
+                var_v[q] = variance(vv[q], bs, n0, n1, n2);
+            }
+            const float sums0 = hh[0] + hh[1], sums1 = vv[0] + vv[1];
+            const float sums2 = hh[2] + hh[3], sums3 = vv[2] + vv[3];
+            float ratio0 = 0.0f, ratio1 = 0.0f;
+            if (c > sqrtf(variance(c, bs, n0, n1, n2))) {
+                const float t0 = c * 2.0f + sums2, t1 = c * 2.0f + sums3;
+                const float t2 = c * -2.0f + sums2, t3 = c * -2.0f + sums3;
+#pragma unroll
+                for (int dir = 0; dir < 2; dir++) {
+                    const float d0 = dir ? vv[0] : hh[0], d1 = dir ? vv[1] : hh[1];
+                    const float d2 = dir ? vv[2] : hh[2], d3 = dir ? vv[3] : hh[3];
+                    const float sd0 = sqrtf(dir ? var_v[0] : var_h[0]);
+                    const float sd1 = sqrtf(dir ? var_v[1] : var_h[1]);
+                    const float sd2 = sqrtf(dir ? var_v[2] : var_h[2]);
+                    const float sd3 = sqrtf(dir ? var_v[3] : var_h[3]);
+                    float a = d0 - d1, b = d2 - d3;
+                    bool any = a * b > 0.0f || (d2 - c) * (d3 - c) > 0.0f ||
+                               sd0 + sd1 > fabsf(a) || sd2 + sd3 > fabsf(b);
+                    bool all = d0 > sd0 && d1 > sd1 && d2 > sd2 && d3 > sd3;
+                    if (any && all) {
+                        if (dir == 0)
+                            ratio0 = t2 / (t0 * 2.0f);
+                        else
+                            ratio1 = t3 / (t1 * 2.0f);
+                    }
+                }
+            }
+            const float gh = sums0 * 0.5f * (1.0f - ratio0);
+            const float gv = sums1 * 0.5f * (1.0f - ratio1);
+            const float dh0 = hh[0] - gh, dh1 = hh[1] - gh, dh2 = hh[2] - c, dh3 = hh[3] - c;
+            const float dv0 = vv[0] - gv, dv1 = vv[1] - gv, dv2 = vv[2] - c, dv3 = vv[3] - c;
+            const float eh = dot4(dh0, dh1, dh2, dh3, dh0, dh1, dh2, dh3) + (var_h[0] + var_h[1]);
+            const float ev = dot4(dv0, dv1, dv2, dv3, dv0, dv1, dv2, dv3) + (var_v[0] + var_v[1]);
+            green[ly][2 * lp + k] = make_float2(c, gv + (gh - gv) * ev / (eh + ev));
+        }
+    }
+    __syncthreads();
+
+    /* Chroma pass (chroma_row_with, demosaic.rs:648-688) and the output
+     * matrix (matrix_planes_impl, demosaic.rs:711-720). */
+    const float off = d.black_over_range;
+    for (uint32_t i = tid; i < DTH * (DTW / 2); i += 256) {
+        const uint32_t ly = i / (DTW / 2), lp = i - ly * (DTW / 2);
+        const uint32_t y = (uint32_t)y0 + ly;
+        if (y >= h)
+            continue;
+        const uint32_t ym = y > 0 ? y - 1 : y + 1, yp = y < h - 1 ? y + 1 : y - 1;
+        const bool on_red_row = (y & 1u) == red_row;
+        const uint32_t rb_col = on_red_row ? red_col : 1 - red_col;
+        const int32_t gr_m = (int32_t)ym - (y0 - 1), gr_c = (int32_t)y - (y0 - 1),
+                      gr_p = (int32_t)yp - (y0 - 1);
+        float res[2][3];
+        uint32_t xs[2];
+#pragma unroll
+        for (uint32_t k = 0; k < 2; k++) {
+            const uint32_t x = (uint32_t)x0 + 2 * lp + k;
+            xs[k] = x;
+            if (x >= w)
+                continue;
+            const uint32_t xm = x > 0 ? x - 1 : x + 1, xp = x < w - 1 ? x + 1 : x - 1;
+            const int32_t cm = (int32_t)xm - (x0 - 2), cc = (int32_t)x - (x0 - 2),
+                          cp = (int32_t)xp - (x0 - 2);
+            const float2 mid = green[gr_c][cc];
+            const float gval = mid.y;
+            const float sigma = sqrtf(variance(gval, off, n0, n1, n2));
+            float rr, bb;
+            if ((x & 1u) == rb_col) {
+                const float2 a = green[gr_m][cm], b = green[gr_p][cm], c2 = green[gr_m][cp],
+                             e = green[gr_p][cp];
+                const float xx[4] = {a.x, b.x, c2.x, e.x};
+                const float gg[4] = {a.y, b.y, c2.y, e.y};
+                const float est = estimate(xx, gg, gval, sigma, off, n0, n1, n2);
+                if (on_red_row) {
+                    rr = mid.x;
+                    bb = est;
+                } else {
+                    rr = est;
+                    bb = mid.x;
+                }
+            } else {
+                const float2 l = green[gr_c][cm], r = green[gr_c][cp];
+                const float2 u = green[gr_m][cc], dn = green[gr_p][cc];
+                const float hx[4] = {l.x, r.x, l.x, r.x}, hg[4] = {l.y, r.y, l.y, r.y};
+                const float vx[4] = {u.x, dn.x, u.x, dn.x}, vg[4] = {u.y, dn.y, u.y, dn.y};
+                const float eh = estimate(hx, hg, gval, sigma, off, n0, n1, n2);
+                const float ev = estimate(vx, vg, gval, sigma, off, n0, n1, n2);
+                if (on_red_row) {
+                    rr = eh;
+                    bb = ev;
+                } else {
+                    rr = ev;
+                    bb = eh;
+                }
+            }
+            float r = fminf(rr * d.wb[0], 1.0f);
+            float g = fminf(gval, 1.0f);
+            float b = fminf(bb * d.wb[1], 1.0f);
+            if (d.has_matrix) {
+                const float r2 = fmaf(b, d.m[2], fmaf(r, d.m[0], g * d.m[1]));
+                const float g2 = fmaf(b, d.m[5], fmaf(r, d.m[3], g * d.m[4]));
+                const float b2 = fmaf(b, d.m[8], fmaf(r, d.m[6], g * d.m[7]));
+                r = r2;
+                g = g2;
+                b = b2;
+            }
+            res[k][0] = r;
+            res[k][1] = g;
+            res[k][2] = b;
+        }
+        const size_t at = (size_t)y * w + xs[0];
+        if (out_g) {
+            if (xs[1] < w) {
+                *(float2 *)(out_r + at) = make_float2(res[0][0], res[1][0]);
+                *(float2 *)(out_g + at) = make_float2(res[0][1], res[1][1]);
+                *(float2 *)(out_b + at) = make_float2(res[0][2], res[1][2]);
+            } else if (xs[0] < w) {
+                out_r[at] = res[0][0];
+                out_g[at] = res[0][1];
+                out_b[at] = res[0][2];
+            }
+        } else {
+#pragma unroll
+            for (uint32_t k = 0; k < 2; k++)
+                if (xs[k] < w) {
+                    out_r[3 * (at + k) + 0] = res[k][0];
+                    out_r[3 * (at + k) + 1] = res[k][1];
+                    out_r[3 * (at + k) + 2] = res[k][2];
+                }
+        }
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* Launchers. */
+
+hipError_t launch_status_init(prr_device_status *status, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_status_init, dim3(1), dim3(64), 0, s, status);
+    return hipGetLastError();
+}
+
+hipError_t launch_scan(const FrameDesc &f, prr_device_status *status, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, f, status);
+    return hipGetLastError();
+}
+
+static uint32_t pick_spw(const DecodeLaunch &cfg, uint32_t per_block, uint32_t amax, size_t &lds)
+{
+    uint32_t spw = cfg.streams_per_wg;
+    for (;;) {
+        lds = group_lds_bytes(spw, stream_stride(per_block, amax), cfg.byte_cap);
+        if (lds <= cfg.lds_limit || spw <= 4)
+            return spw;
+        spw >>= 1;
+    }
+}
+
+hipError_t launch_decode(const FrameDesc &f, int ffmpeg, uint16_t *samples, uint16_t *linear,
+                         uint8_t *flags, prr_device_status *status, const DecodeLaunch &cfg,
+                         hipStream_t s)
+{
+    size_t lds;
+    uint32_t spw = pick_spw(cfg, 64, f.alignment, lds);
+    uint32_t T = spw / 4;
+    uint32_t groups = (f.nb_tiles + T - 1) / T;
+    if (groups == 0)
+        return hipSuccess;
+    if (ffmpeg)
+        hipLaunchKernelGGL(k_decode<1>, dim3(groups), dim3(64), lds, s, f, samples, linear, flags,
+                           status, spw, cfg.byte_cap);
+    else
+        hipLaunchKernelGGL(k_decode<0>, dim3(groups), dim3(64), lds, s, f, samples, linear, flags,
+                           status, spw, cfg.byte_cap);
+    return hipGetLastError();
+}
+
+hipError_t launch_decode_reduced(const FrameDesc &f, int shift, int average, uint16_t *rgb,
+                                 uint8_t *flags, prr_device_status *status,
+                                 const DecodeLaunch &cfg, hipStream_t s)
+{
+    uint32_t per_block = shift == 1 ? 64u : shift == 2 ? 16u : 4u;
+    size_t lds;
+    uint32_t spw = pick_spw(cfg, per_block, f.alignment, lds);
+    uint32_t T = spw / 4;
+    uint32_t groups = (f.nb_tiles + T - 1) / T;
+    if (groups == 0)
+        return hipSuccess;
+    if (shift == 1)
+        hipLaunchKernelGGL(k_reduced<1>, dim3(groups), dim3(64), lds, s, f, rgb, flags, status,
+                           spw, cfg.byte_cap, (uint32_t)average);
+    else if (shift == 2)
+        hipLaunchKernelGGL(k_reduced<2>, dim3(groups), dim3(64), lds, s, f, rgb, flags, status,
+                           spw, cfg.byte_cap, (uint32_t)average);
+    else
+        hipLaunchKernelGGL(k_reduced<3>, dim3(groups), dim3(64), lds, s, f, rgb, flags, status,
+                           spw, cfg.byte_cap, (uint32_t)average);
+    return hipGetLastError();
+}
+
+hipError_t launch_linearize(const uint16_t *samples, uint32_t stride, uint32_t width,
+                            uint32_t height, const uint16_t *curve, uint16_t *out, hipStream_t s)
+{
+    size_t total = (size_t)(width / 2) * height;
+    if (total == 0)
+        return hipSuccess;
+    uint4 lo = make_uint4(curve[0], curve[1], curve[2], curve[3]);
+    uint4 hi = make_uint4(curve[4], curve[5], curve[6], curve[7]);
+    size_t blocks = (total + 255) / 256;
+    if (blocks > 65535)
+        blocks = 65535;
+    hipLaunchKernelGGL(k_linearize, dim3((uint32_t)blocks), dim3(256), 0, s, samples, stride,
+                       width, height, lo, hi, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_develop(const DevelopDesc &d, float *r, float *g, float *b, hipStream_t s)
+{
+    if (d.width == 0 || d.height == 0)
+        return hipSuccess;
+    dim3 grid((d.width + DTW - 1) / DTW, (d.height + DTH - 1) / DTH);
+    hipLaunchKernelGGL(k_develop, grid, dim3(256), 0, s, d, r, g, b);
+    return hipGetLastError();
+}
+
+} // namespace prr

@@ -1,0 +1,110 @@
+"""GPU parity: the HIP path against the oracle on the same packets.
+
+Code values, linear values, reduced RGB and skipped tiles must be
+bit-exact; develop output within 1e-5 relative max-abs (max |gpu - oracle|
+over max |oracle|), with bit-exactness reported."""
+import numpy as np
+import pytest
+
+import proresraw_rs_amd as P
+from oracle import oracle as O
+from proresraw_rs_amd import synth as S
+
+from .helpers import config0, config1
+
+pytestmark = pytest.mark.gpu
+
+DEVELOP_RTOL = 1e-5
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def random_frames(seed, valid=False):
+    rng = S.Rng(seed)
+    for spec in S.specs(rng):
+        spec.valid_tiles = 1 if valid else 0
+        yield spec, S.random_frame(spec, rng)
+
+
+@pytest.mark.parametrize("idct", [P.Idct.Apple, P.Idct.Ffmpeg])
+def test_random_frames_decode_bit_exact(gpu, idct):
+    for i, (spec, pk) in enumerate(random_frames(11)):
+        say(f"random frame {i} {spec.width}x{spec.height} a{spec.alignment} p{spec.pattern} {idct.name}")
+        f = P.decode_frame_with(pk, idct)
+        h, s, sk, mq = O.decode_frame(pk, ffmpeg=idct == P.Idct.Ffmpeg)
+        assert f.skipped_tiles == sk
+        assert f.max_qscale == mq
+        np.testing.assert_array_equal(f.samples, s)
+        np.testing.assert_array_equal(f.linearize(), O.linearize(h, s))
+
+
+@pytest.mark.parametrize("shift", [1, 2, 3])
+@pytest.mark.parametrize("greens", [P.Greens.First, P.Greens.Average])
+def test_random_frames_reduced_bit_exact(gpu, shift, greens):
+    for i, (spec, pk) in enumerate(random_frames(8)):
+        say(f"reduced {i} {spec.width}x{spec.height} a{spec.alignment} shift {shift} {greens.name}")
+        r = P.decode_frame_reduced_with(pk, P.Scale(shift), greens)
+        h, rgb, sk, mq = O.decode_frame_reduced(pk, shift, greens == P.Greens.Average)
+        assert r.skipped_tiles == sk
+        assert r.max_qscale == mq
+        np.testing.assert_array_equal(r.samples, rgb)
+
+
+def check_develop(frame, h, samples, prm, oprm, stage):
+    r, g, b = O.develop_planar(h, samples, oprm, stage)
+    planes = P.develop_planar(frame, prm, P.Stage(stage))
+    worst = 0.0
+    exact = True
+    for got, want in zip(planes.planes, (r, g, b)):
+        scale = max(float(np.abs(want).max()), 1e-30)
+        worst = max(worst, float(np.abs(got - want).max()) / scale)
+        exact &= bool(np.array_equal(got, want))
+    say(f"  stage {stage}: relative max-abs {worst:.3g}, bit-exact {exact}")
+    assert worst <= DEVELOP_RTOL
+    return exact
+
+
+def test_config0_decode_and_develop(gpu):
+    say("config0 1920x1080 RGGB v0 a2 q15")
+    pk = S.encode_scene(config0())
+    f = P.decode_frame(pk)
+    h, s, sk, mq = O.decode_frame(pk)
+    assert f.skipped_tiles == sk == []
+    np.testing.assert_array_equal(f.samples, s)
+    prm = P.DemosaicParams.with_white_balance(f.header, f.quantizer(), 2.0, 1.6)
+    oprm = O.params(h, h.quant_matrix[0] * mq, 2.0, 1.6)
+    for stage in (0, 1, 2):
+        check_develop(f, h, s, prm, oprm, stage)
+
+
+@pytest.mark.parametrize("size", [(4096, 2160), (4104, 2168)])
+def test_config1_both_idcts(gpu, size):
+    w, hh = size
+    say(f"config1 {w}x{hh} BGGR v1 a3 q45, 2% corrupt")
+    pk = S.encode_scene(config1(w, hh))
+    for idct in (P.Idct.Apple, P.Idct.Ffmpeg):
+        f = P.decode_frame_with(pk, idct)
+        h, s, sk, mq = O.decode_frame(pk, ffmpeg=idct == P.Idct.Ffmpeg)
+        assert len(sk) > 0
+        assert f.skipped_tiles == sk
+        np.testing.assert_array_equal(f.samples, s)
+        np.testing.assert_array_equal(f.linearize(), O.linearize(h, s))
+    prm = P.DemosaicParams.new(f.header, f.quantizer())
+    oprm = O.params(h, h.quant_matrix[0] * mq)
+    check_develop(f, h, s, prm, oprm, 2)
+
+
+def test_develop_random_frames_all_stages(gpu):
+    for i, (spec, pk) in enumerate(random_frames(5, valid=True)):
+        if i % 3:
+            continue
+        say(f"develop random {i} {spec.width}x{spec.height} p{spec.pattern}")
+        f = P.decode_frame(pk)
+        h, s, sk, mq = O.decode_frame(pk)
+        q = h.quant_matrix[0] * mq
+        prm = P.DemosaicParams.with_white_balance(f.header, q, 1.8, 1.3)
+        oprm = O.params(h, q, 1.8, 1.3)
+        for stage in (0, 1, 2):
+            check_develop(f, h, s, prm, oprm, stage)
