@@ -873,3 +873,16 @@ size_t prs_encode_scene(const prs_scene *cfg, uint8_t *out, size_t cap)
     scene_release(&c);
     return total;
 }
+
+size_t prs_encode_component(const int16_t *levels, unsigned log2_blocks, uint8_t *out, size_t cap)
+{
+    bitw w;
+    memset(&w, 0, sizeof w);
+    encode_component(&w, levels, log2_blocks);
+    bw_finish(&w);
+    size_t n = w.b.len;
+    if (out && n <= cap)
+        memcpy(out, w.b.data, n);
+    free(w.b.data);
+    return n;
+}

@@ -63,6 +63,10 @@ size_t prs_frame_streams(const prs_spec *spec, uint8_t qscale, const uint8_t *co
  * entropy coding into a frame packet. Returns its length (0 on failure;
  * pass out == NULL to get an upper bound of the length). */
 size_t prs_encode_scene(const prs_scene *cfg, uint8_t *out, size_t cap);
+/* Entropy code one component: `levels` holds 64 << log2_blocks quantized
+ * coefficients, block after block in raster order (the layout
+ * decode_component produces). Returns the stream length in bytes. */
+size_t prs_encode_component(const int16_t *levels, unsigned log2_blocks, uint8_t *out, size_t cap);
 /* The mosaic of linear values the encoder coded (before compression), for
  * comparing the decoded picture with its source: width*height values. */
 void prs_scene_mosaic(const prs_scene *cfg, uint16_t *linear);

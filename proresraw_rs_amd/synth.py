@@ -72,6 +72,9 @@ def lib():
         h.prs_encode_scene.restype = ctypes.c_size_t
         h.prs_encode_scene.argtypes = [ctypes.POINTER(Scene), ctypes.c_void_p, ctypes.c_size_t]
         h.prs_scene_mosaic.argtypes = [ctypes.POINTER(Scene), ctypes.c_void_p]
+        h.prs_encode_component.restype = ctypes.c_size_t
+        h.prs_encode_component.argtypes = [ctypes.c_void_p, ctypes.c_uint, ctypes.c_void_p,
+                                           ctypes.c_size_t]
         _lib = h
     return _lib
 
@@ -185,3 +188,14 @@ def scene_mosaic(sc: Scene) -> np.ndarray:
     out = np.zeros(sc.spec.width * sc.spec.height, dtype=np.uint16)
     lib().prs_scene_mosaic(ctypes.byref(sc), out.ctypes.data)
     return out.reshape(sc.spec.height, sc.spec.width)
+
+
+def encode_component(levels, log2_blocks: int) -> bytes:
+    """Entropy code 64 << log2_blocks quantized coefficients (block after
+    block, raster order) into one component stream."""
+    lv = np.ascontiguousarray(levels, dtype=np.int16)
+    assert lv.size == 64 << log2_blocks
+    n = lib().prs_encode_component(lv.ctypes.data, log2_blocks, None, 0)
+    buf = (ctypes.c_uint8 * max(n, 1))()
+    lib().prs_encode_component(lv.ctypes.data, log2_blocks, buf, n)
+    return bytes(buf)[:n]

@@ -1,0 +1,177 @@
+"""CPU-only checks of the C ABI: the library loads and exports every
+symbol include/*.h declares, and its host-side frame structure functions
+(no device work) agree with the oracle, including every error variant."""
+import ctypes
+import os
+import re
+import struct
+
+import numpy as np
+import pytest
+
+import proresraw_rs_amd as P
+from oracle import oracle as O
+from proresraw_rs_amd import _native as N
+from proresraw_rs_amd import synth as S
+
+from .test_oracle_known_answers import body, empty_tile_packet, packet_of
+
+INCLUDE = os.path.join(os.path.dirname(os.path.dirname(__file__)), "include")
+
+
+def declared_functions():
+    names = set()
+    for fn in os.listdir(INCLUDE):
+        if fn.endswith(".h"):
+            text = open(os.path.join(INCLUDE, fn)).read()
+            text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+            names |= set(re.findall(r"\b(prr_[a-z0-9_]+)\s*\(", text))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    names = declared_functions()
+    assert len(names) >= 20
+    lib = ctypes.CDLL(N.LIB_PATH)
+    missing = [n for n in sorted(names) if not hasattr(lib, n)]
+    assert missing == []
+    # The Python binding declares exactly the header's functions.
+    assert set(N.SIGNATURES) == names
+
+
+def test_version_string():
+    assert b"gfx950" in N.lib().prr_version()
+
+
+def err_of(fn, *args):
+    try:
+        fn(*args)
+    except P.Error as e:
+        return e
+    return None
+
+
+def oracle_err(pk):
+    try:
+        O.parse_frame(pk)
+    except O.OracleError as e:
+        return e
+    return None
+
+
+def frame_info_err(pk):
+    return err_of(P.parse_frame_info, pk)
+
+
+def test_header_fields_match_the_oracle():
+    rng = S.Rng(21)
+    for spec in S.specs(rng):
+        pk = S.random_frame(spec, rng)
+        h, info = P.parse_frame_info(pk)
+        o = O.parse_header(pk)
+        assert (h.width, h.height, h.alignment, int(h.bayer_pattern)) == (o.width, o.height, o.alignment, o.bayer)
+        assert h.quant_matrix == list(o.quant_matrix) and h.curve == list(o.curve)
+        assert h.white_level == o.white_level and h.flags == o.flags
+        assert info.nb_tiles == O.lib().orc_tile_count(o.width, o.height, o.alignment)
+        assert info.tiles_per_row == O.lib().orc_tiles_per_row(o.width, o.alignment)
+        assert P.header_len(pk) == 10 + struct.unpack(">H", pk[8:10])[0] - 2
+
+
+def bad_packets():
+    """One packet per frame-level error the crate reports."""
+    out = []
+    out.append(bytes(4))                                        # Truncated prefix
+    p = bytearray(80)
+    p[:4] = struct.pack(">I", 80)
+    p[4:8] = b"xxxx"
+    out.append(bytes(p))                                        # BadTag
+    p[4:8] = b"prrf"
+    p[8:10] = struct.pack(">H", 61)
+    out.append(bytes(p))                                        # HeaderTooShort
+    p[:4] = struct.pack(">I", 81)
+    out.append(bytes(p))                                        # FrameSizeMismatch
+    p[:4] = struct.pack(">I", 0)
+    out.append(bytes(p))                                        # FrameSizeMismatch (0)
+    b = bytearray(body(0))
+    b[1] = 2
+    out.append(packet_of(bytes(b)))                             # UnsupportedVersion
+    b = bytearray(body(0))
+    b[7] = 65
+    out.append(packet_of(bytes(b)))                             # BadDimensions
+    out.append(packet_of(body(0b1010)))                         # BadAlignment
+    out.append(packet_of(body(1, b"\0" * 10)))                  # Truncated qmat
+    q = bytearray([63] * 64)
+    q[5] = 64
+    out.append(packet_of(body(1, q)))                           # BadQuantMatrix
+    rising = b"".join(struct.pack(">H", v) for v in [0, 512, 1024, 2048, 4096, 8192, 16384, 32768])
+    out.append(packet_of(body(0x10, rising)))                   # UnexpectedCurve (v0)
+    flat = b"".join(struct.pack(">H", v) for v in [0, 512, 512, 2048, 4096, 8192, 16384, 32768])
+    b = bytearray(body(0x10, flat))
+    b[1] = 1
+    out.append(packet_of(bytes(b)))                             # BadCurve
+    short = struct.pack(">I", 20) + b"prrf" + struct.pack(">H", 72) + bytes(10)
+    out.append(short)                                           # Truncated header
+    pk = bytearray(empty_tile_packet())
+    out.append(bytes(pk[:83]))                                  # Truncated size table
+    return out
+
+
+def test_frame_level_errors_match_the_oracle():
+    for i, pk in enumerate(bad_packets()):
+        o = oracle_err(pk)
+        e = frame_info_err(pk)
+        assert o is not None and e is not None, i
+        assert (e.code, e.a, e.b) == (o.code, o.a, o.b), (i, e, o)
+        if e.kind == "BadTag":
+            assert e.tag == o.tag
+        if e.kind == "BadCurve":
+            assert e.points == o.points
+
+
+def test_error_payloads_carry_the_crate_fields():
+    e = frame_info_err(bad_packets()[0])
+    assert e.kind == "Truncated" and (e.needed, e.available) == (10, 4) and e.what == "frame prefix"
+    e = frame_info_err(bad_packets()[3])
+    assert e.kind == "FrameSizeMismatch" and (e.declared, e.actual) == (81, 80)
+    e = frame_info_err(bad_packets()[9])
+    assert e.kind == "BadQuantMatrix" and (e.index, e.value) == (5, 64)
+
+
+def test_demosaic_params_and_matrix_match_the_oracle():
+    rng = S.Rng(23)
+    for spec in S.specs(rng)[:8]:
+        spec.white_level = 4095 + 256 * (spec.pattern % 2)
+        spec.gain = 1.5
+        pk = S.random_frame(spec, rng)
+        h, _ = P.parse_frame_info(pk)
+        o = O.parse_header(pk)
+        for q, wb in [(15, (2.0, 1.6)), (45 * 63, (0.0, 3.0)), (0, (1.0, 1.0))]:
+            p = P.DemosaicParams.with_white_balance(h, q, *wb)
+            po = O.params(o, q, *wb)
+            assert (p.cfa_phase, p.black) == (po.cfa_phase, po.black)
+            assert np.float32(p.one_over_range) == np.float32(po.one_over_range)
+            assert list(np.float32(p.noise)) == list(np.float32(list(po.noise)))
+            assert list(np.float32(p.clip)) == list(np.float32(list(po.clip)))
+            assert np.float32(p.black_over_range) == np.float32(po.black_over_range)
+        for exp in (P.Exposure.Normalized, P.Exposure.Reflection):
+            m = np.array(P.output_matrix(h, exp), np.float32)
+            assert np.array_equal(m, O.output_matrix(o, exp == P.Exposure.Normalized))
+
+
+def test_output_size_is_checked_before_any_device_work():
+    # A wrong caller buffer is reported like the crate's "output size" panic.
+    pk = S.encode_scene(S.scene(32, 32, 0, seed=1, version=0, alignment=1, threads=1))
+    h, info = P.parse_frame_info(pk)
+    lib = N.lib()
+    err = N.PrrError()
+    out = np.zeros(10, np.uint16)
+    hdr = N.PrrHeader()
+    buf = np.frombuffer(pk, np.uint8)
+    rc = lib.prr_decode_frame(None, buf.ctypes.data, buf.size, 0, ctypes.byref(hdr),
+                              out.ctypes.data, out.size, None, 0, None, None, ctypes.byref(err))
+    assert rc == 101  # no context: invalid argument, nothing touched
+
+
+def test_fourccs():
+    assert P.is_prores_raw_fourcc(b"aprn") and P.is_prores_raw_fourcc(b"APRH")
+    assert not P.is_prores_raw_fourcc(b"apch")
