@@ -35,6 +35,8 @@ def parse_args():
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-frames", type=int, default=64)
     return ap.parse_args()
 
 
@@ -54,6 +56,7 @@ def main():
     dev = torch.device("cuda:0")
     lib = N.lib()
     ctx = P.context()
+    stream = torch.cuda.Stream()
     sc = config1(args.width, args.height)
     packet = S.encode_scene(sc)
     hdr, info = P.parse_frame_info(packet)
@@ -71,47 +74,45 @@ def main():
     d_status = torch.zeros(4, dtype=torch.int32, device=dev)
     planes = [torch.empty(npix, dtype=torch.float32, device=dev) for _ in range(3)]
     err = N.PrrError()
+    hs = ctypes.c_void_p(stream.cuda_stream)
 
-    # Params need max_qscale: one decode, read the status back.
-    def decode(stream):
+    def decode():
         rc = lib.prr_decode_frame_device(
             ctx.handle, ctypes.byref(raw_hdr), ctypes.byref(info), d_packet.data_ptr(),
             len(packet), 0, d_samples.data_ptr(), d_linear.data_ptr(), d_flags.data_ptr(),
-            d_status.data_ptr(), stream, ctypes.byref(err))
+            d_status.data_ptr(), hs, ctypes.byref(err))
         if rc:
             raise P.Error(err)
 
-    stream = torch.cuda.current_stream().cuda_stream
-    decode(stream)
-    torch.cuda.synchronize()
+    def develop():
+        # develop(&frame, Rec2020): the noise model from the frame's
+        # max_qscale, read on the device.
+        rc = lib.prr_develop_frame_device(ctx.handle, ctypes.byref(raw_hdr), d_samples.data_ptr(),
+                                          d_status.data_ptr(), 0, 0.0, 0.0, 2,
+                                          planes[0].data_ptr(), planes[1].data_ptr(),
+                                          planes[2].data_ptr(), hs, ctypes.byref(err))
+        if rc:
+            raise P.Error(err)
+
+    def step():
+        decode()
+        develop()
+
+    step()
+    stream.synchronize()
     st = d_status.cpu().numpy().view(np.uint32)
     if st[0] != 0xFFFFFFFF or st[3]:
         raise RuntimeError(f"device status {st}")
-    max_qscale = int(st[1])
-    prm = P.DemosaicParams.new(hdr, hdr.quant_matrix[0] * max_qscale)._raw()
 
-    def develop(stream):
-        rc = lib.prr_develop_device(ctx.handle, ctypes.byref(raw_hdr), d_samples.data_ptr(),
-                                    ctypes.byref(prm), 2, planes[0].data_ptr(),
-                                    planes[1].data_ptr(), planes[2].data_ptr(), stream,
-                                    ctypes.byref(err))
-        if rc:
-            raise P.Error(err)
-
-    def step(stream):
-        decode(stream)
-        develop(stream)
-
-    # Per-kernel durations on the launch stream, eager.
-    def timed(fn, n=20):
-        s = torch.cuda.current_stream()
+    # Per-kernel durations with HIP events on the launch stream.
+    def timed(fn, n=30):
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        fn(s.cuda_stream)
-        e0.record(s)
+        fn()
+        e0.record(stream)
         for _ in range(n):
-            fn(s.cuda_stream)
-        e1.record(s)
-        torch.cuda.synchronize()
+            fn()
+        e1.record(stream)
+        stream.synchronize()
         return e0.elapsed_time(e1) / n
 
     t_decode_ms = timed(decode)
@@ -120,30 +121,26 @@ def main():
     graph = None
     if not args.no_graph:
         graph = torch.cuda.CUDAGraph()
-        s = torch.cuda.Stream()
-        s.wait_stream(torch.cuda.current_stream())
-        with torch.cuda.stream(s):
-            step(s.cuda_stream)  # warm: scratch allocated outside capture
-            torch.cuda.synchronize()
-            with torch.cuda.graph(graph, stream=s):
-                step(s.cuda_stream)
-        torch.cuda.current_stream().wait_stream(s)
-        torch.cuda.synchronize()
+        with torch.cuda.stream(stream):
+            with torch.cuda.graph(graph, stream=stream):
+                step()
+        stream.synchronize()
 
     def run_step():
         if graph is not None:
             graph.replay()
         else:
-            step(torch.cuda.current_stream().cuda_stream)
+            step()
 
-    for _ in range(args.warmup):
-        run_step()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        run_step()
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
+    with torch.cuda.stream(stream):
+        for _ in range(args.warmup):
+            run_step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            run_step()
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - t0
     ms_per_step = 1e3 * elapsed / args.steps
     value = npix * args.steps / elapsed / 1e9
 
@@ -154,7 +151,11 @@ def main():
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": 8000.0, "unit": "GB/s",
                 "frac": round(achieved / 8000.0, 4), "traffic": None,
                 "kernel": "k_develop", "kernel_ms": round(t_develop_ms, 4),
-                "decode_kernels_ms": round(t_decode_ms, 4)}
+                "bytes_per_launch": int(dev_bytes),
+                "decode_ms": round(t_decode_ms, 4),
+                "decode_tiles_per_s": round(info.nb_tiles / (t_decode_ms * 1e-3), 1)}
+
+    batched = None if args.no_batch else batched_bench(args)
 
     cpu = None
     if not args.no_cpu:
@@ -178,10 +179,78 @@ def main():
                    "packet_bytes": len(packet), "graph": graph is not None},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "batched": batched,
     }
     if cpu:
         line["gpu_over_cpu"] = round(value / cpu["value"], 1)
     print(json.dumps(line), flush=True)
+
+
+def batched_bench(args):
+    """The mixed-batch configuration: frames of 1920x1080 / 4096x2160 /
+    5760x3240 / 4032x3024, versions 0 and 1, alignments 0-4, qscale in
+    [15, 45], RGGB and BGGR; half full decode + develop(WhiteBalanced), half
+    reduced at Half / Quarter / Eighth (Greens::Average on a quarter of the
+    Half ones); one persistent launch per batch, replayed from a HIP graph.
+    16 distinct packets are encoded and cycled to fill the batch."""
+    import torch
+
+    import proresraw_rs_amd as P
+    from proresraw_rs_amd import synth as S
+    from proresraw_rs_amd.batch import Batch
+    from proresraw_rs_amd.workloads import seeded_curve, seeded_qmat
+
+    sizes = [(1920, 1080), (4096, 2160), (5760, 3240), (4032, 3024)]
+    distinct = []
+    for i in range(16):
+        w, h = sizes[i % 4]
+        version = (i // 4) % 2
+        sc = S.scene(w, h, 3 * ((i // 2) % 2), seed=1000 + i, version=version, alignment=i % 5,
+                     qscale=(15 + (i * 7) % 31, 15 + (i * 7) % 31),
+                     qmat=seeded_qmat(i) if i % 3 == 0 else None,
+                     curve=seeded_curve(i) if version == 1 else None)
+        distinct.append(S.encode_scene(sc))
+    b = Batch()
+    sensels = 0
+    n = args.batch_frames
+    n_full = 0
+    for k in range(n):
+        pk = distinct[k % 16]
+        hdr, _ = P.parse_frame_info(pk)
+        sensels += hdr.width * hdr.height
+        if k % 2 == 0:
+            b.add_full(pk, P.Idct.Apple, develop=P.Stage.WhiteBalanced)
+            n_full += 1
+        else:
+            j = k // 2
+            scale = [P.Scale.Half, P.Scale.Quarter, P.Scale.Eighth][j % 3]
+            greens = P.Greens.Average if scale == P.Scale.Half and j % 4 == 0 else P.Greens.First
+            b.add_reduced(pk, scale, greens)
+    s = torch.cuda.Stream()
+    b.run(s)
+    s.synchronize()
+    for slot in b.slots:
+        slot.check()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.stream(s):
+        with torch.cuda.graph(g, stream=s):
+            b.run(s)
+    s.synchronize()
+    reps = 5
+    with torch.cuda.stream(s):
+        g.replay()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            g.replay()
+        torch.cuda.synchronize()
+        el = (time.perf_counter() - t0) / reps
+    info = b.info()
+    return {"value": round(sensels / el / 1e9, 3), "unit": "Gsensels/s", "frames": n,
+            "full_develop_frames": n_full, "ms_per_batch": round(el * 1e3, 3),
+            "gsensels": round(sensels / 1e9, 4), "tile_groups": info["tile_groups"],
+            "persistent_grid": info["grid"], "lds_bytes": info["lds_bytes"],
+            "note": "learned demosaic not yet in this path"}
 
 
 def cpu_baseline(packet, seconds):

@@ -12,8 +12,9 @@
  *   - malformed tiles are not errors: they are reported as skipped tiles;
  *   - a prr_context owns one HIP stream and the device scratch of one host
  *     thread; use one context per thread. Host-buffer entry points are
- *     synchronous; *_device entry points enqueue on the given stream (NULL =
- *     the context's stream) and take device pointers.
+ *     synchronous on the context's stream; *_device entry points take
+ *     device pointers and enqueue on the given hipStream_t as is (NULL is
+ *     the HIP null stream; prr_context_stream gives the context's).
  */
 #ifndef PRORESRAW_AMD_H
 #define PRORESRAW_AMD_H
@@ -201,6 +202,61 @@ int prr_decode_frame_reduced_device(prr_context *ctx, const prr_header *hdr,
 int prr_develop_device(prr_context *ctx, const prr_header *hdr, const uint16_t *d_samples,
                        const prr_demosaic_params *params, int stage, float *d_r, float *d_g,
                        float *d_b, void *stream, prr_error *err);
+/* develop / develop_with (demosaic.rs:190-215) on device buffers with the
+ * frame's quantizer read on the device from `d_status` (its max_qscale),
+ * so decode and develop enqueue back to back with no readback:
+ * DemosaicParams::new (explicit_wb = 0) or ::with_white_balance(wb_red,
+ * wb_blue) (explicit_wb = 1). Planes as in prr_develop_device. */
+int prr_develop_frame_device(prr_context *ctx, const prr_header *hdr, const uint16_t *d_samples,
+                             const prr_device_status *d_status, int explicit_wb, float wb_red,
+                             float wb_blue, int stage, float *d_r, float *d_g, float *d_b,
+                             void *stream, prr_error *err);
+
+/* ---- batches (no counterpart in the crate, which decodes one frame per
+ * call): many frames of any size, version, alignment and qscale decoded
+ * in one persistent launch whose waves pull tile groups from a device
+ * queue ordered by coded byte length; each frame is a full decode (with
+ * develop or without) or a reduced decode. Results equal the per-frame
+ * entry points'. ------------------------------------------------------- */
+enum prr_batch_kind { PRR_BATCH_DECODE = 0, PRR_BATCH_DEVELOP = 1, PRR_BATCH_REDUCED = 2 };
+
+typedef struct {
+    const uint8_t *d_packet;  /* prr_decode_frame_device's packet rules */
+    size_t len;
+    int32_t kind;             /* prr_batch_kind */
+    int32_t idct;             /* DECODE / DEVELOP */
+    int32_t scale;            /* REDUCED: prr_scale */
+    int32_t greens;           /* REDUCED: prr_greens */
+    int32_t stage;            /* DEVELOP: prr_stage */
+    int32_t explicit_wb;      /* DEVELOP: 0 header gains, 1 wb_red / wb_blue */
+    float wb_red;
+    float wb_blue;
+    uint16_t *d_samples;      /* DECODE / DEVELOP: coded_width * coded_height */
+    uint16_t *d_linear;       /* DECODE / DEVELOP, optional: width * height */
+    uint16_t *d_rgb;          /* REDUCED */
+    float *d_r, *d_g, *d_b;   /* DEVELOP: planes (or interleaved in d_r) */
+    uint8_t *d_tile_flags;    /* optional, one byte per tile */
+    prr_device_status *d_status;
+} prr_batch_frame;
+
+typedef struct prr_batch prr_batch;
+int prr_batch_create(prr_context *ctx, prr_batch **out, prr_error *err);
+/* Add a frame (at most 256 per batch); `hdr`/`info` from prr_parse_frame_info. */
+int prr_batch_add(prr_batch *batch, const prr_header *hdr, const prr_frame_info *info,
+                  const prr_batch_frame *frame, prr_error *err);
+/* Launch shape: streams per workgroup (16/32/64, before adding frames),
+ * queue entries taken per atomic grab, waves per CU of the persistent
+ * launch (0 = as many as fit). */
+int prr_batch_set_config(prr_batch *batch, uint32_t streams_per_wg, uint32_t chunk,
+                         uint32_t waves_per_cu, prr_error *err);
+/* Enqueue the whole batch on `stream`. The first call after adding frames
+ * also uploads the frame table and synchronizes; later calls only launch
+ * kernels and may be captured in a HIP graph. */
+int prr_batch_run(prr_batch *batch, void *stream, prr_error *err);
+/* Tile groups, persistent grid size and LDS bytes per workgroup of the
+ * last run. */
+int prr_batch_info(prr_batch *batch, uint32_t *n_items, uint32_t *grid, uint32_t *lds_bytes);
+void prr_batch_destroy(prr_batch *batch);
 
 #ifdef __cplusplus
 }

@@ -79,6 +79,29 @@ class PrrDeviceStatus(ctypes.Structure):
     ]
 
 
+class PrrBatchFrame(ctypes.Structure):
+    _fields_ = [
+        ("d_packet", ctypes.c_void_p),
+        ("len", ctypes.c_size_t),
+        ("kind", ctypes.c_int32),
+        ("idct", ctypes.c_int32),
+        ("scale", ctypes.c_int32),
+        ("greens", ctypes.c_int32),
+        ("stage", ctypes.c_int32),
+        ("explicit_wb", ctypes.c_int32),
+        ("wb_red", ctypes.c_float),
+        ("wb_blue", ctypes.c_float),
+        ("d_samples", ctypes.c_void_p),
+        ("d_linear", ctypes.c_void_p),
+        ("d_rgb", ctypes.c_void_p),
+        ("d_r", ctypes.c_void_p),
+        ("d_g", ctypes.c_void_p),
+        ("d_b", ctypes.c_void_p),
+        ("d_tile_flags", ctypes.c_void_p),
+        ("d_status", ctypes.c_void_p),
+    ]
+
+
 E = ctypes.POINTER(PrrError)
 H = ctypes.POINTER(PrrHeader)
 FI = ctypes.POINTER(PrrFrameInfo)
@@ -113,6 +136,14 @@ SIGNATURES = {
     "prr_decode_frame_reduced_device": (i32, [c_vp, H, FI, c_vp, sz, i32, i32, c_vp, c_vp, c_vp,
                                               c_vp, E]),
     "prr_develop_device": (i32, [c_vp, H, c_vp, DP, i32, c_vp, c_vp, c_vp, c_vp, E]),
+    "prr_develop_frame_device": (i32, [c_vp, H, c_vp, c_vp, i32, ctypes.c_float, ctypes.c_float,
+                                       i32, c_vp, c_vp, c_vp, c_vp, E]),
+    "prr_batch_create": (i32, [c_vp, ctypes.POINTER(c_vp), E]),
+    "prr_batch_add": (i32, [c_vp, H, FI, ctypes.POINTER(PrrBatchFrame), E]),
+    "prr_batch_set_config": (i32, [c_vp, u32, u32, u32, E]),
+    "prr_batch_run": (i32, [c_vp, c_vp, E]),
+    "prr_batch_info": (i32, [c_vp, ctypes.POINTER(u32), ctypes.POINTER(u32), ctypes.POINTER(u32)]),
+    "prr_batch_destroy": (None, [c_vp]),
 }
 
 _lib = None

@@ -397,7 +397,7 @@ extern "C" int prr_decode_frame_device(prr_context *ctx, const prr_header *hdr,
         return rc;
     if (!d_samples || (idct != PRR_IDCT_APPLE && idct != PRR_IDCT_FFMPEG))
         return fail(err, PRR_E_INVALID_ARGUMENT, "samples or idct");
-    hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
+    hipStream_t s = (hipStream_t)stream;
     HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
     rc = ensure(ctx, ctx->offsets, sizeof(uint64_t) * ((size_t)info->nb_tiles + 1), err);
     if (rc)
@@ -424,7 +424,7 @@ extern "C" int prr_decode_frame_reduced_device(prr_context *ctx, const prr_heade
     if (!d_rgb || scale < PRR_SCALE_HALF || scale > PRR_SCALE_EIGHTH ||
         (greens != PRR_GREENS_FIRST && greens != PRR_GREENS_AVERAGE))
         return fail(err, PRR_E_INVALID_ARGUMENT, "rgb, scale or greens");
-    hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
+    hipStream_t s = (hipStream_t)stream;
     HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
     rc = ensure(ctx, ctx->offsets, sizeof(uint64_t) * ((size_t)info->nb_tiles + 1), err);
     if (rc)
@@ -443,7 +443,7 @@ extern "C" int prr_linearize_device(prr_context *ctx, const prr_header *hdr,
 {
     if (!ctx || !hdr || !d_samples || !d_out)
         return fail(err, PRR_E_INVALID_ARGUMENT, "null argument");
-    hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
+    hipStream_t s = (hipStream_t)stream;
     HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
     uint32_t stride = ((uint32_t)hdr->width + 15) & ~15u;
     HIP_TRY(launch_linearize(d_samples, stride, hdr->width, hdr->height, hdr->curve, d_out, s),
@@ -483,6 +483,48 @@ static DevelopDesc develop_desc(const prr_header *hdr, const uint16_t *d_samples
     return d;
 }
 
+/* develop(&frame, stage) with the noise model taken on the device from the
+ * frame's status (its max_qscale), so no readback is needed between decode
+ * and develop. explicit_wb selects with_white_balance(red, blue) over the
+ * header's gains. */
+static DevelopDesc develop_desc_device(const prr_header *hdr, const uint16_t *d_samples,
+                                       const prr_device_status *d_status, int explicit_wb,
+                                       float red, float blue, int stage)
+{
+    prr_demosaic_params p;
+    if (explicit_wb)
+        prr_demosaic_params_with_white_balance(hdr, 0, red, blue, &p);
+    else
+        prr_demosaic_params_new(hdr, 0, &p);
+    DevelopDesc d = develop_desc(hdr, d_samples, &p, stage);
+    d.qstatus = d_status;
+    d.qmat0 = hdr->quant_matrix[0];
+    d.range = (uint16_t)(hdr->white_level - 256u);
+    return d;
+}
+
+extern "C" int prr_develop_frame_device(prr_context *ctx, const prr_header *hdr,
+                                        const uint16_t *d_samples,
+                                        const prr_device_status *d_status, int explicit_wb,
+                                        float wb_red, float wb_blue, int stage, float *d_r,
+                                        float *d_g, float *d_b, void *stream, prr_error *err)
+{
+    if (!ctx || !hdr || !d_samples || !d_status || !d_r)
+        return fail(err, PRR_E_INVALID_ARGUMENT, "null argument");
+    if ((d_g == nullptr) != (d_b == nullptr))
+        return fail(err, PRR_E_INVALID_ARGUMENT, "planes");
+    if (stage < PRR_STAGE_DEMOSAICED || stage > PRR_STAGE_REC2020)
+        return fail(err, PRR_E_INVALID_ARGUMENT, "stage", (uint64_t)stage);
+    if (((uintptr_t)d_r | (uintptr_t)d_g | (uintptr_t)d_b) & 7)
+        return fail(err, PRR_E_INVALID_ARGUMENT, "plane alignment");
+    hipStream_t s = (hipStream_t)stream;
+    HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+    DevelopDesc d = develop_desc_device(hdr, d_samples, d_status, explicit_wb, wb_red, wb_blue,
+                                        stage);
+    HIP_TRY(launch_develop(d, d_r, d_g, d_b, s), "develop kernel");
+    return PRR_OK;
+}
+
 extern "C" int prr_develop_device(prr_context *ctx, const prr_header *hdr,
                                   const uint16_t *d_samples, const prr_demosaic_params *params,
                                   int stage, float *d_r, float *d_g, float *d_b, void *stream,
@@ -496,7 +538,7 @@ extern "C" int prr_develop_device(prr_context *ctx, const prr_header *hdr,
         return fail(err, PRR_E_INVALID_ARGUMENT, "stage", (uint64_t)stage);
     if (((uintptr_t)d_r | (uintptr_t)d_g | (uintptr_t)d_b) & 7)
         return fail(err, PRR_E_INVALID_ARGUMENT, "plane alignment");
-    hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
+    hipStream_t s = (hipStream_t)stream;
     HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
     DevelopDesc d = develop_desc(hdr, d_samples, params, stage);
     HIP_TRY(launch_develop(d, d_r, d_g, d_b, s), "develop kernel");
@@ -572,7 +614,7 @@ extern "C" int prr_decode_frame(prr_context *ctx, const uint8_t *packet, size_t 
     rc = prr_decode_frame_device(ctx, hdr, &info, (const uint8_t *)ctx->packet.ptr, len, idct,
                                  (uint16_t *)ctx->samples.ptr, nullptr,
                                  (uint8_t *)ctx->flags.ptr,
-                                 (prr_device_status *)ctx->status.ptr, nullptr, err);
+                                 (prr_device_status *)ctx->status.ptr, ctx->stream, err);
     if (rc)
         return rc;
     rc = finish_frame(ctx, &info, skipped, skipped_cap, n_skipped, max_qscale, err);
@@ -607,7 +649,7 @@ extern "C" int prr_decode_frame_reduced(prr_context *ctx, const uint8_t *packet,
     rc = prr_decode_frame_reduced_device(ctx, hdr, &info, (const uint8_t *)ctx->packet.ptr, len,
                                          scale, greens, (uint16_t *)ctx->out0.ptr,
                                          (uint8_t *)ctx->flags.ptr,
-                                         (prr_device_status *)ctx->status.ptr, nullptr, err);
+                                         (prr_device_status *)ctx->status.ptr, ctx->stream, err);
     if (rc)
         return rc;
     rc = finish_frame(ctx, &info, skipped, skipped_cap, n_skipped, max_qscale, err);
@@ -638,7 +680,7 @@ extern "C" int prr_linearize_into(prr_context *ctx, const prr_header *hdr,
                            ctx->stream),
             "hipMemcpy samples");
     rc = prr_linearize_device(ctx, hdr, (const uint16_t *)ctx->samples.ptr,
-                              (uint16_t *)ctx->out0.ptr, nullptr, err);
+                              (uint16_t *)ctx->out0.ptr, ctx->stream, err);
     if (rc)
         return rc;
     HIP_TRY(hipMemcpyAsync(out, ctx->out0.ptr, n * 2, hipMemcpyDeviceToHost, ctx->stream),
@@ -673,7 +715,7 @@ static int develop_host(prr_context *ctx, const prr_header *hdr, const uint16_t 
             "hipMemcpy samples");
     rc = prr_develop_device(ctx, hdr, (const uint16_t *)ctx->samples.ptr, params, stage,
                             (float *)ctx->out0.ptr, planar ? (float *)ctx->out1.ptr : nullptr,
-                            planar ? (float *)ctx->out2.ptr : nullptr, nullptr, err);
+                            planar ? (float *)ctx->out2.ptr : nullptr, ctx->stream, err);
     if (rc)
         return rc;
     if (planar) {
@@ -702,4 +744,203 @@ extern "C" int prr_develop_into(prr_context *ctx, const prr_header *hdr, const u
 {
     return develop_host(ctx, hdr, samples, samples_len, params, stage, rgb, nullptr, nullptr,
                         rgb_len, 0, err);
+}
+
+/* ------------------------------------------------------------------ */
+/* Batches: many frames, one persistent decode launch. */
+
+struct prr_batch {
+    prr_context *ctx = nullptr;
+    std::vector<BatchFrame> frames;
+    std::vector<BatchDevelop> jobs;
+    std::vector<size_t> offset_base; /* per frame, in u64 entries */
+    size_t offsets_total = 0;
+    DevBuf d_frames, d_jobs, d_queue, d_head, d_offsets;
+    uint32_t n_items = 0;
+    uint32_t max_w = 0, max_h = 0;
+    uint32_t spw = 32, chunk = 1, waves_per_cu = 0;
+    size_t lds = 0;
+    uint32_t grid = 0;
+    bool uploaded = false;
+};
+
+extern "C" int prr_batch_create(prr_context *ctx, prr_batch **out, prr_error *err)
+{
+    if (!ctx || !out)
+        return fail(err, PRR_E_INVALID_ARGUMENT, "null argument");
+    prr_batch *b = new prr_batch();
+    b->ctx = ctx;
+    b->spw = ctx->launch.streams_per_wg;
+    *out = b;
+    return PRR_OK;
+}
+
+extern "C" void prr_batch_destroy(prr_batch *b)
+{
+    if (!b)
+        return;
+    (void)hipSetDevice(b->ctx->device);
+    (void)hipStreamSynchronize(b->ctx->stream);
+    for (DevBuf *d : {&b->d_frames, &b->d_jobs, &b->d_queue, &b->d_head, &b->d_offsets})
+        if (d->ptr)
+            (void)hipFree(d->ptr);
+    delete b;
+}
+
+static uint32_t frame_spw(uint32_t want, uint32_t per_block, uint32_t amax, uint32_t byte_cap,
+                          uint32_t lds_limit, size_t &lds)
+{
+    uint32_t spw = want;
+    for (;;) {
+        lds = decode_lds_bytes(spw, per_block, amax, byte_cap);
+        if (lds <= lds_limit || spw <= 4)
+            return spw;
+        spw >>= 1;
+    }
+}
+
+extern "C" int prr_batch_add(prr_batch *b, const prr_header *hdr, const prr_frame_info *info,
+                             const prr_batch_frame *fr, prr_error *err)
+{
+    if (!b || !hdr || !info || !fr || !fr->d_packet || !fr->d_status)
+        return fail(err, PRR_E_INVALID_ARGUMENT, "null argument");
+    if ((uintptr_t)fr->d_packet & 3)
+        return fail(err, PRR_E_INVALID_ARGUMENT, "packet alignment");
+    if (b->frames.size() >= 256)
+        return fail(err, PRR_E_INVALID_ARGUMENT, "batch holds at most 256 frames");
+    BatchFrame bf;
+    std::memset(&bf, 0, sizeof bf);
+    bf.f = frame_desc(hdr, info, fr->d_packet, fr->len, nullptr);
+    uint32_t per_block = 64;
+    if (fr->kind == PRR_BATCH_REDUCED) {
+        if (fr->scale < PRR_SCALE_HALF || fr->scale > PRR_SCALE_EIGHTH || !fr->d_rgb)
+            return fail(err, PRR_E_INVALID_ARGUMENT, "scale or rgb");
+        bf.mode = fr->scale == PRR_SCALE_HALF ? BATCH_HALF
+                  : fr->scale == PRR_SCALE_QUARTER ? BATCH_QUARTER : BATCH_EIGHTH;
+        bf.average = fr->greens == PRR_GREENS_AVERAGE ? 1 : 0;
+        bf.rgb = fr->d_rgb;
+        per_block = fr->scale == PRR_SCALE_HALF ? 64 : fr->scale == PRR_SCALE_QUARTER ? 16 : 4;
+    } else if (fr->kind == PRR_BATCH_DECODE || fr->kind == PRR_BATCH_DEVELOP) {
+        if (!fr->d_samples || (fr->idct != PRR_IDCT_APPLE && fr->idct != PRR_IDCT_FFMPEG))
+            return fail(err, PRR_E_INVALID_ARGUMENT, "samples or idct");
+        bf.mode = fr->idct == PRR_IDCT_FFMPEG ? BATCH_FULL_FFMPEG : BATCH_FULL_APPLE;
+        bf.samples = fr->d_samples;
+        bf.linear = fr->d_linear;
+    } else {
+        return fail(err, PRR_E_INVALID_ARGUMENT, "kind", (uint64_t)fr->kind);
+    }
+    size_t lds;
+    bf.spw = frame_spw(b->spw, per_block, hdr->alignment, b->ctx->launch.byte_cap,
+                       b->ctx->launch.lds_limit, lds);
+    bf.flags = fr->d_tile_flags;
+    bf.status = fr->d_status;
+    uint32_t T = bf.spw / 4;
+    uint32_t groups = (info->nb_tiles + T - 1) / T;
+    if (groups >= (1u << 20))
+        return fail(err, PRR_E_INVALID_ARGUMENT, "too many tile groups in one frame");
+    bf.first_item = b->n_items;
+    b->n_items += groups;
+    b->offset_base.push_back(b->offsets_total);
+    b->offsets_total += (size_t)info->nb_tiles + 1;
+    b->lds = std::max(b->lds, lds);
+    b->frames.push_back(bf);
+    if (fr->kind == PRR_BATCH_DEVELOP) {
+        if (!fr->d_r || (fr->d_g == nullptr) != (fr->d_b == nullptr))
+            return fail(err, PRR_E_INVALID_ARGUMENT, "planes");
+        if (fr->stage < PRR_STAGE_DEMOSAICED || fr->stage > PRR_STAGE_REC2020)
+            return fail(err, PRR_E_INVALID_ARGUMENT, "stage");
+        BatchDevelop j;
+        j.d = develop_desc_device(hdr, fr->d_samples, fr->d_status, fr->explicit_wb, fr->wb_red,
+                                  fr->wb_blue, fr->stage);
+        j.r = fr->d_r;
+        j.g = fr->d_g;
+        j.b = fr->d_b;
+        b->jobs.push_back(j);
+        b->max_w = std::max<uint32_t>(b->max_w, hdr->width);
+        b->max_h = std::max<uint32_t>(b->max_h, hdr->height);
+    }
+    b->uploaded = false;
+    return PRR_OK;
+}
+
+extern "C" int prr_batch_set_config(prr_batch *b, uint32_t streams_per_wg, uint32_t chunk,
+                                    uint32_t waves_per_cu, prr_error *err)
+{
+    if (!b)
+        return fail(err, PRR_E_INVALID_ARGUMENT, "batch");
+    if (!b->frames.empty() && streams_per_wg && streams_per_wg != b->spw)
+        return fail(err, PRR_E_INVALID_ARGUMENT, "set streams_per_wg before adding frames");
+    if (streams_per_wg) {
+        if (streams_per_wg != 16 && streams_per_wg != 32 && streams_per_wg != 64)
+            return fail(err, PRR_E_INVALID_ARGUMENT, "streams_per_wg", streams_per_wg);
+        b->spw = streams_per_wg;
+    }
+    if (chunk)
+        b->chunk = std::min<uint32_t>(chunk, 64);
+    b->waves_per_cu = waves_per_cu;
+    b->uploaded = false;
+    return PRR_OK;
+}
+
+/* Enqueue the batch: scans, queue, persistent decode, develop. The first
+ * call after adding frames uploads the frame table (a host-to-device copy),
+ * so capture a graph only from the second call on. */
+extern "C" int prr_batch_run(prr_batch *b, void *stream, prr_error *err)
+{
+    if (!b)
+        return fail(err, PRR_E_INVALID_ARGUMENT, "batch");
+    prr_context *ctx = b->ctx;
+    hipStream_t s = (hipStream_t)stream;
+    HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+    if (b->frames.empty())
+        return PRR_OK;
+    if (!b->uploaded) {
+        int rc;
+        if ((rc = ensure(ctx, b->d_offsets, sizeof(uint64_t) * b->offsets_total, err)) ||
+            (rc = ensure(ctx, b->d_frames, sizeof(BatchFrame) * b->frames.size(), err)) ||
+            (rc = ensure(ctx, b->d_jobs, sizeof(BatchDevelop) * std::max<size_t>(b->jobs.size(), 1), err)) ||
+            (rc = ensure(ctx, b->d_queue, sizeof(uint32_t) * std::max<uint32_t>(b->n_items, 1), err)) ||
+            (rc = ensure(ctx, b->d_head, 64, err)))
+            return rc;
+        for (size_t i = 0; i < b->frames.size(); i++)
+            b->frames[i].f.offsets = (uint64_t *)b->d_offsets.ptr + b->offset_base[i];
+        HIP_TRY(hipMemcpyAsync(b->d_frames.ptr, b->frames.data(), sizeof(BatchFrame) * b->frames.size(),
+                               hipMemcpyHostToDevice, s), "hipMemcpy frames");
+        if (!b->jobs.empty())
+            HIP_TRY(hipMemcpyAsync(b->d_jobs.ptr, b->jobs.data(), sizeof(BatchDevelop) * b->jobs.size(),
+                                   hipMemcpyHostToDevice, s), "hipMemcpy jobs");
+        HIP_TRY(hipStreamSynchronize(s), "hipStreamSynchronize");
+        hipDeviceProp_t prop;
+        HIP_TRY(hipGetDeviceProperties(&prop, ctx->device), "hipGetDeviceProperties");
+        int per_cu = 0;
+        HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)batch_decode_kernel(),
+                                                             64, b->lds),
+                "hipOccupancy");
+        if (per_cu < 1)
+            per_cu = 1;
+        if (b->waves_per_cu)
+            per_cu = std::min<int>(per_cu, (int)b->waves_per_cu);
+        b->grid = (uint32_t)std::min<uint64_t>((uint64_t)prop.multiProcessorCount * per_cu,
+                                               std::max<uint32_t>(b->n_items, 1));
+        b->uploaded = true;
+    }
+    HIP_TRY(launch_batch((const BatchFrame *)b->d_frames.ptr, (uint32_t)b->frames.size(), b->n_items,
+                         (uint32_t *)b->d_queue.ptr, (uint32_t *)b->d_head.ptr,
+                         (const BatchDevelop *)b->d_jobs.ptr, (uint32_t)b->jobs.size(), b->max_w,
+                         b->max_h, b->grid, b->chunk, ctx->launch.byte_cap, b->lds, s),
+            "batch kernels");
+    return PRR_OK;
+}
+
+extern "C" int prr_batch_info(prr_batch *b, uint32_t *n_items, uint32_t *grid, uint32_t *lds_bytes)
+{
+    if (!b)
+        return PRR_E_INVALID_ARGUMENT;
+    if (n_items)
+        *n_items = b->n_items;
+    if (grid)
+        *grid = b->grid;
+    if (lds_bytes)
+        *lds_bytes = (uint32_t)b->lds;
+    return PRR_OK;
 }

@@ -45,6 +45,38 @@ struct DevelopDesc {
     int32_t has_matrix;
     float m[9];
     uint16_t curve[8];
+    /* When set, the noise model comes from the quantizer qmat0 * the
+     * frame's max_qscale, read on the device (develop(&frame, stage)). */
+    const prr_device_status *qstatus;
+    uint32_t qmat0;
+    uint32_t range;
+};
+
+enum BatchMode : uint32_t {
+    BATCH_FULL_APPLE = 0,
+    BATCH_FULL_FFMPEG = 1,
+    BATCH_HALF = 2,
+    BATCH_QUARTER = 3,
+    BATCH_EIGHTH = 4,
+};
+
+/* One frame of a batch as the persistent kernel sees it. */
+struct BatchFrame {
+    FrameDesc f;
+    uint32_t mode;
+    uint32_t average;
+    uint32_t spw;
+    uint32_t first_item;     /* index of its first tile group in the batch */
+    uint16_t *samples;       /* full decode */
+    uint16_t *linear;        /* full decode, optional */
+    uint16_t *rgb;           /* reduced decode */
+    uint8_t *flags;
+    prr_device_status *status;
+};
+
+struct BatchDevelop {
+    DevelopDesc d;
+    float *r, *g, *b;
 };
 
 /* Launch configuration of the full and reduced decode kernels. */
@@ -66,6 +98,12 @@ hipError_t launch_linearize(const uint16_t *samples, uint32_t stride, uint32_t w
                             hipStream_t s);
 hipError_t launch_develop(const DevelopDesc &d, float *r, float *g, float *b, hipStream_t s);
 hipError_t launch_status_init(prr_device_status *status, hipStream_t s);
+hipError_t launch_batch(const BatchFrame *d_frames, uint32_t n_frames, uint32_t n_items,
+                        uint32_t *d_queue, uint32_t *d_head, const BatchDevelop *d_jobs,
+                        uint32_t n_jobs, uint32_t max_w, uint32_t max_h, uint32_t grid,
+                        uint32_t chunk, uint32_t byte_cap, size_t lds, hipStream_t s);
+size_t decode_lds_bytes(uint32_t spw, uint32_t per_block, uint32_t amax, uint32_t byte_cap);
+const void *batch_decode_kernel();
 
 } // namespace prr
 

@@ -34,8 +34,8 @@ namespace prr {
 enum { TBL_AC = 0, TBL_RN = 96, TBL_LN = 128, TBL_DC = 144, TBL_N = 160 };
 /* Scans in LDS after the codebooks: 8x8 at 0, 4x4 at 64, 2x2 at 80. */
 enum { SCAN8 = 0, SCAN4 = 64, SCAN2 = 80, SCAN_N = 96 };
-/* Bytes before the coefficient area: codebooks, scans, curve. */
-enum { LDS_HEAD = 512 };
+/* Bytes before the per-tile records: codebooks, scans, curve, quant matrix. */
+enum { LDS_HEAD = 576 };
 
 __device__ const uint16_t g_codebooks[TBL_N] = {
     /* AC_CB: runs 1x000, 1x211, 2x111, 3x222, 3x122, 8x233, 2x133, 22x244, 52x355, 1x166 */
@@ -493,6 +493,7 @@ struct Group {
     uint16_t *tbl;
     uint8_t *scan;
     uint32_t *curve;     /* 9 entries */
+    uint8_t *qmat;       /* 64 entries */
     TileInfo *tiles;     /* up to 16 */
     int16_t *coefs;
     uint16_t *stage;     /* 4096 entries */
@@ -502,7 +503,7 @@ struct Group {
     bool staged;
 };
 
-/* Carve the dynamic LDS. Layout: [head 512 B][tiles 16 x 20 B][coefs][stage 8 KB][bytes]. */
+/* Carve the dynamic LDS. Layout: [head 576 B][tiles 16 x 20 B][coefs][stage 8 KB][bytes]. */
 __device__ inline Group carve(uint32_t *smem, uint32_t spw, uint32_t stride)
 {
     Group g;
@@ -510,6 +511,7 @@ __device__ inline Group carve(uint32_t *smem, uint32_t spw, uint32_t stride)
     g.tbl = (uint16_t *)base;                                /* 320 B */
     g.scan = base + TBL_N * 2;                               /* 96 B */
     g.curve = (uint32_t *)(base + TBL_N * 2 + SCAN_N);       /* 36 B, ends at 452 */
+    g.qmat = base + 512;                                     /* 64 B */
     g.tiles = (TileInfo *)(base + LDS_HEAD);                 /* 320 B */
     g.coefs = (int16_t *)(base + LDS_HEAD + 16 * sizeof(TileInfo));
     size_t coef_bytes = ((size_t)spw * stride * 2 + 15) & ~(size_t)15;
@@ -521,28 +523,44 @@ __device__ inline Group carve(uint32_t *smem, uint32_t spw, uint32_t stride)
 __host__ inline size_t group_lds_bytes(uint32_t spw, uint32_t stride, uint32_t byte_cap)
 {
     size_t coef_bytes = ((size_t)spw * stride * 2 + 15) & ~(size_t)15;
-    return LDS_HEAD + 16 * 20 + coef_bytes + 8192 + byte_cap + 16;
+    return LDS_HEAD + 16 * sizeof(TileInfo) + coef_bytes + 8192 + byte_cap + 16;
 }
 
 /* Everything up to and including the entropy decode. Each lane `l` below
  * 4T decodes stream l & 3 of the group's tile l >> 2 whose streams are
  * listed in `want` (bit s set = decode stream s). Returns the lane's
  * overrun flag. */
-__device__ inline bool front_end(const FrameDesc &f, Group &g, uint32_t spw, uint32_t stride,
-                                 uint32_t byte_cap, uint32_t log2_coefs, uint32_t want,
-                                 uint8_t *flags, prr_device_status *status)
+__device__ inline bool front_end(const FrameDesc &f, Group &g, uint32_t group, uint32_t spw,
+                                 uint32_t stride, uint32_t byte_cap, uint32_t log2_coefs,
+                                 uint32_t want, uint8_t *flags, prr_device_status *status)
 {
     const uint32_t lane = threadIdx.x;
     g.T = spw >> 2;
-    g.t0 = blockIdx.x * g.T;
+    g.t0 = group * g.T;
     g.t_end = min(g.t0 + g.T, f.nb_tiles);
 
     for (uint32_t i = lane; i < TBL_N; i += 64)
         g.tbl[i] = g_codebooks[i];
     for (uint32_t i = lane; i < SCAN_N; i += 64)
         g.scan[i] = g_scans[i];
-    if (lane < 9)
-        g.curve[lane] = lane < 8 ? f.curve[lane] : 65536u;
+    /* Selects with constant indices keep `f` in scalar registers when it
+     * is a register copy (the batch kernel's frame record). */
+    {
+        uint32_t cv = 65536u, qv = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < 8; k++)
+            cv = lane == k ? (uint32_t)f.curve[k] : cv;
+#pragma unroll
+        for (uint32_t k = 0; k < 16; k++)
+            qv = lane == k ? ((uint32_t)f.qmat[4 * k] | ((uint32_t)f.qmat[4 * k + 1] << 8) |
+                              ((uint32_t)f.qmat[4 * k + 2] << 16) |
+                              ((uint32_t)f.qmat[4 * k + 3] << 24))
+                           : qv;
+        if (lane < 9)
+            g.curve[lane] = cv;
+        if (lane < 16)
+            ((uint32_t *)g.qmat)[lane] = qv;
+    }
     uint32_t zero_words = (spw * stride) / 2;
     for (uint32_t i = lane; i < zero_words; i += 64)
         ((uint32_t *)g.coefs)[i] = 0;
@@ -636,10 +654,10 @@ __global__ void k_status_init(prr_device_status *status)
     }
 }
 
-/* Size-table walk of parse_frame (decoder.rs:104-116) as a block scan. */
-__global__ __launch_bounds__(1024) void k_scan(FrameDesc f, prr_device_status *status)
+/* Size-table walk of parse_frame (decoder.rs:104-116) as a block scan by
+ * one 1024-lane workgroup. */
+__device__ inline void scan_frame(const FrameDesc &f, prr_device_status *status, uint64_t *sums)
 {
-    __shared__ uint64_t sums[1024];
     const uint32_t n = f.nb_tiles, t = threadIdx.x;
     const uint32_t per = (n + 1023) / 1024;
     const uint32_t lo = min(t * per, n), hi = min(lo + per, n);
@@ -674,18 +692,24 @@ __global__ __launch_bounds__(1024) void k_scan(FrameDesc f, prr_device_status *s
         atomicMax(&status->max_qscale, mq);
 }
 
-/* Full decode (decoder.rs:226-307). One 64-lane workgroup per group of
- * spw/4 tiles. */
-template <int FFMPEG>
-__global__ __launch_bounds__(64) void k_decode(FrameDesc f, uint16_t *samples, uint16_t *linear,
-                                                uint8_t *flags, prr_device_status *status,
-                                                uint32_t spw, uint32_t byte_cap)
+__global__ __launch_bounds__(1024) void k_scan(FrameDesc f, prr_device_status *status)
 {
-    extern __shared__ uint32_t smem[];
+    __shared__ uint64_t sums[1024];
+    scan_frame(f, status, sums);
+}
+
+/* Full decode (decoder.rs:226-307) of tile group `group` (spw/4
+ * consecutive tiles) by one 64-lane wave. */
+template <int FFMPEG>
+__device__ inline void decode_group_full(const FrameDesc &f, uint32_t group, uint16_t *samples,
+                                         uint16_t *linear, uint8_t *flags,
+                                         prr_device_status *status, uint32_t spw,
+                                         uint32_t byte_cap, uint32_t *smem)
+{
     const uint32_t amax = f.alignment;
     const uint32_t stride = stream_stride(64, amax);
     Group g = carve(smem, spw, stride);
-    front_end(f, g, spw, stride, byte_cap, 6, 0xFu, flags, status);
+    front_end(f, g, group, spw, stride, byte_cap, 6, 0xFu, flags, status);
 
     const uint32_t lane = threadIdx.x;
     const uint32_t blocks_max = 1u << amax;
@@ -706,7 +730,7 @@ __global__ __launch_bounds__(64) void k_decode(FrameDesc f, uint16_t *samples, u
                 int32_t d[64];
 #pragma unroll
                 for (int k = 0; k < 64; k++)
-                    d[k] = dequantize(src[k << amax], (int32_t)f.qmat[k] * (int32_t)ti.qscale);
+                    d[k] = dequantize(src[k << amax], (int32_t)g.qmat[k] * (int32_t)ti.qscale);
                 uint32_t v[64];
                 if (FFMPEG)
                     ffmpeg_8x8(d, v);
@@ -753,21 +777,21 @@ __global__ __launch_bounds__(64) void k_decode(FrameDesc f, uint16_t *samples, u
     }
 }
 
-/* Reduced decode (reduced.rs:147-273): streams 3, 0, (1,) 2 as R, G, (G,) B,
- * shift 1/2/3 for Half/Quarter/Eighth. */
+/* Reduced decode (reduced.rs:147-273) of one tile group: streams 3, 0,
+ * (1,) 2 as R, G, (G,) B; shift 1/2/3 for Half/Quarter/Eighth. */
 template <uint32_t shift>
-__global__ __launch_bounds__(64) void k_reduced(FrameDesc f, uint16_t *rgb, uint8_t *flags,
-                                                 prr_device_status *status, uint32_t spw,
-                                                 uint32_t byte_cap, uint32_t average)
+__device__ inline void decode_group_reduced(const FrameDesc &f, uint32_t group, uint16_t *rgb,
+                                            uint8_t *flags, prr_device_status *status,
+                                            uint32_t spw, uint32_t byte_cap, uint32_t average,
+                                            uint32_t *smem)
 {
-    extern __shared__ uint32_t smem[];
     const uint32_t amax = f.alignment;
     constexpr uint32_t log2_coefs = shift == 1 ? 6u : shift == 2 ? 4u : 2u;
     constexpr uint32_t per_block = 1u << log2_coefs;
     const uint32_t stride = stream_stride(per_block, amax);
     Group g = carve(smem, spw, stride);
     const uint32_t want = average ? 0xFu : 0xDu;
-    front_end(f, g, spw, stride, byte_cap, log2_coefs, want, flags, status);
+    front_end(f, g, group, spw, stride, byte_cap, log2_coefs, want, flags, status);
 
     const uint32_t lane = threadIdx.x;
     const uint32_t blocks_max = 1u << amax;
@@ -790,21 +814,21 @@ __global__ __launch_bounds__(64) void k_reduced(FrameDesc f, uint16_t *rgb, uint
                     int32_t d[64];
 #pragma unroll
                     for (int k = 0; k < 64; k++)
-                        d[k] = dequantize(src[k << amax], (int32_t)f.qmat[k] * (int32_t)ti.qscale);
+                        d[k] = dequantize(src[k << amax], (int32_t)g.qmat[k] * (int32_t)ti.qscale);
                     apple_8x8(d, v);
                 } else if constexpr (shift == 2) {
                     int32_t d[16];
 #pragma unroll
                     for (int k = 0; k < 16; k++)
                         d[k] = dequantize(src[k << amax],
-                                          (int32_t)f.qmat[8 * (k >> 2) + (k & 3)] * (int32_t)ti.qscale);
+                                          (int32_t)g.qmat[8 * (k >> 2) + (k & 3)] * (int32_t)ti.qscale);
                     apple_4x4(d, v);
                 } else {
                     int32_t d[4];
                     const int qi[4] = {0, 1, 8, 9};
 #pragma unroll
                     for (int k = 0; k < 4; k++)
-                        d[k] = dequantize(src[k << amax], (int32_t)f.qmat[qi[k]] * (int32_t)ti.qscale);
+                        d[k] = dequantize(src[k << amax], (int32_t)g.qmat[qi[k]] * (int32_t)ti.qscale);
                     apple_2x2(d, v);
                 }
                 uint16_t *dst = g.stage + ((tc * 4 + s) * side) * W + b * side;
@@ -857,6 +881,24 @@ __global__ __launch_bounds__(64) void k_reduced(FrameDesc f, uint16_t *rgb, uint
         }
         __syncthreads();
     }
+}
+
+template <int FFMPEG>
+__global__ __launch_bounds__(64) void k_decode(FrameDesc f, uint16_t *samples, uint16_t *linear,
+                                                uint8_t *flags, prr_device_status *status,
+                                                uint32_t spw, uint32_t byte_cap)
+{
+    extern __shared__ uint32_t smem[];
+    decode_group_full<FFMPEG>(f, blockIdx.x, samples, linear, flags, status, spw, byte_cap, smem);
+}
+
+template <uint32_t shift>
+__global__ __launch_bounds__(64) void k_reduced(FrameDesc f, uint16_t *rgb, uint8_t *flags,
+                                                 prr_device_status *status, uint32_t spw,
+                                                 uint32_t byte_cap, uint32_t average)
+{
+    extern __shared__ uint32_t smem[];
+    decode_group_reduced<shift>(f, blockIdx.x, rgb, flags, status, spw, byte_cap, average, smem);
 }
 
 /* Frame::linearize_into (decoder.rs:197-216), two values per lane. */
@@ -959,15 +1001,37 @@ enum { DTW = 64, DTH = 32, NORM_W = DTW + 6, NORM_H = DTH + 6, GRN_W = DTW + 4, 
  * with a 3-sensel halo and green-pass values with a 1-sensel halo live in
  * LDS; each lane handles a red/blue + green pair of sensels so every lane
  * does the same work. */
-__global__ __launch_bounds__(256) void k_develop(DevelopDesc d, float *out_r, float *out_g,
-                                                 float *out_b)
+struct DevelopLds {
+    float norm[NORM_H][NORM_W];
+    float2 green[GRN_H][GRN_W];
+    uint32_t curve[9];
+};
+
+/* The noise model of DemosaicParams::with_white_balance (demosaic.rs:86-101)
+ * from the frame's quantizer, in f64 as the crate computes it. */
+__device__ inline void noise_from_quantizer(uint32_t quantizer, uint32_t range, float *noise)
 {
-    __shared__ float norm[NORM_H][NORM_W];
-    __shared__ float2 green[GRN_H][GRN_W];
-    __shared__ uint32_t curve[9];
+    const double q = (double)quantizer;
+    const double base = q * q * 5.0 + 64.0 / 3.0;
+    const double range2 = (double)range * (double)range;
+    noise[0] = (float)(base / 150994944.0);
+    noise[1] = (float)(base / (range2 * 256.0));
+    noise[2] = (float)(64.0 / 3.0 / range2);
+}
+
+__device__ inline void develop_tile(const DevelopDesc &dd, float *out_r, float *out_g,
+                                    float *out_b, uint32_t bx, uint32_t by, DevelopLds &lds)
+{
+    auto &norm = lds.norm;
+    auto &green = lds.green;
+    auto &curve = lds.curve;
+    const DevelopDesc &d = dd;
+    float noise[3] = {d.noise[0], d.noise[1], d.noise[2]};
+    if (d.qstatus)
+        noise_from_quantizer(d.qmat0 * d.qstatus->max_qscale, d.range, noise);
     const uint32_t tid = threadIdx.x;
     const uint32_t w = d.width, h = d.height;
-    const int32_t x0 = (int32_t)(blockIdx.x * DTW), y0 = (int32_t)(blockIdx.y * DTH);
+    const int32_t x0 = (int32_t)(bx * DTW), y0 = (int32_t)(by * DTH);
     if (tid < 9)
         curve[tid] = tid < 8 ? d.curve[tid] : 65536u;
     __syncthreads();
@@ -986,7 +1050,7 @@ __global__ __launch_bounds__(256) void k_develop(DevelopDesc d, float *out_r, fl
     __syncthreads();
 
     /* Green pass (green_row_impl, demosaic.rs:481-541) over pairs. */
-    const float n0 = d.noise[0], n1 = d.noise[1], n2 = d.noise[2];
+    const float n0 = noise[0], n1 = noise[1], n2 = noise[2];
     const float bs = (float)d.black * d.one_over_range;
     const uint32_t red_row = (uint32_t)(d.cfa_phase >> 1), red_col = (uint32_t)(d.cfa_phase & 1);
     for (uint32_t i = tid; i < GRN_H * (GRN_W / 2); i += 256) {
@@ -1160,6 +1224,167 @@ __global__ __launch_bounds__(256) void k_develop(DevelopDesc d, float *out_r, fl
     }
 }
 
+__global__ __launch_bounds__(256) void k_develop(DevelopDesc d, float *out_r, float *out_g,
+                                                 float *out_b)
+{
+    __shared__ DevelopLds lds;
+    develop_tile(d, out_r, out_g, out_b, blockIdx.x, blockIdx.y, lds);
+}
+
+
+/* Load a struct the whole wave agrees on into scalar registers. */
+template <class T>
+__device__ inline T load_uniform(const T *p)
+{
+    static_assert(sizeof(T) % 4 == 0, "dword-sized struct");
+    T out;
+    const uint32_t *src = (const uint32_t *)p;
+    uint32_t *dst = (uint32_t *)&out;
+#pragma unroll
+    for (uint32_t i = 0; i < sizeof(T) / 4; i++)
+        dst[i] = __builtin_amdgcn_readfirstlane(src[i]);
+    return out;
+}
+
+/* ------------------------------------------------------------------ */
+/* Batched decode: one persistent launch over tile groups of many frames. */
+
+/* Scan every frame of the batch (one workgroup per frame) and reset its
+ * status; the last workgroup also resets the queue head. */
+__global__ __launch_bounds__(1024) void k_batch_scan(const BatchFrame *frames, uint32_t *head)
+{
+    __shared__ uint64_t sums[1024];
+    const BatchFrame &bf = frames[blockIdx.x];
+    if (threadIdx.x == 0) {
+        bf.status->oob_tile = 0xFFFFFFFFu;
+        bf.status->max_qscale = 0;
+        bf.status->n_skipped = 0;
+        bf.status->overrun = 0;
+        if (blockIdx.x == 0)
+            *head = 0;
+    }
+    __syncthreads();
+    scan_frame(bf.f, bf.status, sums);
+}
+
+/* Order the tile groups of the batch by coded length, longest first: a
+ * bucket sort on a half-octave log scale of each group's byte count. The
+ * group list is item = (frame << 20) | group. */
+__global__ __launch_bounds__(1024) void k_build_queue(const BatchFrame *frames, uint32_t n_frames,
+                                                      uint32_t n_items, uint32_t *queue)
+{
+    constexpr uint32_t NB = 64;
+    __shared__ uint32_t hist[NB], base[NB];
+    __shared__ uint32_t first_item[256];
+    const uint32_t tid = threadIdx.x;
+    if (tid < NB)
+        hist[tid] = 0;
+    for (uint32_t i = tid; i < n_frames && i < 256; i += 1024)
+        first_item[i] = frames[i].first_item;
+    __syncthreads();
+    auto bucket_of = [&](uint32_t item, uint32_t &packed) {
+        uint32_t lo = 0, hi = n_frames - 1;
+        while (lo < hi) {
+            uint32_t mid = (lo + hi + 1) / 2;
+            if (first_item[mid] <= item)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        const BatchFrame &bf = frames[lo];
+        uint32_t g = item - bf.first_item;
+        uint32_t T = bf.spw >> 2;
+        uint32_t t0 = g * T, t1 = min(t0 + T, bf.f.nb_tiles);
+        uint64_t bytes = bf.f.offsets[t1] - bf.f.offsets[t0];
+        packed = (lo << 20) | g;
+        float l2 = bytes ? __log2f((float)bytes) : 0.0f;
+        int b = (int)(2.0f * l2);
+        b = b < 0 ? 0 : b > (int)NB - 1 ? (int)NB - 1 : b;
+        return (uint32_t)(NB - 1 - b);
+    };
+    for (uint32_t i = tid; i < n_items; i += 1024) {
+        uint32_t packed;
+        atomicAdd(&hist[bucket_of(i, packed)], 1u);
+    }
+    __syncthreads();
+    if (tid == 0) {
+        uint32_t run = 0;
+        for (uint32_t b = 0; b < NB; b++) {
+            base[b] = run;
+            run += hist[b];
+        }
+    }
+    __syncthreads();
+    for (uint32_t i = tid; i < n_items; i += 1024) {
+        uint32_t packed;
+        uint32_t b = bucket_of(i, packed);
+        uint32_t at = atomicAdd(&base[b], 1u);
+        queue[at] = packed;
+    }
+}
+
+/* The persistent decode: every wave pulls `chunk` queue entries at a time
+ * until the queue is empty, and decodes each entry's tile group in the
+ * frame's mode. */
+__global__ __launch_bounds__(64) void k_batch_decode(const BatchFrame *__restrict__ frames,
+                                                     const uint32_t *__restrict__ queue, uint32_t n_items,
+                                                     uint32_t *head, uint32_t chunk,
+                                                     uint32_t byte_cap)
+{
+    extern __shared__ uint32_t smem[];
+    __shared__ uint32_t s_base;
+    for (;;) {
+        if (threadIdx.x == 0)
+            s_base = atomicAdd(head, chunk);
+        __syncthreads();
+        /* Uniform across the wave: keep it in scalar registers so the
+         * frame table is read with scalar loads. */
+        const uint32_t start = __builtin_amdgcn_readfirstlane(s_base);
+        __syncthreads();
+        if (start >= n_items)
+            break;
+        const uint32_t stop = min(start + chunk, n_items);
+        for (uint32_t i = start; i < stop; i++) {
+            const uint32_t item = __builtin_amdgcn_readfirstlane(queue[i]);
+            const BatchFrame bf = load_uniform(frames + (item >> 20));
+            const uint32_t g = item & 0xFFFFFu;
+            switch (bf.mode) {
+            case BATCH_FULL_APPLE:
+                decode_group_full<0>(bf.f, g, bf.samples, bf.linear, bf.flags, bf.status, bf.spw,
+                                     byte_cap, smem);
+                break;
+            case BATCH_FULL_FFMPEG:
+                decode_group_full<1>(bf.f, g, bf.samples, bf.linear, bf.flags, bf.status, bf.spw,
+                                     byte_cap, smem);
+                break;
+            case BATCH_HALF:
+                decode_group_reduced<1>(bf.f, g, bf.rgb, bf.flags, bf.status, bf.spw, byte_cap,
+                                        bf.average, smem);
+                break;
+            case BATCH_QUARTER:
+                decode_group_reduced<2>(bf.f, g, bf.rgb, bf.flags, bf.status, bf.spw, byte_cap,
+                                        bf.average, smem);
+                break;
+            default:
+                decode_group_reduced<3>(bf.f, g, bf.rgb, bf.flags, bf.status, bf.spw, byte_cap,
+                                        bf.average, smem);
+                break;
+            }
+        }
+    }
+}
+
+/* Develop every full-decode frame of a batch: blockIdx.z picks the frame;
+ * workgroups past a smaller frame's extent return at once. */
+__global__ __launch_bounds__(256) void k_batch_develop(const BatchDevelop *jobs)
+{
+    __shared__ DevelopLds lds;
+    const BatchDevelop &j = jobs[blockIdx.z];
+    if (blockIdx.x * DTW >= j.d.width || blockIdx.y * DTH >= j.d.height)
+        return;
+    develop_tile(j.d, j.r, j.g, j.b, blockIdx.x, blockIdx.y, lds);
+}
+
 /* ------------------------------------------------------------------ */
 /* Launchers. */
 
@@ -1251,6 +1476,34 @@ hipError_t launch_develop(const DevelopDesc &d, float *r, float *g, float *b, hi
     dim3 grid((d.width + DTW - 1) / DTW, (d.height + DTH - 1) / DTH);
     hipLaunchKernelGGL(k_develop, grid, dim3(256), 0, s, d, r, g, b);
     return hipGetLastError();
+}
+
+hipError_t launch_batch(const BatchFrame *d_frames, uint32_t n_frames, uint32_t n_items,
+                        uint32_t *d_queue, uint32_t *d_head, const BatchDevelop *d_jobs,
+                        uint32_t n_jobs, uint32_t max_w, uint32_t max_h, uint32_t grid,
+                        uint32_t chunk, uint32_t byte_cap, size_t lds, hipStream_t s)
+{
+    if (n_frames == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_batch_scan, dim3(n_frames), dim3(1024), 0, s, d_frames, d_head);
+    if (n_items) {
+        hipLaunchKernelGGL(k_build_queue, dim3(1), dim3(1024), 0, s, d_frames, n_frames, n_items,
+                           d_queue);
+        hipLaunchKernelGGL(k_batch_decode, dim3(grid), dim3(64), lds, s, d_frames, d_queue,
+                           n_items, d_head, chunk, byte_cap);
+    }
+    if (n_jobs) {
+        dim3 g((max_w + DTW - 1) / DTW, (max_h + DTH - 1) / DTH, n_jobs);
+        hipLaunchKernelGGL(k_batch_develop, g, dim3(256), 0, s, d_jobs);
+    }
+    return hipGetLastError();
+}
+
+const void *batch_decode_kernel() { return (const void *)k_batch_decode; }
+
+size_t decode_lds_bytes(uint32_t spw, uint32_t per_block, uint32_t amax, uint32_t byte_cap)
+{
+    return group_lds_bytes(spw, stream_stride(per_block, amax), byte_cap);
 }
 
 } // namespace prr
