@@ -67,6 +67,8 @@ struct BatchFrame {
     uint32_t average;
     uint32_t spw;
     uint32_t first_item;     /* index of its first tile group in the batch */
+    uint32_t byte_cap;       /* packet bytes staged in LDS per group */
+    uint32_t pad_;
     uint16_t *samples;       /* full decode */
     uint16_t *linear;        /* full decode, optional */
     uint16_t *rgb;           /* reduced decode */
@@ -82,7 +84,7 @@ struct BatchDevelop {
 /* Launch configuration of the full and reduced decode kernels. */
 struct DecodeLaunch {
     uint32_t streams_per_wg; /* 16, 32 or 64 lanes decoding streams */
-    uint32_t byte_cap;       /* LDS bytes staged per workgroup */
+    uint32_t byte_cap;       /* LDS bytes staged per workgroup; 0 = sized per frame */
     uint32_t lds_limit;      /* bytes of LDS a workgroup may use */
 };
 

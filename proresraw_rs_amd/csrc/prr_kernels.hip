@@ -750,12 +750,12 @@ __device__ inline void decode_group_full(const FrameDesc &f, uint32_t group, uin
         /* Coalesced copy of the chunk's tiles: two sensels per lane and
          * store, code values and (fused) linear values. */
         const uint32_t first_tile = (c0 >> amax) / 4;
-        const uint32_t dw_per_row = W / 2;
-        const uint32_t total = tiles_per_chunk * 16 * dw_per_row;
+        const uint32_t sh_row = 3 + amax;          /* log2 of dwords per stage row */
+        const uint32_t sh_tile = 4 + sh_row;       /* log2 of dwords per stage tile */
+        const uint32_t total = tiles_per_chunk << sh_tile;
         for (uint32_t i = lane; i < total; i += 64) {
-            const uint32_t tcc = i / (16 * dw_per_row);
-            const uint32_t rem = i - tcc * 16 * dw_per_row;
-            const uint32_t row = rem / dw_per_row, dcol = rem - row * dw_per_row;
+            const uint32_t tcc = i >> sh_tile;
+            const uint32_t row = (i >> sh_row) & 15u, dcol = i & ((1u << sh_row) - 1);
             const uint32_t tli = first_tile + tcc;
             if (tli >= g.T || g.t0 + tli >= g.t_end)
                 continue;
@@ -843,11 +843,9 @@ __device__ inline void decode_group_reduced(const FrameDesc &f, uint32_t group, 
         /* Interleaved RGB rows: two values per dword store. */
         const uint32_t first_tile = (c0 >> amax) / 4;
         const uint32_t dw_per_row = (3 * W) / 2;
-        const uint32_t total = tiles_per_chunk * side * dw_per_row;
-        for (uint32_t i = lane; i < total; i += 64) {
-            const uint32_t tcc = i / (side * dw_per_row);
-            const uint32_t rem = i - tcc * side * dw_per_row;
-            const uint32_t row = rem / dw_per_row, dcol = rem - row * dw_per_row;
+        for (uint32_t tr = 0; tr < tiles_per_chunk * side; tr++)
+        for (uint32_t dcol = lane; dcol < dw_per_row; dcol += 64) {
+            const uint32_t tcc = tr / side, row = tr % side; /* side is a constant */
             const uint32_t tli = first_tile + tcc;
             if (tli >= g.T || g.t0 + tli >= g.t_end)
                 continue;
@@ -941,20 +939,19 @@ __device__ inline float dot4(float a0, float a1, float a2, float a3, float b0, f
     return fmaf(a3, b3, fmaf(a2, b2, fmaf(a0, b0, a1 * b1)));
 }
 
-/* mirror (demosaic.rs:547-557). */
+/* mirror (demosaic.rs:547-557), as selects (the branches of the crate's
+ * if / else-if chain, in order) so the four values stay in registers. */
 __device__ inline void mirror4(float *d, uint32_t pos, uint32_t len, float c)
 {
-    if (pos == 0) {
-        float a = d[1], b = d[3];
-        d[0] = a; d[1] = a; d[2] = b; d[3] = b;
-    } else if (pos == 1) {
-        d[2] = c;
-    } else if (pos == len - 1) {
-        float a = d[0], b = d[2];
-        d[0] = a; d[1] = a; d[2] = b; d[3] = b;
-    } else if (pos == len - 2) {
-        d[3] = c;
-    }
+    const bool p0 = pos == 0;
+    const bool p1 = !p0 && pos == 1;
+    const bool pl = !p0 && !p1 && pos == len - 1;
+    const bool pl2 = !p0 && !p1 && !pl && pos == len - 2;
+    const float d0 = d[0], d1 = d[1], d2 = d[2], d3 = d[3];
+    d[0] = p0 ? d1 : d0;
+    d[1] = p0 ? d1 : pl ? d0 : d1;
+    d[2] = p0 ? d3 : p1 ? c : d2;
+    d[3] = p0 ? d3 : pl ? d2 : pl2 ? c : d3;
 }
 
 /* estimate (demosaic.rs:572-604). */
@@ -995,15 +992,56 @@ __device__ inline float estimate(const float *xx, const float *gg, float cg, flo
     return fmaf(mean_mid - mean_x, weight, mean_x);
 }
 
+/* estimate (demosaic.rs:572-604) with sx[i] = sqrt(variance(xx[i])) and
+ * sg[i] = sqrt(variance(gg[i])) already known (the same operations on the
+ * same values, so the same bits). */
+__device__ inline float estimate_pre(const float *xx, const float *gg, const float *sx,
+                                     const float *sg, float cg, float cs, float off, float n0,
+                                     float n1, float n2)
+{
+    float mean_x = dot4(xx[0], xx[1], xx[2], xx[3], 0.25f, 0.25f, 0.25f, 0.25f);
+    if (!(cs < cg))
+        return mean_x;
+    bool above = true;
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+        above = above & (sx[i] < xx[i]) & (sg[i] < gg[i]);
+    if (!above)
+        return mean_x;
+    float var_x[4], lo[4], mid[4], hi[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        var_x[i] = variance(xx[i], off, n0, n1, n2);
+        lo[i] = ((cg - cs) * (xx[i] - sx[i])) / (sg[i] + gg[i]);
+        mid[i] = (cg * xx[i]) / gg[i];
+        hi[i] = ((cs + cg) * (sx[i] + xx[i])) / (gg[i] - sg[i]);
+    }
+    float spread = dot4(mid[0] - lo[0], mid[1] - lo[1], mid[2] - lo[2], mid[3] - lo[3],
+                        hi[0] - mid[0], hi[1] - mid[1], hi[2] - mid[2], hi[3] - mid[3]);
+    float mean_mid = dot4(mid[0], mid[1], mid[2], mid[3], 0.25f, 0.25f, 0.25f, 0.25f);
+    float dm0 = mid[0] - mean_mid, dm1 = mid[1] - mean_mid, dm2 = mid[2] - mean_mid,
+          dm3 = mid[3] - mean_mid;
+    float b = fmaf(spread, 0.75f, dot4(dm0, dm1, dm2, dm3, dm0, dm1, dm2, dm3));
+    float dx0 = xx[0] - mean_x, dx1 = xx[1] - mean_x, dx2 = xx[2] - mean_x, dx3 = xx[3] - mean_x;
+    float a = fmaf(dot4(var_x[0], var_x[1], var_x[2], var_x[3], 1.0f, 1.0f, 1.0f, 1.0f), 0.75f,
+                   dot4(dx0, dx1, dx2, dx3, dx0, dx1, dx2, dx3));
+    float weight = a / (b + a);
+    return fmaf(mean_mid - mean_x, weight, mean_x);
+}
+
 enum { DTW = 64, DTH = 32, NORM_W = DTW + 6, NORM_H = DTH + 6, GRN_W = DTW + 4, GRN_H = DTH + 2 };
 
 /* One 64x32 tile of output per 256-lane workgroup: normalized code values
  * with a 3-sensel halo and green-pass values with a 1-sensel halo live in
  * LDS; each lane handles a red/blue + green pair of sensels so every lane
  * does the same work. */
+/* LDS of one develop tile. Each green-pass cell holds its red/blue value,
+ * its green, and the two noise deviations every chroma estimate that reads
+ * the cell needs: sqrt(variance(rb)) and sqrt(variance(g)) at the chroma
+ * offset, computed once per cell instead of once per use. */
 struct DevelopLds {
     float norm[NORM_H][NORM_W];
-    float2 green[GRN_H][GRN_W];
+    float4 green[GRN_H][GRN_W];
     uint32_t curve[9];
 };
 
@@ -1049,9 +1087,13 @@ __device__ inline void develop_tile(const DevelopDesc &dd, float *out_r, float *
     }
     __syncthreads();
 
-    /* Green pass (green_row_impl, demosaic.rs:481-541) over pairs. */
+    /* Green pass (green_row_impl, demosaic.rs:481-541). Each lane takes one
+     * (green, red/blue) pair of a row and does the green sensel, then the
+     * red/blue one, so every lane of a wave runs the same code whatever the
+     * Bayer phase of its row. */
     const float n0 = noise[0], n1 = noise[1], n2 = noise[2];
     const float bs = (float)d.black * d.one_over_range;
+    const float coff = d.black_over_range; /* the chroma pass offset */
     const uint32_t red_row = (uint32_t)(d.cfa_phase >> 1), red_col = (uint32_t)(d.cfa_phase & 1);
     for (uint32_t i = tid; i < GRN_H * (GRN_W / 2); i += 256) {
         const int32_t ly = (int32_t)(i / (GRN_W / 2));
@@ -1060,20 +1102,26 @@ __device__ inline void develop_tile(const DevelopDesc &dd, float *out_r, float *
         if (gy < 0 || gy >= (int32_t)h)
             continue;
         const bool phase = ((uint32_t)gy & 1u) == red_row;
-        const uint32_t first = phase ? red_col : 1 - red_col;
+        const int32_t first = (int32_t)(phase ? red_col : 1 - red_col);
         const float clip = phase ? d.clip[0] : d.clip[1];
         const int32_t ny = gy - (y0 - 3);
-#pragma unroll
-        for (uint32_t k = 0; k < 2; k++) {
-            const int32_t gx = x0 - 2 + 2 * lp + (int32_t)k;
-            if (gx < x0 - 1 || gx > x0 + DTW || gx < 0 || gx >= (int32_t)w)
-                continue;
-            const int32_t nx = gx - (x0 - 3);
-            const float centre = norm[ny][nx];
-            if (((uint32_t)gx & 1u) != first) {
-                green[ly][2 * lp + k] = make_float2(0.0f, centre);
-                continue;
+        auto in_range = [&](int32_t gx) {
+            return gx >= x0 - 1 && gx <= x0 + DTW && gx >= 0 && gx < (int32_t)w;
+        };
+        {
+            const int32_t gx = x0 - 2 + 2 * lp + (1 - first);
+            if (in_range(gx)) {
+                const float centre = norm[ny][gx - (x0 - 3)];
+                green[ly][gx - (x0 - 2)] =
+                    make_float4(0.0f, centre, 0.0f, sqrtf(variance(centre, coff, n0, n1, n2)));
             }
+        }
+        const int32_t gx = x0 - 2 + 2 * lp + first;
+        if (!in_range(gx))
+            continue;
+        const int32_t nx = gx - (x0 - 3);
+        const float centre = norm[ny][nx];
+        {
             const float c = fminf(centre, clip);
             float hh[4] = {norm[ny][nx - 1], norm[ny][nx + 1], norm[ny][nx - 2], norm[ny][nx + 2]};
             float vv[4] = {norm[ny - 1][nx], norm[ny + 1][nx], norm[ny - 2][nx], norm[ny + 2][nx]};
@@ -1125,13 +1173,17 @@ __device__ inline void develop_tile(const DevelopDesc &dd, float *out_r, float *
             const float dv0 = vv[0] - gv, dv1 = vv[1] - gv, dv2 = vv[2] - c, dv3 = vv[3] - c;
             const float eh = dot4(dh0, dh1, dh2, dh3, dh0, dh1, dh2, dh3) + (var_h[0] + var_h[1]);
             const float ev = dot4(dv0, dv1, dv2, dv3, dv0, dv1, dv2, dv3) + (var_v[0] + var_v[1]);
-            green[ly][2 * lp + k] = make_float2(c, gv + (gh - gv) * ev / (eh + ev));
+            const float gout = gv + (gh - gv) * ev / (eh + ev);
+            green[ly][gx - (x0 - 2)] =
+                make_float4(c, gout, sqrtf(variance(c, coff, n0, n1, n2)),
+                            sqrtf(variance(gout, coff, n0, n1, n2)));
         }
     }
     __syncthreads();
 
     /* Chroma pass (chroma_row_with, demosaic.rs:648-688) and the output
-     * matrix (matrix_planes_impl, demosaic.rs:711-720). */
+     * matrix (matrix_planes_impl, demosaic.rs:711-720). Each lane takes the
+     * red/blue pixel and the green pixel of one pair. */
     const float off = d.black_over_range;
     for (uint32_t i = tid; i < DTH * (DTW / 2); i += 256) {
         const uint32_t ly = i / (DTW / 2), lp = i - ly * (DTW / 2);
@@ -1143,49 +1195,10 @@ __device__ inline void develop_tile(const DevelopDesc &dd, float *out_r, float *
         const uint32_t rb_col = on_red_row ? red_col : 1 - red_col;
         const int32_t gr_m = (int32_t)ym - (y0 - 1), gr_c = (int32_t)y - (y0 - 1),
                       gr_p = (int32_t)yp - (y0 - 1);
-        float res[2][3];
-        uint32_t xs[2];
-#pragma unroll
-        for (uint32_t k = 0; k < 2; k++) {
-            const uint32_t x = (uint32_t)x0 + 2 * lp + k;
-            xs[k] = x;
-            if (x >= w)
-                continue;
-            const uint32_t xm = x > 0 ? x - 1 : x + 1, xp = x < w - 1 ? x + 1 : x - 1;
-            const int32_t cm = (int32_t)xm - (x0 - 2), cc = (int32_t)x - (x0 - 2),
-                          cp = (int32_t)xp - (x0 - 2);
-            const float2 mid = green[gr_c][cc];
-            const float gval = mid.y;
-            const float sigma = sqrtf(variance(gval, off, n0, n1, n2));
-            float rr, bb;
-            if ((x & 1u) == rb_col) {
-                const float2 a = green[gr_m][cm], b = green[gr_p][cm], c2 = green[gr_m][cp],
-                             e = green[gr_p][cp];
-                const float xx[4] = {a.x, b.x, c2.x, e.x};
-                const float gg[4] = {a.y, b.y, c2.y, e.y};
-                const float est = estimate(xx, gg, gval, sigma, off, n0, n1, n2);
-                if (on_red_row) {
-                    rr = mid.x;
-                    bb = est;
-                } else {
-                    rr = est;
-                    bb = mid.x;
-                }
-            } else {
-                const float2 l = green[gr_c][cm], r = green[gr_c][cp];
-                const float2 u = green[gr_m][cc], dn = green[gr_p][cc];
-                const float hx[4] = {l.x, r.x, l.x, r.x}, hg[4] = {l.y, r.y, l.y, r.y};
-                const float vx[4] = {u.x, dn.x, u.x, dn.x}, vg[4] = {u.y, dn.y, u.y, dn.y};
-                const float eh = estimate(hx, hg, gval, sigma, off, n0, n1, n2);
-                const float ev = estimate(vx, vg, gval, sigma, off, n0, n1, n2);
-                if (on_red_row) {
-                    rr = eh;
-                    bb = ev;
-                } else {
-                    rr = ev;
-                    bb = eh;
-                }
-            }
+        const uint32_t xb = (uint32_t)x0 + 2 * lp;
+        if (xb >= w)
+            continue;
+        auto finish = [&](float rr, float gval, float bb, float *res) {
             float r = fminf(rr * d.wb[0], 1.0f);
             float g = fminf(gval, 1.0f);
             float b = fminf(bb * d.wb[1], 1.0f);
@@ -1197,10 +1210,51 @@ __device__ inline void develop_tile(const DevelopDesc &dd, float *out_r, float *
                 g = g2;
                 b = b2;
             }
-            res[k][0] = r;
-            res[k][1] = g;
-            res[k][2] = b;
+            res[0] = r;
+            res[1] = g;
+            res[2] = b;
+        };
+        float res_rb[3], res_g[3];
+        {
+            /* The red or blue pixel: diagonal neighbours. */
+            const uint32_t x = xb + rb_col;
+            const uint32_t xm = x > 0 ? x - 1 : x + 1, xp = x < w - 1 ? x + 1 : x - 1;
+            const int32_t cm = (int32_t)xm - (x0 - 2), cc = (int32_t)x - (x0 - 2),
+                          cp = (int32_t)xp - (x0 - 2);
+            const float4 mid = green[gr_c][cc];
+            const float4 a = green[gr_m][cm], b = green[gr_p][cm], c2 = green[gr_m][cp],
+                         e = green[gr_p][cp];
+            const float xx[4] = {a.x, b.x, c2.x, e.x};
+            const float gg[4] = {a.y, b.y, c2.y, e.y};
+            const float sx[4] = {a.z, b.z, c2.z, e.z};
+            const float sg[4] = {a.w, b.w, c2.w, e.w};
+            const float est = estimate_pre(xx, gg, sx, sg, mid.y, mid.w, off, n0, n1, n2);
+            finish(on_red_row ? mid.x : est, mid.y, on_red_row ? est : mid.x, res_rb);
         }
+        {
+            /* The green pixel: horizontal and vertical neighbour pairs. */
+            const uint32_t x = xb + (1 - rb_col);
+            const uint32_t xm = x > 0 ? x - 1 : x + 1, xp = x < w - 1 ? x + 1 : x - 1;
+            const int32_t cm = (int32_t)xm - (x0 - 2), cc = (int32_t)x - (x0 - 2),
+                          cp = (int32_t)xp - (x0 - 2);
+            const float4 mid = green[gr_c][cc];
+            const float4 l = green[gr_c][cm], r = green[gr_c][cp];
+            const float4 u = green[gr_m][cc], dn = green[gr_p][cc];
+            const float hx[4] = {l.x, r.x, l.x, r.x}, hg[4] = {l.y, r.y, l.y, r.y};
+            const float hsx[4] = {l.z, r.z, l.z, r.z}, hsg[4] = {l.w, r.w, l.w, r.w};
+            const float vx[4] = {u.x, dn.x, u.x, dn.x}, vg[4] = {u.y, dn.y, u.y, dn.y};
+            const float vsx[4] = {u.z, dn.z, u.z, dn.z}, vsg[4] = {u.w, dn.w, u.w, dn.w};
+            const float eh = estimate_pre(hx, hg, hsx, hsg, mid.y, mid.w, off, n0, n1, n2);
+            const float ev = estimate_pre(vx, vg, vsx, vsg, mid.y, mid.w, off, n0, n1, n2);
+            finish(on_red_row ? eh : ev, mid.y, on_red_row ? ev : eh, res_g);
+        }
+        float res[2][3];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            res[0][c] = rb_col == 0 ? res_rb[c] : res_g[c];
+            res[1][c] = rb_col == 0 ? res_g[c] : res_rb[c];
+        }
+        const uint32_t xs[2] = {xb, xb + 1};
         const size_t at = (size_t)y * w + xs[0];
         if (out_g) {
             if (xs[1] < w) {
@@ -1351,23 +1405,23 @@ __global__ __launch_bounds__(64) void k_batch_decode(const BatchFrame *__restric
             switch (bf.mode) {
             case BATCH_FULL_APPLE:
                 decode_group_full<0>(bf.f, g, bf.samples, bf.linear, bf.flags, bf.status, bf.spw,
-                                     byte_cap, smem);
+                                     bf.byte_cap, smem);
                 break;
             case BATCH_FULL_FFMPEG:
                 decode_group_full<1>(bf.f, g, bf.samples, bf.linear, bf.flags, bf.status, bf.spw,
-                                     byte_cap, smem);
+                                     bf.byte_cap, smem);
                 break;
             case BATCH_HALF:
-                decode_group_reduced<1>(bf.f, g, bf.rgb, bf.flags, bf.status, bf.spw, byte_cap,
-                                        bf.average, smem);
+                decode_group_reduced<1>(bf.f, g, bf.rgb, bf.flags, bf.status, bf.spw,
+                                        bf.byte_cap, bf.average, smem);
                 break;
             case BATCH_QUARTER:
-                decode_group_reduced<2>(bf.f, g, bf.rgb, bf.flags, bf.status, bf.spw, byte_cap,
-                                        bf.average, smem);
+                decode_group_reduced<2>(bf.f, g, bf.rgb, bf.flags, bf.status, bf.spw,
+                                        bf.byte_cap, bf.average, smem);
                 break;
             default:
-                decode_group_reduced<3>(bf.f, g, bf.rgb, bf.flags, bf.status, bf.spw, byte_cap,
-                                        bf.average, smem);
+                decode_group_reduced<3>(bf.f, g, bf.rgb, bf.flags, bf.status, bf.spw,
+                                        bf.byte_cap, bf.average, smem);
                 break;
             }
         }
