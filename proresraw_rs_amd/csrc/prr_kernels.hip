@@ -30,12 +30,16 @@ namespace prr {
 /* ------------------------------------------------------------------ */
 /* Tables (codebook.rs:26-110, idct.rs:51-87). */
 
-/* Codebook table in LDS: AC 0..94, RN 96..123, LN 128..142, DC 144..156. */
+/* The codebooks as restated (codebook.rs:26-110): AC 0..94, RN 96..123,
+ * LN 128..142, DC 144..156. */
 enum { TBL_AC = 0, TBL_RN = 96, TBL_LN = 128, TBL_DC = 144, TBL_N = 160 };
-/* Scans in LDS after the codebooks: 8x8 at 0, 4x4 at 64, 2x2 at 80. */
-enum { SCAN8 = 0, SCAN4 = 64, SCAN2 = 80, SCAN_N = 96 };
-/* Bytes before the per-tile records: codebooks, scans, curve, quant matrix. */
-enum { LDS_HEAD = 576 };
+/* The table the decode loop reads from LDS, one dword per entry: entry i
+ * below 95 holds AC_CB[i] in its low half and LN_CB[min(i, 14)] in its high
+ * half (the two codebooks a level after a run of zeros updates), then
+ * RN_CB at K_RN and DC_CB at K_DC. */
+enum { K_RN = 95, K_DC = 123, K_N = 136 };
+/* Bytes before the per-tile records: the LDS table, curve, quant matrix. */
+enum { LDS_CURVE = K_N * 4, LDS_QMAT = 592, LDS_HEAD = 656 };
 
 __device__ const uint16_t g_codebooks[TBL_N] = {
     /* AC_CB: runs 1x000, 1x211, 2x111, 3x222, 3x122, 8x233, 2x133, 22x244, 52x355, 1x166 */
@@ -57,12 +61,30 @@ __device__ const uint16_t g_codebooks[TBL_N] = {
     0x010, 0x021, 0x032, 0x033, 0x033, 0x033, 0x044, 0x044, 0x044, 0x044,
     0x044, 0x044, 0x076, 0, 0, 0};
 
-__device__ const uint8_t g_scans[SCAN_N] = {
+/* Scan orders (codebook.rs:98-110): position -> raster index of the block, or of its
+ * 4x4 or 2x2 corner. The decode loop stores coefficients by scan position;
+ * the transforms read raster index k from position kInv*.v[k]. */
+constexpr uint8_t kScan8[64] = {
     0,  8,  1,  9,  16, 24, 17, 25, 2,  10, 3,  11, 18, 26, 19, 27, 32, 40, 33, 34, 41, 48,
     56, 49, 42, 35, 43, 50, 57, 58, 51, 59, 4,  12, 5,  6,  13, 20, 28, 21, 14, 7,  15, 22,
-    29, 36, 44, 37, 30, 23, 31, 38, 45, 52, 60, 53, 46, 39, 47, 54, 61, 62, 55, 63,
-    0,  4,  1,  5,  8,  12, 9,  13, 2,  6,  3,  7,  10, 14, 11, 15, /* 4x4 */
-    0,  2,  1,  3,  0,  0,  0,  0,  0,  0,  0,  0,  0,  0,  0,  0}; /* 2x2 */
+    29, 36, 44, 37, 30, 23, 31, 38, 45, 52, 60, 53, 46, 39, 47, 54, 61, 62, 55, 63};
+constexpr uint8_t kScan4[16] = {0, 4, 1, 5, 8, 12, 9, 13, 2, 6, 3, 7, 10, 14, 11, 15};
+constexpr uint8_t kScan2[4] = {0, 2, 1, 3};
+
+struct ScanInv {
+    uint8_t v[64];
+};
+template <int N>
+constexpr ScanInv invert_scan(const uint8_t (&scan)[N])
+{
+    ScanInv r{};
+    for (int i = 0; i < N; i++)
+        r.v[scan[i]] = (uint8_t)i;
+    return r;
+}
+__device__ constexpr ScanInv kInv8 = invert_scan(kScan8);
+__device__ constexpr ScanInv kInv4 = invert_scan(kScan4);
+__device__ constexpr ScanInv kInv2 = invert_scan(kScan2);
 
 __device__ constexpr int32_t kRowTables[4][4][8] = {
     {{4096, 5352, 4096, 2217, 4096, -2217, 4096, -5352},
@@ -134,11 +156,19 @@ __device__ inline void tile_geometry(const FrameDesc &f, uint32_t t, uint32_t &x
 }
 
 /* ------------------------------------------------------------------ */
-/* Bit reader over a dword-addressed byte source (bits.rs:17-111). The
- * source is the group's bytes staged in LDS, or the packet in global
- * memory when they do not fit; positions are byte offsets into it. */
+/* Bit reader (bits.rs:17-111) over a dword-addressed byte source: the
+ * group's bytes staged in LDS, or the packet in global memory when they do
+ * not fit. Positions are byte offsets into the source. The source pointer
+ * carries its address space so loads are ds_read or global_load, never
+ * flat. */
 
-__device__ inline uint32_t load_be(const uint32_t *buf, uint32_t pos)
+#define PRR_LDS __attribute__((address_space(3)))
+#define PRR_GLOBAL __attribute__((address_space(1)))
+typedef const PRR_LDS uint32_t *lds_words;
+typedef const PRR_GLOBAL uint32_t *global_words;
+
+template <typename P>
+__device__ inline uint32_t load_be(P buf, uint32_t pos)
 {
     uint32_t i = pos >> 2;
     uint32_t lo = buf[i], hi = buf[i + 1];
@@ -156,65 +186,80 @@ __device__ inline uint32_t keep_bytes(uint32_t w, int32_t n)
     return w & (0xFFFFFFFFu << (32 - 8 * n));
 }
 
-struct Reader {
-    const uint32_t *buf;
-    uint32_t pos;
-    uint32_t end;
-    uint64_t window;
+/* Reader state. After BitReader::new every word a refill reads starts on a
+ * 4-byte boundary of the packet (new() reads up to the boundary first), so
+ * the word after the one in `nxt` is one aligned dword: `raw`, loaded one
+ * refill ahead so its latency overlaps the symbols in between. Words at
+ * or past the end of the stream read as zero (next_word, bits.rs:79-89);
+ * the one partial word, at `ql`, is kept whole in `lw`. */
+template <typename P>
+struct Bits {
+    P buf;
+    uint32_t hi, lo; /* the 64-bit window */
     int32_t extra;
-
-    /* BitReader::new (bits.rs:44-76). */
-    __device__ inline void init(const uint32_t *b, uint32_t start, uint32_t stop, uint32_t align)
-    {
-        buf = b;
-        end = stop;
-        uint32_t len = stop - start, remaining = len, address = align;
-        uint32_t first;
-        if (remaining < 4) {
-            first = keep_bytes(load_be(b, start), (int32_t)remaining);
-            pos = start + remaining;
-            address += remaining;
-            remaining = 0;
-        } else {
-            first = load_be(b, start);
-            pos = start + 4;
-            address += 4;
-            remaining -= 4;
-        }
-        uint32_t low = 0;
-        extra = 0;
-        uint32_t to_boundary = (4u - (address & 3u)) & 3u;
-        if (to_boundary != 0 && to_boundary <= remaining) {
-            low = keep_bytes(load_be(b, pos), (int32_t)to_boundary);
-            pos += to_boundary;
-            extra = 8 * (int32_t)to_boundary;
-        }
-        window = ((uint64_t)first << 32) | low;
-    }
-
-    /* next_word (bits.rs:79-89): four bytes zero-filled past the end. */
-    __device__ inline uint32_t next_word()
-    {
-        int32_t n = (int32_t)(end - pos);
-        uint32_t w = n > 0 ? keep_bytes(load_be(buf, pos), n) : 0u;
-        pos = n >= 4 ? pos + 4 : end;
-        return w;
-    }
-
-    __device__ inline uint32_t top() const { return (uint32_t)(window >> 32); }
-
-    /* consume (bits.rs:97-105). */
-    __device__ inline void consume(uint32_t len)
-    {
-        window <<= (len & 63u);
-        extra = (int32_t)((uint32_t)extra - len);
-        if (extra < 0) {
-            uint32_t shift = (uint32_t)(-(int32_t)(int8_t)(uint8_t)(uint32_t)extra) & 0x3fu;
-            window |= (uint64_t)next_word() << shift;
-            extra += 32;
-        }
-    }
+    uint32_t nxt;    /* the next word a refill ORs in */
+    uint32_t raw;    /* source dword at min(q, ql) */
+    uint32_t q;      /* byte position of the word after nxt */
+    uint32_t ql;     /* stop & ~3 */
+    uint32_t lw;     /* the partial word at ql, zero filled (0 if none) */
 };
+
+/* BitReader::new (bits.rs:44-76) over bytes [start, stop) of buf; align is
+ * the packet address of `start` mod 4. */
+template <typename P>
+__device__ inline void bits_init(Bits<P> &b, P buf, uint32_t start, uint32_t stop, uint32_t align)
+{
+    b.buf = buf;
+    uint32_t remaining = stop - start, address = align, pos, first;
+    if (remaining < 4) {
+        first = keep_bytes(load_be(buf, start), (int32_t)remaining);
+        pos = start + remaining;
+        address += remaining;
+        remaining = 0;
+    } else {
+        first = load_be(buf, start);
+        pos = start + 4;
+        address += 4;
+        remaining -= 4;
+    }
+    uint32_t low = 0;
+    b.extra = 0;
+    const uint32_t to_boundary = (4u - (address & 3u)) & 3u;
+    if (to_boundary != 0 && to_boundary <= remaining) {
+        low = keep_bytes(load_be(buf, pos), (int32_t)to_boundary);
+        pos += to_boundary;
+        b.extra = 8 * (int32_t)to_boundary;
+    }
+    b.hi = first;
+    b.lo = low;
+    /* The first next_word(). When pos is not on a boundary here (a stream
+     * too short to reach one), fewer than 4 bytes are left, so q below is
+     * past the end and every later word is zero. */
+    const int32_t n = (int32_t)(stop - pos);
+    b.nxt = n > 0 ? keep_bytes(load_be(buf, pos), n) : 0u;
+    b.q = pos + 4;
+    b.ql = stop & ~3u;
+    b.lw = (stop & 3u) ? keep_bytes(__builtin_bswap32(buf[b.ql >> 2]), (int32_t)(stop & 3u)) : 0u;
+    b.raw = buf[min(b.q, b.ql) >> 2];
+}
+
+/* consume (bits.rs:97-105). */
+template <typename P>
+__device__ inline void bits_consume(Bits<P> &b, uint32_t len)
+{
+    uint64_t w = (((uint64_t)b.hi << 32) | b.lo) << (len & 63u);
+    b.extra = (int32_t)((uint32_t)b.extra - len);
+    if (b.extra < 0) {
+        const uint32_t shift = (uint32_t)(-(int32_t)(int8_t)(uint8_t)(uint32_t)b.extra) & 0x3fu;
+        w |= (uint64_t)b.nxt << shift;
+        b.extra += 32;
+        b.nxt = b.q < b.ql ? __builtin_bswap32(b.raw) : b.q == b.ql ? b.lw : 0u;
+        b.q += 4;
+        b.raw = b.buf[min(b.q, b.ql) >> 2];
+    }
+    b.hi = (uint32_t)(w >> 32);
+    b.lo = (uint32_t)w;
+}
 
 /* codeword (bits.rs:116-134) of a non-zero view. The first branch of the
  * reference (top bit set) is the Rice branch with zero leading zeros. */
@@ -222,99 +267,88 @@ __device__ inline void codeword(uint32_t p, uint32_t cb, uint32_t &value, uint32
 {
     uint32_t lz = (uint32_t)__builtin_clz(p);
     uint32_t s = cb >> 8, e = (cb >> 4) & 15u, r = cb & 15u;
-    if (lz <= s) {
-        uint32_t tail = ((p << ((lz + 1) & 31u)) >> 1) >> ((31u - r) & 31u);
-        value = (lz << r) + tail;
-        len = r + 1 + lz;
-    } else {
-        uint32_t l = 2 * lz - s + e;
-        value = ((s + 1) << r) + (p >> ((32u - l) & 31u)) - (1u << e);
-        len = l;
-    }
+    uint32_t tail = ((p << ((lz + 1) & 31u)) >> 1) >> ((31u - r) & 31u);
+    uint32_t rice_value = (lz << r) + tail, rice_len = r + 1 + lz;
+    uint32_t l = 2 * lz - s + e;
+    uint32_t golomb_value = ((s + 1) << r) + (p >> ((32u - l) & 31u)) - (1u << e);
+    const bool rice = lz <= s;
+    value = rice ? rice_value : golomb_value;
+    len = rice ? rice_len : l;
 }
 
-enum { ST_FIRST_DC, ST_DC, ST_RUN, ST_LEVEL, ST_ZEROS, ST_AFTER_ZEROS };
+enum { S_RUN = 0, S_ZEROS = 1, S_LEVEL = 2, S_AFTER_ZEROS = 3 };
 
-/* decode_component_reduced (coefficients.rs:37-146) as a one-symbol-per-
- * iteration state machine, so lanes decoding different streams stay in
- * step. Coefficient (raster k, block b) goes to coefs[(k << kshift) + b],
- * kshift being the frame's largest log2_blocks: the frequency-major layout
- * the transform phase reads without bank conflicts. Returns true if the symbol bound was hit. */
-__device__ inline bool decode_stream(Reader &r, uint32_t log2_blocks, uint32_t log2_coefs,
-                                     uint32_t kshift, const uint8_t *scan, const uint16_t *tbl,
-                                     int16_t *coefs, uint32_t budget)
+/* decode_component_reduced (coefficients.rs:37-146) of one stream by one
+ * lane. The DC values are a short loop of their own; the AC part runs one
+ * symbol per iteration through four states (run length, zero run, level,
+ * level after zeros) with selects rather than branches, so lanes decoding
+ * different streams stay in step. Coefficient n (scan position n >> lb of
+ * block n & mask) goes to coefs[((n >> lb) << kshift) + (n & mask)]: the
+ * frequency-major layout the transform reads without bank conflicts.
+ * Symbols that store nothing write coefs[dummy], a padding slot. Returns
+ * true if the symbol budget ran out. */
+template <typename P>
+__device__ inline bool decode_stream(Bits<P> &b, uint32_t lb, uint32_t log2_coefs,
+                                     uint32_t kshift, const PRR_LDS uint32_t *tbl,
+                                     PRR_LDS int16_t *coefs, uint32_t dummy, uint32_t budget)
 {
-    const uint32_t blocks = 1u << log2_blocks;
-    const uint32_t codes = 1u << (log2_blocks + log2_coefs);
+    const uint32_t blocks = 1u << lb;
+    const uint32_t codes = 1u << (lb + log2_coefs);
     const uint32_t mask = blocks - 1;
-    uint32_t state = ST_FIRST_DC, n = 0, left = 0, sign = 0;
-    int32_t dc = 0;
-    uint32_t dc_cb = 0x064, ac_cb = 0x031, rn_cb = 0x000, ln_cb = 0x042;
-    for (uint32_t it = 0;; ++it) {
-        uint32_t top = r.top();
-        if (top == 0)
+    uint32_t sym, len;
+    if (b.hi == 0)
+        return false;
+    codeword(b.hi, 0x2BCu, sym, len);
+    bits_consume(b, len);
+    const uint32_t odd = sym & 1u;
+    int32_t dc = (int32_t)((((odd + sym) >> 1) ^ (0u - odd)) + odd);
+    coefs[0] = (int16_t)dc;
+    uint32_t dc_cb = 0x064u, sign = 0;
+    for (uint32_t blk = 1; blk < blocks; blk++) {
+        if (b.hi == 0)
             return false;
-        if (it >= budget)
-            return true;
-        uint32_t cb = state == ST_FIRST_DC ? 0x2BCu
-                      : state == ST_DC     ? dc_cb
-                      : state == ST_RUN    ? ln_cb
-                      : state == ST_ZEROS  ? rn_cb
-                                           : ac_cb;
-        uint32_t sym, len;
-        codeword(top, cb, sym, len);
-        bool level = state == ST_LEVEL || state == ST_AFTER_ZEROS;
-        r.consume(level ? len + 1 : len);
-        if (level) {
-            bool negative = ((top << (len & 31u)) >> 31) != 0;
-            uint16_t magnitude = (uint16_t)((uint16_t)sym + 1u);
-            int16_t value = (int16_t)(negative ? (uint16_t)(0u - magnitude) : magnitude);
-            uint32_t k = scan[n >> log2_blocks];
-            coefs[(k << kshift) + (n & mask)] = value;
-            ac_cb = tbl[TBL_AC + (sym < 94u ? sym : 94u)];
-            n += 1;
-            if (state == ST_AFTER_ZEROS) {
-                ln_cb = tbl[TBL_LN + (sym < 14u ? sym : 14u)];
-                if (n >= codes)
-                    return false;
-                state = ST_RUN;
-            } else {
-                if (n == codes)
-                    return false;
-                left -= 1;
-                state = left ? ST_LEVEL : ST_ZEROS;
-            }
-        } else if (state == ST_RUN) {
-            left = sym;
-            state = left ? ST_LEVEL : ST_ZEROS;
-        } else if (state == ST_ZEROS) {
-            rn_cb = tbl[TBL_RN + (sym < 27u ? sym : 27u)];
-            n = n + 1 + sym;
-            if (n >= codes)
-                return false;
-            state = ST_AFTER_ZEROS;
-        } else {
-            if (state == ST_FIRST_DC) {
-                uint32_t odd = sym & 1u;
-                dc = (int32_t)((((odd + sym) >> 1) ^ (0u - odd)) + odd);
-                state = ST_DC;
-            } else {
-                sign ^= sym & 1u;
-                uint32_t magnitude = ((sym & 1u) + sym) >> 1;
-                int32_t delta = (int32_t)(((0u - sign) ^ magnitude) + sign);
-                dc_cb = tbl[TBL_DC + (magnitude < 12u ? magnitude : 12u)];
-                sign = delta < 0 ? 1u : 0u;
-                dc = (int32_t)((uint32_t)dc + (uint32_t)delta);
-            }
-            coefs[n] = (int16_t)dc; /* raster 0 of block n */
-            n += 1;
-            if (n == blocks) {
-                if (n >= codes)
-                    return false;
-                state = ST_RUN;
-            }
-        }
+        codeword(b.hi, dc_cb, sym, len);
+        sign ^= sym & 1u;
+        const uint32_t magnitude = ((sym & 1u) + sym) >> 1;
+        const int32_t delta = (int32_t)(((0u - sign) ^ magnitude) + sign);
+        dc_cb = tbl[K_DC + min(magnitude, 12u)];
+        sign = delta < 0 ? 1u : 0u;
+        bits_consume(b, len);
+        dc = (int32_t)((uint32_t)dc + (uint32_t)delta);
+        coefs[blk] = (int16_t)dc;
     }
+
+    uint32_t n = blocks, state = S_RUN, left = 0;
+    uint32_t ac_cb = 0x031u, rn_cb = 0x000u, ln_cb = 0x042u, cb = 0x042u;
+    bool done = false;
+    /* One exit test per symbol: an empty view, the last coefficient, or
+     * the budget. */
+    for (uint32_t it = blocks; !((b.hi == 0) | done | (it >= budget)); ++it) {
+        const uint32_t top = b.hi;
+        codeword(top, cb, sym, len);
+        const bool zeros = state == S_ZEROS, level = state >= S_LEVEL;
+        const uint32_t tv = tbl[zeros ? K_RN + min(sym, 27u) : min(sym, 94u)];
+        bits_consume(b, level ? len + 1 : len);
+        const bool negative = (int32_t)(top << (len & 31u)) < 0;
+        const uint32_t magnitude = (sym + 1u) & 0xFFFFu;
+        const uint32_t value = negative ? 0u - magnitude : magnitude;
+        coefs[level ? ((n >> lb) << kshift) + (n & mask) : dummy] = (int16_t)value;
+        const uint32_t n2 = zeros ? n + 1u + sym : level ? n + 1u : n;
+        const uint32_t left2 = state == S_RUN ? sym : left - 1u;
+        const uint32_t next = zeros                    ? (uint32_t)S_AFTER_ZEROS
+                              : state == S_AFTER_ZEROS ? (uint32_t)S_RUN
+                              : left2                  ? (uint32_t)S_LEVEL
+                                                       : (uint32_t)S_ZEROS;
+        ac_cb = level ? (tv & 0xFFFFu) : ac_cb;
+        rn_cb = zeros ? tv : rn_cb;
+        ln_cb = state == S_AFTER_ZEROS ? tv >> 16 : ln_cb;
+        cb = next == S_RUN ? ln_cb : next == S_ZEROS ? rn_cb : ac_cb;
+        state = next;
+        n = n2;
+        left = left2;
+        done = n2 >= codes;
+    }
+    return b.hi != 0 && !done;
 }
 
 /* ------------------------------------------------------------------ */
@@ -490,8 +524,7 @@ __device__ __host__ inline uint32_t stream_stride(uint32_t per_block, uint32_t a
 }
 
 struct Group {
-    uint16_t *tbl;
-    uint8_t *scan;
+    uint32_t *tbl;       /* K_N entries */
     uint32_t *curve;     /* 9 entries */
     uint8_t *qmat;       /* 64 entries */
     TileInfo *tiles;     /* up to 16 */
@@ -503,16 +536,16 @@ struct Group {
     bool staged;
 };
 
-/* Carve the dynamic LDS. Layout: [head 576 B][tiles 16 x 20 B][coefs][stage 8 KB][bytes]. */
+/* Carve the dynamic LDS. Layout: [table 544 B][curve 36 B][qmat 64 B at
+ * 592][tiles 16 x 20 B at 656][coefs][stage 8 KB][bytes]. */
 __device__ inline Group carve(uint32_t *smem, uint32_t spw, uint32_t stride)
 {
     Group g;
     uint8_t *base = (uint8_t *)smem;
-    g.tbl = (uint16_t *)base;                                /* 320 B */
-    g.scan = base + TBL_N * 2;                               /* 96 B */
-    g.curve = (uint32_t *)(base + TBL_N * 2 + SCAN_N);       /* 36 B, ends at 452 */
-    g.qmat = base + 512;                                     /* 64 B */
-    g.tiles = (TileInfo *)(base + LDS_HEAD);                 /* 320 B */
+    g.tbl = (uint32_t *)base;
+    g.curve = (uint32_t *)(base + LDS_CURVE);
+    g.qmat = base + LDS_QMAT;
+    g.tiles = (TileInfo *)(base + LDS_HEAD);
     g.coefs = (int16_t *)(base + LDS_HEAD + 16 * sizeof(TileInfo));
     size_t coef_bytes = ((size_t)spw * stride * 2 + 15) & ~(size_t)15;
     g.stage = (uint16_t *)((uint8_t *)g.coefs + coef_bytes);
@@ -539,10 +572,16 @@ __device__ inline bool front_end(const FrameDesc &f, Group &g, uint32_t group, u
     g.t0 = group * g.T;
     g.t_end = min(g.t0 + g.T, f.nb_tiles);
 
-    for (uint32_t i = lane; i < TBL_N; i += 64)
-        g.tbl[i] = g_codebooks[i];
-    for (uint32_t i = lane; i < SCAN_N; i += 64)
-        g.scan[i] = g_scans[i];
+    for (uint32_t i = lane; i < K_N; i += 64) {
+        uint32_t v;
+        if (i < K_RN)
+            v = g_codebooks[TBL_AC + i] | ((uint32_t)g_codebooks[TBL_LN + min(i, 14u)] << 16);
+        else if (i < K_DC)
+            v = g_codebooks[TBL_RN + (i - K_RN)];
+        else
+            v = g_codebooks[TBL_DC + (i - K_DC)];
+        g.tbl[i] = v;
+    }
     /* Selects with constant indices keep `f` in scalar registers when it
      * is a register copy (the batch kernel's frame record). */
     {
@@ -625,16 +664,19 @@ __device__ inline bool front_end(const FrameDesc &f, Group &g, uint32_t group, u
         uint32_t align = (uint32_t)(start & 3);
         uint32_t codes = 1u << (tlog2 + log2_coefs);
         uint32_t budget = 16u * (8u * (uint32_t)(stop - start) + codes) + 1024u;
-        const uint8_t *scan = g.scan + (log2_coefs == 6 ? SCAN8 : log2_coefs == 4 ? SCAN4 : SCAN2);
-        int16_t *mine = g.coefs + lane * stride;
-        Reader rd;
+        PRR_LDS int16_t *mine = (PRR_LDS int16_t *)(g.coefs + lane * stride);
+        const PRR_LDS uint32_t *tbl = (const PRR_LDS uint32_t *)g.tbl;
+        const uint32_t dummy = (1u << log2_coefs) << f.alignment; /* first padding slot */
+        const uint32_t b0 = (uint32_t)(start - g.base4), b1 = (uint32_t)(stop - g.base4);
         if (g.staged) {
-            rd.init(g.bytes, (uint32_t)(start - g.base4), (uint32_t)(stop - g.base4), align);
+            Bits<lds_words> rd;
+            bits_init(rd, (lds_words)g.bytes, b0, b1, align);
+            overrun = decode_stream(rd, tlog2, log2_coefs, f.alignment, tbl, mine, dummy, budget);
         } else {
-            rd.init((const uint32_t *)(f.packet + g.base4), (uint32_t)(start - g.base4),
-                    (uint32_t)(stop - g.base4), align);
+            Bits<global_words> rd;
+            bits_init(rd, (global_words)(f.packet + g.base4), b0, b1, align);
+            overrun = decode_stream(rd, tlog2, log2_coefs, f.alignment, tbl, mine, dummy, budget);
         }
-        overrun = decode_stream(rd, tlog2, log2_coefs, f.alignment, scan, g.tbl, mine, budget);
     }
     if (overrun)
         atomicOr(&status->overrun, 1u);
@@ -730,7 +772,7 @@ __device__ inline void decode_group_full(const FrameDesc &f, uint32_t group, uin
                 int32_t d[64];
 #pragma unroll
                 for (int k = 0; k < 64; k++)
-                    d[k] = dequantize(src[k << amax], (int32_t)g.qmat[k] * (int32_t)ti.qscale);
+                    d[k] = dequantize(src[kInv8.v[k] << amax], (int32_t)g.qmat[k] * (int32_t)ti.qscale);
                 uint32_t v[64];
                 if (FFMPEG)
                     ffmpeg_8x8(d, v);
@@ -814,13 +856,13 @@ __device__ inline void decode_group_reduced(const FrameDesc &f, uint32_t group, 
                     int32_t d[64];
 #pragma unroll
                     for (int k = 0; k < 64; k++)
-                        d[k] = dequantize(src[k << amax], (int32_t)g.qmat[k] * (int32_t)ti.qscale);
+                        d[k] = dequantize(src[kInv8.v[k] << amax], (int32_t)g.qmat[k] * (int32_t)ti.qscale);
                     apple_8x8(d, v);
                 } else if constexpr (shift == 2) {
                     int32_t d[16];
 #pragma unroll
                     for (int k = 0; k < 16; k++)
-                        d[k] = dequantize(src[k << amax],
+                        d[k] = dequantize(src[kInv4.v[k] << amax],
                                           (int32_t)g.qmat[8 * (k >> 2) + (k & 3)] * (int32_t)ti.qscale);
                     apple_4x4(d, v);
                 } else {
@@ -828,7 +870,7 @@ __device__ inline void decode_group_reduced(const FrameDesc &f, uint32_t group, 
                     const int qi[4] = {0, 1, 8, 9};
 #pragma unroll
                     for (int k = 0; k < 4; k++)
-                        d[k] = dequantize(src[k << amax], (int32_t)g.qmat[qi[k]] * (int32_t)ti.qscale);
+                        d[k] = dequantize(src[kInv2.v[k] << amax], (int32_t)g.qmat[qi[k]] * (int32_t)ti.qscale);
                     apple_2x2(d, v);
                 }
                 uint16_t *dst = g.stage + ((tc * 4 + s) * side) * W + b * side;
