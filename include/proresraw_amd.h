@@ -127,10 +127,9 @@ void *prr_context_stream(prr_context *ctx);
 /* Version string of this library. */
 const char *prr_version(void);
 /* Decode launch shape (no counterpart in the crate, which has no launch to
- * shape): lanes decoding streams per workgroup (16, 32 or 64) and the bytes
- * of the packet staged in LDS per workgroup; 0 keeps the current value. */
-int prr_set_decode_config(prr_context *ctx, uint32_t streams_per_wg, uint32_t byte_cap,
-                          prr_error *err);
+ * shape): lanes decoding streams per workgroup (16, 32 or 64); 0 keeps the
+ * current value. */
+int prr_set_decode_config(prr_context *ctx, uint32_t streams_per_wg, prr_error *err);
 
 /* ---- host-side frame structure (no device work) ----------------------- */
 /* header_len (decoder.rs:58-73). */

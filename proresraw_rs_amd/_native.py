@@ -116,7 +116,7 @@ SIGNATURES = {
     "prr_context_destroy": (None, [c_vp]),
     "prr_context_stream": (c_vp, [c_vp]),
     "prr_version": (ctypes.c_char_p, []),
-    "prr_set_decode_config": (i32, [c_vp, u32, u32, E]),
+    "prr_set_decode_config": (i32, [c_vp, u32, E]),
     "prr_header_len": (i32, [c_vp, sz, ctypes.POINTER(sz), E]),
     "prr_parse_header": (i32, [c_vp, sz, H, E]),
     "prr_parse_frame_info": (i32, [c_vp, sz, H, FI, E]),

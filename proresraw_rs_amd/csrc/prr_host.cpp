@@ -263,7 +263,7 @@ struct DevBuf {
 struct prr_context {
     int device = 0;
     hipStream_t stream = nullptr;
-    DecodeLaunch launch{16, 0, 65536};
+    DecodeLaunch launch{16, 65536};
     DevBuf packet, offsets, status, flags, samples, out0, out1, out2;
     std::vector<uint8_t> host_flags;
 };
@@ -306,8 +306,6 @@ extern "C" int prr_context_create(int device, prr_context **out, prr_error *err)
     ctx->launch.lds_limit = (uint32_t)std::min<size_t>(prop.sharedMemPerBlock, 65536);
     if (const char *v = std::getenv("PRR_STREAMS_PER_WG"))
         ctx->launch.streams_per_wg = (uint32_t)std::atoi(v);
-    if (const char *v = std::getenv("PRR_BYTE_CAP"))
-        ctx->launch.byte_cap = (uint32_t)std::atoi(v);
     *out = ctx;
     return PRR_OK;
 }
@@ -328,10 +326,9 @@ extern "C" void prr_context_destroy(prr_context *ctx)
 
 extern "C" void *prr_context_stream(prr_context *ctx) { return ctx ? ctx->stream : nullptr; }
 
-/* Decode launch parameters: streams per workgroup (16, 32 or 64) and the
- * LDS bytes staged per workgroup; 0 keeps the current value. */
-extern "C" int prr_set_decode_config(prr_context *ctx, uint32_t streams_per_wg,
-                                     uint32_t byte_cap, prr_error *err)
+/* Decode launch parameter: streams per workgroup (16, 32 or 64); 0 keeps
+ * the current value. */
+extern "C" int prr_set_decode_config(prr_context *ctx, uint32_t streams_per_wg, prr_error *err)
 {
     if (!ctx)
         return fail(err, PRR_E_INVALID_ARGUMENT, "ctx");
@@ -340,29 +337,7 @@ extern "C" int prr_set_decode_config(prr_context *ctx, uint32_t streams_per_wg,
             return fail(err, PRR_E_INVALID_ARGUMENT, "streams_per_wg", streams_per_wg);
         ctx->launch.streams_per_wg = streams_per_wg;
     }
-    if (byte_cap) {
-        if (byte_cap < 256 || (byte_cap & 15))
-            return fail(err, PRR_E_INVALID_ARGUMENT, "byte_cap", byte_cap);
-        ctx->launch.byte_cap = byte_cap;
-    }
     return PRR_OK;
-}
-
-/* Packet bytes staged in LDS per tile group: about three times the
- * frame's mean group size (groups beyond it read the packet from global
- * memory instead), so that more groups fit on a CU. */
-static uint32_t byte_cap_for(const DecodeLaunch &cfg, const prr_frame_info *info, size_t len,
-                             uint32_t spw)
-{
-    if (cfg.byte_cap)
-        return cfg.byte_cap;
-    size_t table_end = info->header_end + 2 * (size_t)info->nb_tiles;
-    size_t payload = len > table_end ? len - table_end : 0;
-    size_t per_group = info->nb_tiles ? payload * (spw / 4) / info->nb_tiles : 0;
-    size_t cap = 3 * per_group + 64;
-    cap = std::max<size_t>(cap, 1024);
-    cap = std::min<size_t>((cap + 63) & ~(size_t)63, 16384);
-    return (uint32_t)cap;
 }
 
 /* ------------------------------------------------------------------ */
@@ -422,9 +397,7 @@ extern "C" int prr_decode_frame_device(prr_context *ctx, const prr_header *hdr,
     FrameDesc f = frame_desc(hdr, info, d_packet, len, (uint64_t *)ctx->offsets.ptr);
     HIP_TRY(launch_status_init(d_status, s), "status kernel");
     HIP_TRY(launch_scan(f, d_status, s), "scan kernel");
-    DecodeLaunch cfg = ctx->launch;
-    cfg.byte_cap = byte_cap_for(ctx->launch, info, len, cfg.streams_per_wg);
-    HIP_TRY(launch_decode(f, idct, d_samples, d_linear, d_tile_flags, d_status, cfg, s),
+    HIP_TRY(launch_decode(f, idct, d_samples, d_linear, d_tile_flags, d_status, ctx->launch, s),
             "decode kernel");
     return PRR_OK;
 }
@@ -451,9 +424,8 @@ extern "C" int prr_decode_frame_reduced_device(prr_context *ctx, const prr_heade
     FrameDesc f = frame_desc(hdr, info, d_packet, len, (uint64_t *)ctx->offsets.ptr);
     HIP_TRY(launch_status_init(d_status, s), "status kernel");
     HIP_TRY(launch_scan(f, d_status, s), "scan kernel");
-    DecodeLaunch cfg = ctx->launch;
-    cfg.byte_cap = byte_cap_for(ctx->launch, info, len, cfg.streams_per_wg);
-    HIP_TRY(launch_decode_reduced(f, scale, greens, d_rgb, d_tile_flags, d_status, cfg, s),
+    HIP_TRY(launch_decode_reduced(f, scale, greens, d_rgb, d_tile_flags, d_status, ctx->launch,
+                                  s),
             "reduced kernel");
     return PRR_OK;
 }
@@ -808,12 +780,12 @@ extern "C" void prr_batch_destroy(prr_batch *b)
     delete b;
 }
 
-static uint32_t frame_spw(uint32_t want, uint32_t per_block, uint32_t amax, uint32_t byte_cap,
-                          uint32_t lds_limit, size_t &lds)
+static uint32_t frame_spw(uint32_t want, uint32_t per_block, uint32_t amax, uint32_t lds_limit,
+                          size_t &lds)
 {
     uint32_t spw = want;
     for (;;) {
-        lds = decode_lds_bytes(spw, per_block, amax, byte_cap);
+        lds = decode_lds_bytes(spw, per_block, amax);
         if (lds <= lds_limit || spw <= 4)
             return spw;
         spw >>= 1;
@@ -851,9 +823,7 @@ extern "C" int prr_batch_add(prr_batch *b, const prr_header *hdr, const prr_fram
         return fail(err, PRR_E_INVALID_ARGUMENT, "kind", (uint64_t)fr->kind);
     }
     size_t lds;
-    bf.byte_cap = byte_cap_for(b->ctx->launch, info, fr->len, b->spw);
-    bf.spw = frame_spw(b->spw, per_block, hdr->alignment, bf.byte_cap, b->ctx->launch.lds_limit,
-                       lds);
+    bf.spw = frame_spw(b->spw, per_block, hdr->alignment, b->ctx->launch.lds_limit, lds);
     bf.flags = fr->d_tile_flags;
     bf.status = fr->d_status;
     uint32_t T = bf.spw / 4;
@@ -949,7 +919,7 @@ extern "C" int prr_batch_run(prr_batch *b, void *stream, prr_error *err)
     HIP_TRY(launch_batch((const BatchFrame *)b->d_frames.ptr, (uint32_t)b->frames.size(), b->n_items,
                          (uint32_t *)b->d_queue.ptr, (uint32_t *)b->d_head.ptr,
                          (const BatchDevelop *)b->d_jobs.ptr, (uint32_t)b->jobs.size(), b->max_w,
-                         b->max_h, b->grid, b->chunk, 0, b->lds, s),
+                         b->max_h, b->grid, b->chunk, b->lds, s),
             "batch kernels");
     return PRR_OK;
 }

@@ -67,8 +67,6 @@ struct BatchFrame {
     uint32_t average;
     uint32_t spw;
     uint32_t first_item;     /* index of its first tile group in the batch */
-    uint32_t byte_cap;       /* packet bytes staged in LDS per group */
-    uint32_t pad_;
     uint16_t *samples;       /* full decode */
     uint16_t *linear;        /* full decode, optional */
     uint16_t *rgb;           /* reduced decode */
@@ -84,7 +82,6 @@ struct BatchDevelop {
 /* Launch configuration of the full and reduced decode kernels. */
 struct DecodeLaunch {
     uint32_t streams_per_wg; /* 16, 32 or 64 lanes decoding streams */
-    uint32_t byte_cap;       /* LDS bytes staged per workgroup; 0 = sized per frame */
     uint32_t lds_limit;      /* bytes of LDS a workgroup may use */
 };
 
@@ -103,8 +100,8 @@ hipError_t launch_status_init(prr_device_status *status, hipStream_t s);
 hipError_t launch_batch(const BatchFrame *d_frames, uint32_t n_frames, uint32_t n_items,
                         uint32_t *d_queue, uint32_t *d_head, const BatchDevelop *d_jobs,
                         uint32_t n_jobs, uint32_t max_w, uint32_t max_h, uint32_t grid,
-                        uint32_t chunk, uint32_t byte_cap, size_t lds, hipStream_t s);
-size_t decode_lds_bytes(uint32_t spw, uint32_t per_block, uint32_t amax, uint32_t byte_cap);
+                        uint32_t chunk, size_t lds, hipStream_t s);
+size_t decode_lds_bytes(uint32_t spw, uint32_t per_block, uint32_t amax);
 const void *batch_decode_kernel();
 
 } // namespace prr

@@ -3,9 +3,9 @@
  *  k_status_init  per-frame status word (no hipMemsetAsync: graph safe)
  *  k_scan         size table -> tile offsets, max_qscale, first OOB tile
  *                 (parse_frame, decoder.rs:104-116)
- *  k_decode<F>    one wave per group of tiles: stage the group's bytes in
- *                 LDS, one lane per component stream runs the entropy
- *                 decoder (bits.rs, coefficients.rs) into LDS coefficient
+ *  k_decode<F>    one wave per group of tiles: one lane per component
+ *                 stream runs the entropy decoder (bits.rs,
+ *                 coefficients.rs) over the packet into LDS coefficient
  *                 blocks, then the wave runs the 8x8 inverse transform
  *                 (idct.rs Apple or FFmpeg), places each stream on its
  *                 Bayer phase (decoder.rs:272-307) through an LDS tile and
@@ -156,15 +156,15 @@ __device__ inline void tile_geometry(const FrameDesc &f, uint32_t t, uint32_t &x
 }
 
 /* ------------------------------------------------------------------ */
-/* Bit reader (bits.rs:17-111) over a dword-addressed byte source: the
- * group's bytes staged in LDS, or the packet in global memory when they do
- * not fit. Positions are byte offsets into the source. The source pointer
- * carries its address space so loads are ds_read or global_load, never
- * flat. */
+/* Bit reader (bits.rs:17-111) over the packet in global memory, addressed
+ * by dword from a 4-byte aligned base; positions are byte offsets from it.
+ * Reads go through the vector memory counter, apart from the LDS traffic
+ * of the decode loop (codebook reads, coefficient stores), so waiting for
+ * a codebook entry never waits for a refill word. The pointer carries its
+ * address space so the loads are global_load, never flat. */
 
 #define PRR_LDS __attribute__((address_space(3)))
 #define PRR_GLOBAL __attribute__((address_space(1)))
-typedef const PRR_LDS uint32_t *lds_words;
 typedef const PRR_GLOBAL uint32_t *global_words;
 
 template <typename P>
@@ -508,8 +508,8 @@ __device__ inline void apple_2x2(const int32_t *d, uint32_t *out)
 }
 
 /* ------------------------------------------------------------------ */
-/* Shared front end of the decode kernels: LDS layout, staging, tile
- * headers and the entropy decode of every stream of the group. */
+/* Shared front end of the decode kernels: LDS layout, tile headers and
+ * the entropy decode of every stream of the group. */
 
 struct TileInfo {
     uint32_t x, y, log2, qscale, valid;
@@ -530,14 +530,11 @@ struct Group {
     TileInfo *tiles;     /* up to 16 */
     int16_t *coefs;
     uint16_t *stage;     /* 4096 entries */
-    uint32_t *bytes;
     uint32_t t0, t_end, T;
-    uint64_t base4;
-    bool staged;
 };
 
 /* Carve the dynamic LDS. Layout: [table 544 B][curve 36 B][qmat 64 B at
- * 592][tiles 16 x 20 B at 656][coefs][stage 8 KB][bytes]. */
+ * 592][tiles 16 x 20 B at 656][coefs][stage 8 KB]. */
 __device__ inline Group carve(uint32_t *smem, uint32_t spw, uint32_t stride)
 {
     Group g;
@@ -549,14 +546,13 @@ __device__ inline Group carve(uint32_t *smem, uint32_t spw, uint32_t stride)
     g.coefs = (int16_t *)(base + LDS_HEAD + 16 * sizeof(TileInfo));
     size_t coef_bytes = ((size_t)spw * stride * 2 + 15) & ~(size_t)15;
     g.stage = (uint16_t *)((uint8_t *)g.coefs + coef_bytes);
-    g.bytes = (uint32_t *)((uint8_t *)g.stage + 8192);
     return g;
 }
 
-__host__ inline size_t group_lds_bytes(uint32_t spw, uint32_t stride, uint32_t byte_cap)
+__host__ inline size_t group_lds_bytes(uint32_t spw, uint32_t stride)
 {
     size_t coef_bytes = ((size_t)spw * stride * 2 + 15) & ~(size_t)15;
-    return LDS_HEAD + 16 * sizeof(TileInfo) + coef_bytes + 8192 + byte_cap + 16;
+    return LDS_HEAD + 16 * sizeof(TileInfo) + coef_bytes + 8192;
 }
 
 /* Everything up to and including the entropy decode. Each lane `l` below
@@ -564,7 +560,7 @@ __host__ inline size_t group_lds_bytes(uint32_t spw, uint32_t stride, uint32_t b
  * listed in `want` (bit s set = decode stream s). Returns the lane's
  * overrun flag. */
 __device__ inline bool front_end(const FrameDesc &f, Group &g, uint32_t group, uint32_t spw,
-                                 uint32_t stride, uint32_t byte_cap, uint32_t log2_coefs,
+                                 uint32_t stride, uint32_t log2_coefs,
                                  uint32_t want, uint8_t *flags, prr_device_status *status)
 {
     const uint32_t lane = threadIdx.x;
@@ -603,21 +599,6 @@ __device__ inline bool front_end(const FrameDesc &f, Group &g, uint32_t group, u
     uint32_t zero_words = (spw * stride) / 2;
     for (uint32_t i = lane; i < zero_words; i += 64)
         ((uint32_t *)g.coefs)[i] = 0;
-
-    uint64_t r0 = f.offsets[g.t0], r1 = f.offsets[g.t_end];
-    if (r1 > f.len)
-        r1 = f.len;
-    if (r0 > r1)
-        r0 = r1;
-    g.base4 = r0 & ~(uint64_t)3;
-    uint64_t nbytes = r1 - g.base4;
-    g.staged = nbytes + 8 <= byte_cap;
-    if (g.staged) {
-        const uint32_t *src = (const uint32_t *)(f.packet + g.base4);
-        uint32_t nd = (uint32_t)((nbytes + 3) >> 2) + 2;
-        for (uint32_t i = lane; i < nd; i += 64)
-            g.bytes[i] = src[i];
-    }
 
     /* Tile::parse (tile.rs:126-154), by the four lanes of each tile. */
     const uint32_t tl = lane >> 2, s = lane & 3;
@@ -667,16 +648,11 @@ __device__ inline bool front_end(const FrameDesc &f, Group &g, uint32_t group, u
         PRR_LDS int16_t *mine = (PRR_LDS int16_t *)(g.coefs + lane * stride);
         const PRR_LDS uint32_t *tbl = (const PRR_LDS uint32_t *)g.tbl;
         const uint32_t dummy = (1u << log2_coefs) << f.alignment; /* first padding slot */
-        const uint32_t b0 = (uint32_t)(start - g.base4), b1 = (uint32_t)(stop - g.base4);
-        if (g.staged) {
-            Bits<lds_words> rd;
-            bits_init(rd, (lds_words)g.bytes, b0, b1, align);
-            overrun = decode_stream(rd, tlog2, log2_coefs, f.alignment, tbl, mine, dummy, budget);
-        } else {
-            Bits<global_words> rd;
-            bits_init(rd, (global_words)(f.packet + g.base4), b0, b1, align);
-            overrun = decode_stream(rd, tlog2, log2_coefs, f.alignment, tbl, mine, dummy, budget);
-        }
+        const uint64_t base = start & ~(uint64_t)3;
+        Bits<global_words> rd;
+        bits_init(rd, (global_words)(f.packet + base), (uint32_t)(start - base),
+                  (uint32_t)(stop - base), align);
+        overrun = decode_stream(rd, tlog2, log2_coefs, f.alignment, tbl, mine, dummy, budget);
     }
     if (overrun)
         atomicOr(&status->overrun, 1u);
@@ -746,12 +722,12 @@ template <int FFMPEG>
 __device__ inline void decode_group_full(const FrameDesc &f, uint32_t group, uint16_t *samples,
                                          uint16_t *linear, uint8_t *flags,
                                          prr_device_status *status, uint32_t spw,
-                                         uint32_t byte_cap, uint32_t *smem)
+                                         uint32_t *smem)
 {
     const uint32_t amax = f.alignment;
     const uint32_t stride = stream_stride(64, amax);
     Group g = carve(smem, spw, stride);
-    front_end(f, g, group, spw, stride, byte_cap, 6, 0xFu, flags, status);
+    front_end(f, g, group, spw, stride, 6, 0xFu, flags, status);
 
     const uint32_t lane = threadIdx.x;
     const uint32_t blocks_max = 1u << amax;
@@ -795,6 +771,7 @@ __device__ inline void decode_group_full(const FrameDesc &f, uint32_t group, uin
         const uint32_t sh_row = 3 + amax;          /* log2 of dwords per stage row */
         const uint32_t sh_tile = 4 + sh_row;       /* log2 of dwords per stage tile */
         const uint32_t total = tiles_per_chunk << sh_tile;
+#pragma unroll 8
         for (uint32_t i = lane; i < total; i += 64) {
             const uint32_t tcc = i >> sh_tile;
             const uint32_t row = (i >> sh_row) & 15u, dcol = i & ((1u << sh_row) - 1);
@@ -824,7 +801,7 @@ __device__ inline void decode_group_full(const FrameDesc &f, uint32_t group, uin
 template <uint32_t shift>
 __device__ inline void decode_group_reduced(const FrameDesc &f, uint32_t group, uint16_t *rgb,
                                             uint8_t *flags, prr_device_status *status,
-                                            uint32_t spw, uint32_t byte_cap, uint32_t average,
+                                            uint32_t spw, uint32_t average,
                                             uint32_t *smem)
 {
     const uint32_t amax = f.alignment;
@@ -833,7 +810,7 @@ __device__ inline void decode_group_reduced(const FrameDesc &f, uint32_t group, 
     const uint32_t stride = stream_stride(per_block, amax);
     Group g = carve(smem, spw, stride);
     const uint32_t want = average ? 0xFu : 0xDu;
-    front_end(f, g, group, spw, stride, byte_cap, log2_coefs, want, flags, status);
+    front_end(f, g, group, spw, stride, log2_coefs, want, flags, status);
 
     const uint32_t lane = threadIdx.x;
     const uint32_t blocks_max = 1u << amax;
@@ -926,19 +903,19 @@ __device__ inline void decode_group_reduced(const FrameDesc &f, uint32_t group, 
 template <int FFMPEG>
 __global__ __launch_bounds__(64) void k_decode(FrameDesc f, uint16_t *samples, uint16_t *linear,
                                                 uint8_t *flags, prr_device_status *status,
-                                                uint32_t spw, uint32_t byte_cap)
+                                                uint32_t spw)
 {
     extern __shared__ uint32_t smem[];
-    decode_group_full<FFMPEG>(f, blockIdx.x, samples, linear, flags, status, spw, byte_cap, smem);
+    decode_group_full<FFMPEG>(f, blockIdx.x, samples, linear, flags, status, spw, smem);
 }
 
 template <uint32_t shift>
 __global__ __launch_bounds__(64) void k_reduced(FrameDesc f, uint16_t *rgb, uint8_t *flags,
                                                  prr_device_status *status, uint32_t spw,
-                                                 uint32_t byte_cap, uint32_t average)
+                                                 uint32_t average)
 {
     extern __shared__ uint32_t smem[];
-    decode_group_reduced<shift>(f, blockIdx.x, rgb, flags, status, spw, byte_cap, average, smem);
+    decode_group_reduced<shift>(f, blockIdx.x, rgb, flags, status, spw, average, smem);
 }
 
 /* Frame::linearize_into (decoder.rs:197-216), two values per lane. */
@@ -1424,8 +1401,7 @@ __global__ __launch_bounds__(1024) void k_build_queue(const BatchFrame *frames, 
  * frame's mode. */
 __global__ __launch_bounds__(64) void k_batch_decode(const BatchFrame *__restrict__ frames,
                                                      const uint32_t *__restrict__ queue, uint32_t n_items,
-                                                     uint32_t *head, uint32_t chunk,
-                                                     uint32_t byte_cap)
+                                                     uint32_t *head, uint32_t chunk)
 {
     extern __shared__ uint32_t smem[];
     __shared__ uint32_t s_base;
@@ -1447,23 +1423,23 @@ __global__ __launch_bounds__(64) void k_batch_decode(const BatchFrame *__restric
             switch (bf.mode) {
             case BATCH_FULL_APPLE:
                 decode_group_full<0>(bf.f, g, bf.samples, bf.linear, bf.flags, bf.status, bf.spw,
-                                     bf.byte_cap, smem);
+                                     smem);
                 break;
             case BATCH_FULL_FFMPEG:
                 decode_group_full<1>(bf.f, g, bf.samples, bf.linear, bf.flags, bf.status, bf.spw,
-                                     bf.byte_cap, smem);
+                                     smem);
                 break;
             case BATCH_HALF:
                 decode_group_reduced<1>(bf.f, g, bf.rgb, bf.flags, bf.status, bf.spw,
-                                        bf.byte_cap, bf.average, smem);
+                                        bf.average, smem);
                 break;
             case BATCH_QUARTER:
                 decode_group_reduced<2>(bf.f, g, bf.rgb, bf.flags, bf.status, bf.spw,
-                                        bf.byte_cap, bf.average, smem);
+                                        bf.average, smem);
                 break;
             default:
                 decode_group_reduced<3>(bf.f, g, bf.rgb, bf.flags, bf.status, bf.spw,
-                                        bf.byte_cap, bf.average, smem);
+                                        bf.average, smem);
                 break;
             }
         }
@@ -1500,7 +1476,7 @@ static uint32_t pick_spw(const DecodeLaunch &cfg, uint32_t per_block, uint32_t a
 {
     uint32_t spw = cfg.streams_per_wg;
     for (;;) {
-        lds = group_lds_bytes(spw, stream_stride(per_block, amax), cfg.byte_cap);
+        lds = group_lds_bytes(spw, stream_stride(per_block, amax));
         if (lds <= cfg.lds_limit || spw <= 4)
             return spw;
         spw >>= 1;
@@ -1519,10 +1495,10 @@ hipError_t launch_decode(const FrameDesc &f, int ffmpeg, uint16_t *samples, uint
         return hipSuccess;
     if (ffmpeg)
         hipLaunchKernelGGL(k_decode<1>, dim3(groups), dim3(64), lds, s, f, samples, linear, flags,
-                           status, spw, cfg.byte_cap);
+                           status, spw);
     else
         hipLaunchKernelGGL(k_decode<0>, dim3(groups), dim3(64), lds, s, f, samples, linear, flags,
-                           status, spw, cfg.byte_cap);
+                           status, spw);
     return hipGetLastError();
 }
 
@@ -1539,13 +1515,13 @@ hipError_t launch_decode_reduced(const FrameDesc &f, int shift, int average, uin
         return hipSuccess;
     if (shift == 1)
         hipLaunchKernelGGL(k_reduced<1>, dim3(groups), dim3(64), lds, s, f, rgb, flags, status,
-                           spw, cfg.byte_cap, (uint32_t)average);
+                           spw, (uint32_t)average);
     else if (shift == 2)
         hipLaunchKernelGGL(k_reduced<2>, dim3(groups), dim3(64), lds, s, f, rgb, flags, status,
-                           spw, cfg.byte_cap, (uint32_t)average);
+                           spw, (uint32_t)average);
     else
         hipLaunchKernelGGL(k_reduced<3>, dim3(groups), dim3(64), lds, s, f, rgb, flags, status,
-                           spw, cfg.byte_cap, (uint32_t)average);
+                           spw, (uint32_t)average);
     return hipGetLastError();
 }
 
@@ -1577,7 +1553,7 @@ hipError_t launch_develop(const DevelopDesc &d, float *r, float *g, float *b, hi
 hipError_t launch_batch(const BatchFrame *d_frames, uint32_t n_frames, uint32_t n_items,
                         uint32_t *d_queue, uint32_t *d_head, const BatchDevelop *d_jobs,
                         uint32_t n_jobs, uint32_t max_w, uint32_t max_h, uint32_t grid,
-                        uint32_t chunk, uint32_t byte_cap, size_t lds, hipStream_t s)
+                        uint32_t chunk, size_t lds, hipStream_t s)
 {
     if (n_frames == 0)
         return hipSuccess;
@@ -1586,7 +1562,7 @@ hipError_t launch_batch(const BatchFrame *d_frames, uint32_t n_frames, uint32_t 
         hipLaunchKernelGGL(k_build_queue, dim3(1), dim3(1024), 0, s, d_frames, n_frames, n_items,
                            d_queue);
         hipLaunchKernelGGL(k_batch_decode, dim3(grid), dim3(64), lds, s, d_frames, d_queue,
-                           n_items, d_head, chunk, byte_cap);
+                           n_items, d_head, chunk);
     }
     if (n_jobs) {
         dim3 g((max_w + DTW - 1) / DTW, (max_h + DTH - 1) / DTH, n_jobs);
@@ -1597,9 +1573,9 @@ hipError_t launch_batch(const BatchFrame *d_frames, uint32_t n_frames, uint32_t 
 
 const void *batch_decode_kernel() { return (const void *)k_batch_decode; }
 
-size_t decode_lds_bytes(uint32_t spw, uint32_t per_block, uint32_t amax, uint32_t byte_cap)
+size_t decode_lds_bytes(uint32_t spw, uint32_t per_block, uint32_t amax)
 {
-    return group_lds_bytes(spw, stream_stride(per_block, amax), byte_cap);
+    return group_lds_bytes(spw, stream_stride(per_block, amax));
 }
 
 } // namespace prr
