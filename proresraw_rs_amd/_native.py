@@ -1,4 +1,5 @@
-"""ctypes declarations of the C ABI in include/proresraw_amd.h.
+"""ctypes declarations of the C ABI in include/proresraw_amd.h and
+include/proresraw_learned.h.
 
 Loading fails loudly when the native library has not been built: there is
 no fallback implementation behind this module.
@@ -102,6 +103,19 @@ class PrrBatchFrame(ctypes.Structure):
     ]
 
 
+class PrrNetFrame(ctypes.Structure):
+    _fields_ = [
+        ("d_samples", ctypes.c_void_p),
+        ("stride", ctypes.c_uint32),
+        ("header", PrrHeader),
+        ("params", PrrDemosaicParams),
+        ("d_target", ctypes.c_void_p),
+    ]
+
+
+NET_PARAMS = 73984
+NET_LAYER_PARAMS = 9248
+
 E = ctypes.POINTER(PrrError)
 H = ctypes.POINTER(PrrHeader)
 FI = ctypes.POINTER(PrrFrameInfo)
@@ -144,15 +158,41 @@ SIGNATURES = {
     "prr_batch_run": (i32, [c_vp, c_vp, E]),
     "prr_batch_info": (i32, [c_vp, ctypes.POINTER(u32), ctypes.POINTER(u32), ctypes.POINTER(u32)]),
     "prr_batch_destroy": (None, [c_vp]),
+    "prr_net_create": (i32, [c_vp, ctypes.POINTER(c_vp), E]),
+    "prr_net_destroy": (None, [c_vp]),
+    "prr_net_set_params": (i32, [c_vp, c_vp, E]),
+    "prr_net_get_params": (i32, [c_vp, c_vp, E]),
+    "prr_net_set_range": (i32, [c_vp, ctypes.c_float, ctypes.c_float, E]),
+    "prr_net_train_setup": (i32, [c_vp, ctypes.POINTER(PrrNetFrame), u32, c_vp, u32, u32, u32,
+                                  u32, E]),
+    "prr_net_train_step": (i32, [c_vp, ctypes.c_float, c_vp, E]),
+    "prr_net_read": (i32, [c_vp, i32, u32, c_vp, sz, E]),
+    "prr_net_develop_device": (i32, [c_vp, H, c_vp, DP, c_vp, i32, u32, ctypes.c_float, c_vp,
+                                     c_vp, c_vp, c_vp, E]),
+    "prr_net_last_check": (i32, [c_vp, ctypes.POINTER(u32), ctypes.POINTER(ctypes.c_float), E]),
 }
 
 _lib = None
+
+
+def _share_torch_runtime():
+    """torch ships its own HIP runtime under the same soname
+    (libamdhip64.so.7) as the one the library links. Loading torch first
+    makes the library bind to that runtime, so a process that also uses
+    torch has one HIP runtime (device memory, streams and graphs shared);
+    loading the library first would give the process two runtimes, and
+    torch would then find no device."""
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
 
 
 def lib():
     """The loaded C-ABI library (raises OSError when it is not built)."""
     global _lib
     if _lib is None:
+        _share_torch_runtime()
         if not os.path.exists(LIB_PATH):
             raise OSError(
                 f"{LIB_PATH} is missing: build it with `python -m proresraw_rs_amd.build` "

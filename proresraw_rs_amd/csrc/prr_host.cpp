@@ -326,6 +326,8 @@ extern "C" void prr_context_destroy(prr_context *ctx)
 
 extern "C" void *prr_context_stream(prr_context *ctx) { return ctx ? ctx->stream : nullptr; }
 
+int prr::context_device(const prr_context *ctx) { return ctx->device; }
+
 /* Decode launch parameter: streams per workgroup (16, 32 or 64); 0 keeps
  * the current value. */
 extern "C" int prr_set_decode_config(prr_context *ctx, uint32_t streams_per_wg, prr_error *err)
@@ -445,8 +447,8 @@ extern "C" int prr_linearize_device(prr_context *ctx, const prr_header *hdr,
 }
 
 /* Conversion::new (demosaic.rs:286-304): the stage folded into the params. */
-static DevelopDesc develop_desc(const prr_header *hdr, const uint16_t *d_samples,
-                                const prr_demosaic_params *p, int stage)
+DevelopDesc prr::develop_desc(const prr_header *hdr, const uint16_t *d_samples,
+                              const prr_demosaic_params *p, int stage)
 {
     DevelopDesc d;
     std::memset(&d, 0, sizeof d);
@@ -480,9 +482,9 @@ static DevelopDesc develop_desc(const prr_header *hdr, const uint16_t *d_samples
  * frame's status (its max_qscale), so no readback is needed between decode
  * and develop. explicit_wb selects with_white_balance(red, blue) over the
  * header's gains. */
-static DevelopDesc develop_desc_device(const prr_header *hdr, const uint16_t *d_samples,
-                                       const prr_device_status *d_status, int explicit_wb,
-                                       float red, float blue, int stage)
+DevelopDesc prr::develop_desc_device(const prr_header *hdr, const uint16_t *d_samples,
+                                     const prr_device_status *d_status, int explicit_wb,
+                                     float red, float blue, int stage)
 {
     prr_demosaic_params p;
     if (explicit_wb)

@@ -50,6 +50,9 @@ struct DevelopDesc {
     const prr_device_status *qstatus;
     uint32_t qmat0;
     uint32_t range;
+    /* When set, the develop kernel runs only if *gate is non-zero (the
+     * learned path's fallback decided on the device). */
+    const uint32_t *gate;
 };
 
 enum BatchMode : uint32_t {
@@ -102,6 +105,16 @@ hipError_t launch_batch(const BatchFrame *d_frames, uint32_t n_frames, uint32_t 
                         uint32_t n_jobs, uint32_t max_w, uint32_t max_h, uint32_t grid,
                         uint32_t chunk, size_t lds, hipStream_t s);
 size_t decode_lds_bytes(uint32_t spw, uint32_t per_block, uint32_t amax);
+
+/* Host side (prr_host.cpp). */
+/* Conversion::new (demosaic.rs:286-304): the stage folded into the params. */
+DevelopDesc develop_desc(const prr_header *hdr, const uint16_t *d_samples,
+                         const prr_demosaic_params *p, int stage);
+/* develop(&frame, stage) with the noise model read on the device. */
+DevelopDesc develop_desc_device(const prr_header *hdr, const uint16_t *d_samples,
+                                const prr_device_status *d_status, int explicit_wb, float red,
+                                float blue, int stage);
+int context_device(const prr_context *ctx);
 const void *batch_decode_kernel();
 
 } // namespace prr

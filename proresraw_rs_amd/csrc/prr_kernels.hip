@@ -24,6 +24,7 @@
  * defaults.
  */
 #include "prr_internal.h"
+#include "prr_net.h"
 
 namespace prr {
 
@@ -1076,8 +1077,11 @@ __device__ inline void noise_from_quantizer(uint32_t quantizer, uint32_t range, 
     noise[2] = (float)(64.0 / 3.0 / range2);
 }
 
+/* Output sensel (x, y) goes to (y - oy) * ostride + (x - ox) of the planes
+ * (ox = oy = 0, ostride = width for a whole frame). */
 __device__ inline void develop_tile(const DevelopDesc &dd, float *out_r, float *out_g,
-                                    float *out_b, uint32_t bx, uint32_t by, DevelopLds &lds)
+                                    float *out_b, uint32_t bx, uint32_t by, DevelopLds &lds,
+                                    uint32_t ox, uint32_t oy, uint32_t ostride)
 {
     auto &norm = lds.norm;
     auto &green = lds.green;
@@ -1274,7 +1278,7 @@ __device__ inline void develop_tile(const DevelopDesc &dd, float *out_r, float *
             res[1][c] = rb_col == 0 ? res_g[c] : res_rb[c];
         }
         const uint32_t xs[2] = {xb, xb + 1};
-        const size_t at = (size_t)y * w + xs[0];
+        const size_t at = (size_t)(y - oy) * ostride + (xs[0] - ox);
         if (out_g) {
             if (xs[1] < w) {
                 *(float2 *)(out_r + at) = make_float2(res[0][0], res[1][0]);
@@ -1301,7 +1305,68 @@ __global__ __launch_bounds__(256) void k_develop(DevelopDesc d, float *out_r, fl
                                                  float *out_b)
 {
     __shared__ DevelopLds lds;
-    develop_tile(d, out_r, out_g, out_b, blockIdx.x, blockIdx.y, lds);
+    if (d.gate && *d.gate == 0)
+        return;
+    develop_tile(d, out_r, out_g, out_b, blockIdx.x, blockIdx.y, lds, 0, 0, d.width);
+}
+
+/* Plain develop of sampled 64x64 tiles (two 64x32 develop tiles each) into
+ * scratch[k] = r, g, b planes of 64x64, for the learned path's check. */
+__global__ __launch_bounds__(256) void k_develop_sampled(DevelopDesc d, const uint32_t *tiles,
+                                                         float *scratch)
+{
+    __shared__ DevelopLds lds;
+    const uint32_t k = blockIdx.x >> 1, half = blockIdx.x & 1;
+    const uint32_t bx = tiles[2 * k], by2 = tiles[2 * k + 1];
+    if (2 * by2 + half >= (d.height + DTH - 1) / DTH)
+        return;
+    float *base = scratch + (size_t)k * 3 * 4096;
+    develop_tile(d, base, base + 4096, base + 8192, bx, 2 * by2 + half, lds, DTW * bx,
+                 2 * DTH * by2, 64);
+}
+
+/* PSNR (peak 1) of the planes against the sampled plain tiles; any tile
+ * below min_db sets *use_plain. One workgroup per tile. */
+__global__ __launch_bounds__(256) void k_check_psnr(uint32_t width, uint32_t height,
+                                                    const float *r, const float *g,
+                                                    const float *b, const uint32_t *tiles,
+                                                    const float *scratch, float min_db,
+                                                    float *psnr, uint32_t *use_plain)
+{
+    __shared__ double s_sum[256];
+    __shared__ uint32_t s_n[256];
+    const uint32_t k = blockIdx.x;
+    const uint32_t x0 = DTW * tiles[2 * k], y0 = 2 * DTH * tiles[2 * k + 1];
+    const float *base = scratch + (size_t)k * 3 * 4096;
+    double sum = 0.0;
+    uint32_t n = 0;
+    for (uint32_t i = threadIdx.x; i < 4096; i += 256) {
+        const uint32_t y = y0 + i / 64, x = x0 + i % 64;
+        if (y >= height || x >= width)
+            continue;
+        const size_t at = (size_t)y * width + x;
+        const double e0 = (double)r[at] - base[i], e1 = (double)g[at] - base[4096 + i],
+                     e2 = (double)b[at] - base[8192 + i];
+        sum += e0 * e0 + e1 * e1 + e2 * e2;
+        n += 3;
+    }
+    s_sum[threadIdx.x] = sum;
+    s_n[threadIdx.x] = n;
+    __syncthreads();
+    for (uint32_t s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            s_sum[threadIdx.x] += s_sum[threadIdx.x + s];
+            s_n[threadIdx.x] += s_n[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double mse = s_n[0] ? s_sum[0] / s_n[0] : 0.0;
+        const float db = mse > 0.0 ? (float)(-10.0 * log10(mse)) : 200.0f;
+        psnr[k] = db;
+        if (!(db >= min_db))
+            atomicOr(use_plain, 1u);
+    }
 }
 
 
@@ -1454,7 +1519,7 @@ __global__ __launch_bounds__(256) void k_batch_develop(const BatchDevelop *jobs)
     const BatchDevelop &j = jobs[blockIdx.z];
     if (blockIdx.x * DTW >= j.d.width || blockIdx.y * DTH >= j.d.height)
         return;
-    develop_tile(j.d, j.r, j.g, j.b, blockIdx.x, blockIdx.y, lds);
+    develop_tile(j.d, j.r, j.g, j.b, blockIdx.x, blockIdx.y, lds, 0, 0, j.d.width);
 }
 
 /* ------------------------------------------------------------------ */
@@ -1572,6 +1637,20 @@ hipError_t launch_batch(const BatchFrame *d_frames, uint32_t n_frames, uint32_t 
 }
 
 const void *batch_decode_kernel() { return (const void *)k_batch_decode; }
+
+namespace net {
+hipError_t launch_check(const DevelopDesc &d, const float *r, const float *g, const float *b,
+                        const uint32_t *tiles, uint32_t n_tiles, float *scratch, float min_db,
+                        float *psnr, uint32_t *use_plain, hipStream_t s)
+{
+    if (n_tiles == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_develop_sampled, dim3(2 * n_tiles), dim3(256), 0, s, d, tiles, scratch);
+    hipLaunchKernelGGL(k_check_psnr, dim3(n_tiles), dim3(256), 0, s, d.width, d.height, r, g, b,
+                       tiles, scratch, min_db, psnr, use_plain);
+    return hipGetLastError();
+}
+} // namespace net
 
 size_t decode_lds_bytes(uint32_t spw, uint32_t per_block, uint32_t amax)
 {

@@ -1,0 +1,104 @@
+/* Internal interface of the learned demosaic (prr_net.hip): tensor
+ * layout, parameter layout and the kernel launchers the host layer
+ * (prr_net_host.cpp) uses. */
+#ifndef PRR_NET_INTERNAL_H
+#define PRR_NET_INTERNAL_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "prr_internal.h"
+
+namespace prr {
+namespace net {
+
+enum {
+    C = 32,                          /* channels of every activation */
+    TAPS = 9,
+    LAYER_W = TAPS * C * C,          /* W[tap][co][ci] */
+    LAYER_PARAMS = LAYER_W + C,      /* then b[co] */
+    LAYERS = 8,
+    PARAMS = LAYERS * LAYER_PARAMS,
+    OUT_CHANNELS = 12,               /* 2x2 sub-pixels x 3 colours */
+};
+
+/* Conv epilogues. */
+enum {
+    EPI_RELU = 0,        /* out = bf16(relu(v + b)) */
+    EPI_RESID_RELU = 1,  /* r = bf16(relu(v + b)); out = bf16(resid + r); aux = r */
+    EPI_LINEAR_F32 = 2,  /* out_f32 = v + b */
+    EPI_LINEAR = 3,      /* out = bf16(v) */
+    EPI_ADD = 4,         /* out = bf16(resid + v) */
+};
+
+/* One mosaic the network reads: code values, their conversion to the
+ * normalized values develop works on, and the flips that bring its
+ * pattern to RGGB. */
+struct NetFrame {
+    const uint16_t *samples;
+    uint32_t stride, width, height;
+    uint32_t curve[9];
+    int32_t black;
+    float one_over_range;
+    float clip_r, clip_b;   /* red and blue sensels clipped at these */
+    float feature;          /* 128 * sqrt(noise[0]) */
+    /* When set, the feature comes from the quantizer qmat0 * max_qscale
+     * read on the device (develop(&frame, stage) after a device decode). */
+    const prr_device_status *qstatus;
+    uint32_t qmat0;
+    uint32_t flip_y, flip_x;
+    const float *target;    /* training: develop(Demosaiced) planes, width*height each */
+};
+
+/* A training crop: frame index and RGGB-order origin (even) of the
+ * (2*H2) x (2*W2) window. */
+struct Crop {
+    uint32_t frame, y0, x0, pad_;
+};
+
+/* Compose parameters: white balance, clip and matrix after the network. */
+struct Finish {
+    float wb[2];
+    int32_t has_matrix;
+    float m[9];
+};
+
+hipError_t launch_weights(const float *params, uint16_t *wf, uint16_t *wt, hipStream_t s);
+hipError_t launch_conv(int epi, const uint16_t *in, int n, int h, int w, const uint16_t *wgt,
+                       const float *bias, const uint16_t *resid, uint16_t *out, uint16_t *aux,
+                       float *out_f32, hipStream_t s);
+size_t wgrad_tiles(int n, int h, int w);
+hipError_t launch_wgrad(const uint16_t *x, const uint16_t *g, int n, int h, int w, float *part,
+                        float *grad, hipStream_t s);
+hipError_t launch_relu_grad(const uint16_t *dh, const uint16_t *mask, uint16_t *dz, size_t count,
+                            hipStream_t s);
+hipError_t launch_adam(float *params, const float *grads, float *m, float *v, uint32_t *step,
+                       float lr, hipStream_t s);
+/* Training: crops[(step % steps) * n + k] (step read from *step) packed
+ * into x0. */
+hipError_t launch_pack_crops(const NetFrame *frames, const Crop *crops, const uint32_t *step,
+                             uint32_t steps, int n, int h2, int w2, uint16_t *x0, hipStream_t s);
+hipError_t launch_loss(const NetFrame *frames, const Crop *crops, const uint32_t *step,
+                       uint32_t steps, int n, int h2, int w2, int margin, const float *out,
+                       uint16_t *dout, float *loss_part, float *loss, hipStream_t s);
+/* Inference over a whole frame (even width and height). */
+hipError_t launch_pack_frame(const NetFrame &f, uint16_t *x0, hipStream_t s);
+hipError_t launch_compose(const NetFrame &f, const Finish &fin, const float *out, float *r,
+                          float *g, float *b, hipStream_t s);
+/* Sampled-tile check: PSNR of the network's planes against plain develop
+ * over 64x64 tiles (tiles[2k], tiles[2k+1] = column of 64, row of 64)
+ * through scratch (n_tiles * 3 * 4096 floats); sets *use_plain when any is
+ * below min_db. */
+hipError_t launch_check(const DevelopDesc &d, const float *r, const float *g, const float *b,
+                        const uint32_t *tiles, uint32_t n_tiles, float *scratch, float min_db,
+                        float *psnr, uint32_t *use_plain, hipStream_t s);
+hipError_t launch_gate_clear(uint32_t *flag, uint32_t value, hipStream_t s);
+/* Sets *use_plain when the frame's noise feature lies outside [lo, hi]. */
+hipError_t launch_noise_gate(const NetFrame &f, float lo, float hi, uint32_t *use_plain,
+                             hipStream_t s);
+hipError_t launch_step_inc(uint32_t *step, hipStream_t s);
+
+} // namespace net
+} // namespace prr
+
+#endif

@@ -1,0 +1,680 @@
+/* Learned demosaic for gfx950: a small residual convolutional network,
+ * trained and run with hand-written HIP kernels.
+ *
+ * Network (half resolution, NHWC, 32 channels):
+ *   x0      the RGGB mosaic packed 4 channels per 2x2 cell (R, G on the
+ *           red row, G on the blue row, B) in the normalized units develop
+ *           works on, red and blue clipped at the stage's clip values; the
+ *           noise feature 128*sqrt(noise[0]) in channel 4; zeros after
+ *   layer 0 h0 = relu(conv(x0) + b)
+ *   1..6    h = h + relu(conv(h) + b)
+ *   7       out = conv(h) + b; channels 3*(2*dy+dx)+c are the colours of
+ *           the cell's 2x2 sensels, added to a bilinear demosaic
+ * Weights: fp32 master copy (Adam state) used as bf16 on the matrix cores
+ * (v_mfma_f32_16x16x32_bf16, fp32 accumulation); activations and their
+ * gradients are bf16. Weight gradients are an fp32 reduction with
+ * per-workgroup partials summed in a fixed order, so training is
+ * deterministic.
+ *
+ * The network has no counterpart in the proresraw_rs crate; it is trained
+ * against the crate's develop(Stage::Demosaiced) (demosaic.rs:190-402) on
+ * the normalized values of Conversion::new (demosaic.rs:275-304).
+ */
+#include "prr_net.h"
+
+namespace prr {
+namespace net {
+
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4;
+
+__device__ inline float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
+
+/* fp32 -> bf16, round to nearest even, NaN kept quiet. */
+__device__ inline uint16_t f2bf(float f)
+{
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7F800000u) == 0x7F800000u && (u & 0x7FFFFFu))
+        return (uint16_t)((u >> 16) | 0x40u);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+__device__ inline uint32_t pack2(uint16_t a, uint16_t b) { return (uint32_t)a | ((uint32_t)b << 16); }
+
+/* ------------------------------------------------------------------ */
+/* Parameters -> bf16 forward weights W[l][t][co][ci] and the flipped,
+ * transposed weights of the input gradient Wt[l][t][ci][co] =
+ * W[l][8-t][co][ci]. */
+
+__global__ void k_weights(const float *__restrict__ params, uint16_t *__restrict__ wf,
+                          uint16_t *__restrict__ wt)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= LAYERS * LAYER_W)
+        return;
+    const uint32_t l = i / LAYER_W, r = i - l * LAYER_W;
+    const uint32_t t = r / (C * C), co = (r / C) % C, ci = r % C;
+    const uint16_t v = f2bf(params[l * LAYER_PARAMS + r]);
+    wf[i] = v;
+    wt[l * LAYER_W + (TAPS - 1 - t) * C * C + ci * C + co] = v;
+}
+
+/* ------------------------------------------------------------------ */
+/* 3x3 convolution, 32 -> 32 channels, on the matrix cores.
+ *
+ * One 256-lane workgroup computes 4 rows x 64 pixels x 32 output channels,
+ * one row per wave. The input rows with a one-pixel halo (zero outside the
+ * image) and the 9x32x32 weights are staged in LDS with an 80-byte pixel
+ * stride. For each tap a wave issues 8 MFMAs: 2 output-channel tiles x 4
+ * pixel tiles of 16, K = the 32 input channels. Operand layout of
+ * v_mfma_f32_16x16x32_bf16: lane l holds A[l%16][8(l/16)..+7] (A = weights,
+ * rows = output channels), B[8(l/16)..+7][l%16] (B = input, columns =
+ * pixels) and gets D[4(l/16)+i][l%16], i = 0..3. */
+
+enum { CT_ROWS = 4, CT_COLS = 64, CP = 40 };
+
+template <int EPI>
+__global__ __launch_bounds__(256) void k_conv(const uint16_t *__restrict__ in, int H, int W,
+                                              const uint16_t *__restrict__ wgt,
+                                              const float *__restrict__ bias,
+                                              const uint16_t *__restrict__ resid,
+                                              uint16_t *__restrict__ out,
+                                              uint16_t *__restrict__ aux,
+                                              float *__restrict__ out_f32)
+{
+    __shared__ __attribute__((aligned(16))) uint16_t s_in[CT_ROWS + 2][CT_COLS + 2][CP];
+    __shared__ __attribute__((aligned(16))) uint16_t s_w[TAPS][C][CP];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int x0 = blockIdx.x * CT_COLS, y0 = blockIdx.y * CT_ROWS;
+    const size_t img = (size_t)blockIdx.z * H * W;
+
+    for (int i = tid; i < TAPS * C * 4; i += 256) {
+        const int row = i >> 2, q = i & 3;
+        const uint4 v = ((const uint4 *)(wgt + (size_t)row * C))[q];
+        *(uint4 *)&s_w[row / C][row % C][8 * q] = v;
+    }
+    for (int i = tid; i < (CT_ROWS + 2) * (CT_COLS + 2) * 4; i += 256) {
+        const int q = i & 3, p = i >> 2;
+        const int ly = p / (CT_COLS + 2), lx = p - ly * (CT_COLS + 2);
+        const int y = y0 + ly - 1, x = x0 + lx - 1;
+        uint4 v = make_uint4(0, 0, 0, 0);
+        if (y >= 0 && y < H && x >= 0 && x < W)
+            v = ((const uint4 *)(in + (img + (size_t)y * W + x) * C))[q];
+        *(uint4 *)&s_in[ly][lx][8 * q] = v;
+    }
+    __syncthreads();
+
+    f32x4 acc[2][4];
+#pragma unroll
+    for (int mt = 0; mt < 2; mt++)
+#pragma unroll
+        for (int nt = 0; nt < 4; nt++)
+            acc[mt][nt] = (f32x4){0.f, 0.f, 0.f, 0.f};
+    const int r16 = lane & 15, kq = (lane >> 4) * 8;
+#pragma unroll
+    for (int t = 0; t < TAPS; t++) {
+        const int ky = t / 3, kx = t % 3;
+        bf16x8 a[2], b[4];
+#pragma unroll
+        for (int mt = 0; mt < 2; mt++)
+            a[mt] = *(const bf16x8 *)&s_w[t][16 * mt + r16][kq];
+#pragma unroll
+        for (int nt = 0; nt < 4; nt++)
+            b[nt] = *(const bf16x8 *)&s_in[wave + ky][16 * nt + r16 + kx][kq];
+#pragma unroll
+        for (int mt = 0; mt < 2; mt++)
+#pragma unroll
+            for (int nt = 0; nt < 4; nt++)
+                acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[mt], b[nt], acc[mt][nt], 0,
+                                                                      0, 0);
+    }
+
+    const int y = y0 + wave;
+    if (y >= H)
+        return;
+#pragma unroll
+    for (int nt = 0; nt < 4; nt++) {
+        const int x = x0 + 16 * nt + r16;
+        if (x >= W)
+            continue;
+        const size_t px = img + (size_t)y * W + x;
+#pragma unroll
+        for (int mt = 0; mt < 2; mt++) {
+            const int co = 16 * mt + (lane >> 4) * 4;
+            float v[4];
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+                v[i] = acc[mt][nt][i] + (bias ? bias[co + i] : 0.0f);
+            if (EPI == EPI_LINEAR_F32) {
+                *(float4 *)(out_f32 + px * C + co) = make_float4(v[0], v[1], v[2], v[3]);
+                continue;
+            }
+            uint16_t o[4];
+            if (EPI == EPI_RELU) {
+#pragma unroll
+                for (int i = 0; i < 4; i++)
+                    o[i] = f2bf(fmaxf(v[i], 0.0f));
+            } else if (EPI == EPI_RESID_RELU || EPI == EPI_ADD) {
+                const uint2 rv = *(const uint2 *)(resid + px * C + co);
+                const uint16_t rs[4] = {(uint16_t)rv.x, (uint16_t)(rv.x >> 16), (uint16_t)rv.y,
+                                        (uint16_t)(rv.y >> 16)};
+                uint16_t rr[4];
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    if (EPI == EPI_RESID_RELU) {
+                        rr[i] = f2bf(fmaxf(v[i], 0.0f));
+                        o[i] = f2bf(bf2f(rs[i]) + bf2f(rr[i]));
+                    } else {
+                        o[i] = f2bf(bf2f(rs[i]) + v[i]);
+                    }
+                }
+                if (EPI == EPI_RESID_RELU && aux)
+                    *(uint2 *)(aux + px * C + co) =
+                        make_uint2(pack2(rr[0], rr[1]), pack2(rr[2], rr[3]));
+            } else { /* EPI_LINEAR */
+#pragma unroll
+                for (int i = 0; i < 4; i++)
+                    o[i] = f2bf(v[i]);
+            }
+            *(uint2 *)(out + px * C + co) = make_uint2(pack2(o[0], o[1]), pack2(o[2], o[3]));
+        }
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* Weight and bias gradients: dW[t][co][ci] = sum_p g[p][co] x[p+off(t)][ci]
+ * and db[co] = sum_p g[p][co], over a tile of 8 x 32 pixels per workgroup.
+ * Lanes 0..287 own one (tap, ci) each and all 32 co; lanes 288..319 one
+ * bias each. Partials go to part[tile][LAYER_PARAMS]; k_reduce sums them
+ * in tile order. */
+
+enum { GT_ROWS = 8, GT_COLS = 32, GT_THREADS = 320 };
+
+__global__ __launch_bounds__(GT_THREADS) void k_wgrad(const uint16_t *__restrict__ x,
+                                                      const uint16_t *__restrict__ g, int H, int W,
+                                                      float *__restrict__ part)
+{
+    __shared__ __attribute__((aligned(16))) float s_g[GT_ROWS * GT_COLS][C];
+    __shared__ uint16_t s_x[GT_ROWS + 2][GT_COLS + 2][C];
+    const int tid = threadIdx.x;
+    const int x0 = blockIdx.x * GT_COLS, y0 = blockIdx.y * GT_ROWS;
+    const size_t img = (size_t)blockIdx.z * H * W;
+    const size_t tile = ((size_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+
+    for (int i = tid; i < GT_ROWS * GT_COLS * (C / 2); i += GT_THREADS) {
+        const int p = i / (C / 2), c2 = i - p * (C / 2);
+        const int y = y0 + p / GT_COLS, xx = x0 + p % GT_COLS;
+        uint32_t v = 0;
+        if (y < H && xx < W)
+            v = ((const uint32_t *)(g + (img + (size_t)y * W + xx) * C))[c2];
+        s_g[p][2 * c2] = bf2f((uint16_t)v);
+        s_g[p][2 * c2 + 1] = bf2f((uint16_t)(v >> 16));
+    }
+    for (int i = tid; i < (GT_ROWS + 2) * (GT_COLS + 2) * (C / 2); i += GT_THREADS) {
+        const int c2 = i % (C / 2), p = i / (C / 2);
+        const int ly = p / (GT_COLS + 2), lx = p - ly * (GT_COLS + 2);
+        const int y = y0 + ly - 1, xx = x0 + lx - 1;
+        uint32_t v = 0;
+        if (y >= 0 && y < H && xx >= 0 && xx < W)
+            v = ((const uint32_t *)(x + (img + (size_t)y * W + xx) * C))[c2];
+        *(uint32_t *)&s_x[ly][lx][2 * c2] = v;
+    }
+    __syncthreads();
+
+    float *dst = part + tile * LAYER_PARAMS;
+    if (tid < TAPS * C) {
+        const int t = tid / C, ci = tid % C, ky = t / 3, kx = t % 3;
+        float acc[C];
+#pragma unroll
+        for (int co = 0; co < C; co++)
+            acc[co] = 0.0f;
+        for (int p = 0; p < GT_ROWS * GT_COLS; p++) {
+            const float xv = bf2f(s_x[p / GT_COLS + ky][p % GT_COLS + kx][ci]);
+#pragma unroll
+            for (int c4 = 0; c4 < C / 4; c4++) {
+                const float4 gv = *(const float4 *)&s_g[p][4 * c4];
+                acc[4 * c4 + 0] = fmaf(gv.x, xv, acc[4 * c4 + 0]);
+                acc[4 * c4 + 1] = fmaf(gv.y, xv, acc[4 * c4 + 1]);
+                acc[4 * c4 + 2] = fmaf(gv.z, xv, acc[4 * c4 + 2]);
+                acc[4 * c4 + 3] = fmaf(gv.w, xv, acc[4 * c4 + 3]);
+            }
+        }
+#pragma unroll
+        for (int co = 0; co < C; co++)
+            dst[t * C * C + co * C + ci] = acc[co];
+    } else {
+        const int co = tid - TAPS * C;
+        float acc = 0.0f;
+        for (int p = 0; p < GT_ROWS * GT_COLS; p++)
+            acc += s_g[p][co];
+        dst[LAYER_W + co] = acc;
+    }
+}
+
+__global__ void k_reduce(const float *__restrict__ part, uint32_t tiles, float *__restrict__ grad)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= LAYER_PARAMS)
+        return;
+    float s = 0.0f;
+    for (uint32_t t = 0; t < tiles; t++)
+        s += part[(size_t)t * LAYER_PARAMS + i];
+    grad[i] = s;
+}
+
+/* dz = dh where the layer's ReLU output is positive, else 0. */
+__global__ void k_relu_grad(const uint16_t *__restrict__ dh, const uint16_t *__restrict__ mask,
+                            uint16_t *__restrict__ dz, size_t count)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count)
+        return;
+    const uint16_t m = mask[i];
+    dz[i] = (m != 0 && !(m & 0x8000u)) ? dh[i] : (uint16_t)0;
+}
+
+/* Adam (betas 0.9 / 0.999, eps 1e-8); the step count is read on the
+ * device so a captured graph replays correctly. */
+__global__ void k_adam(float *__restrict__ p, const float *__restrict__ g, float *__restrict__ m,
+                       float *__restrict__ v, const uint32_t *__restrict__ step, float lr)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= PARAMS)
+        return;
+    const double t = (double)(*step + 1);
+    const float c1 = (float)(1.0 - pow(0.9, t));
+    const float c2 = (float)(1.0 - pow(0.999, t));
+    const float gi = g[i];
+    const float mi = 0.9f * m[i] + 0.1f * gi;
+    const float vi = 0.999f * v[i] + 0.001f * (gi * gi);
+    m[i] = mi;
+    v[i] = vi;
+    const float num = lr * (mi / c1);
+    const float den = sqrtf(vi / c2) + 1e-8f;
+    p[i] = p[i] - num / den;
+}
+
+__global__ void k_step_inc(uint32_t *step)
+{
+    if (threadIdx.x == 0)
+        *step += 1;
+}
+
+__global__ void k_set(uint32_t *flag, uint32_t value)
+{
+    if (threadIdx.x == 0)
+        *flag = value;
+}
+
+__device__ inline float feature_of(const NetFrame &f);
+
+__global__ void k_noise_gate(NetFrame f, float lo, float hi, uint32_t *use_plain)
+{
+    if (threadIdx.x != 0)
+        return;
+    const float v = feature_of(f);
+    if (!(v >= lo && v <= hi))
+        *use_plain = 1u;
+}
+
+/* ------------------------------------------------------------------ */
+/* Mosaic access in RGGB order. */
+
+__device__ inline uint32_t lin_at(const NetFrame &f, uint32_t code)
+{
+    const uint32_t seg = code >> 13, frac = code & 0x1FFFu;
+    const uint32_t c0 = f.curve[seg], c1 = f.curve[seg + 1];
+    const uint32_t v = (c0 * 8192u + ((c1 - c0) & 0xFFFFu) * frac + 4096u) >> 13;
+    return v < 65535u ? v : 65535u;
+}
+
+/* Normalized value of the sensel at RGGB-order (y, x), red and blue
+ * clipped. */
+__device__ inline float norm_at(const NetFrame &f, int y, int x)
+{
+    const uint32_t sy = f.flip_y ? f.height - 1 - (uint32_t)y : (uint32_t)y;
+    const uint32_t sx = f.flip_x ? f.width - 1 - (uint32_t)x : (uint32_t)x;
+    const uint32_t code = f.samples[(size_t)sy * f.stride + sx];
+    float v = f.one_over_range * (float)((int32_t)lin_at(f, code) - f.black);
+    if (((y | x) & 1) == 0)
+        v = fminf(v, f.clip_r);
+    else if ((y & x & 1) == 1)
+        v = fminf(v, f.clip_b);
+    return v;
+}
+
+__device__ inline int mirror(int i, int n) { return i < 0 ? -i : i >= n ? 2 * n - 2 - i : i; }
+
+/* Bilinear demosaic at (y, x) of the window of rows/columns [oy, oy+hh) x
+ * [ox, ox+ww), mirrored at the window's edges. */
+__device__ inline void bilinear(const NetFrame &f, int oy, int ox, int hh, int ww, int y, int x,
+                                float *rgb)
+{
+    auto v = [&](int dy, int dx) {
+        return norm_at(f, oy + mirror(y + dy, hh), ox + mirror(x + dx, ww));
+    };
+    const int py = y & 1, px = x & 1;
+    const float c = v(0, 0);
+    if (py == px) {
+        const float cross = ((v(-1, 0) + v(1, 0)) + (v(0, -1) + v(0, 1))) * 0.25f;
+        const float diag = ((v(-1, -1) + v(-1, 1)) + (v(1, -1) + v(1, 1))) * 0.25f;
+        rgb[0] = py ? diag : c;
+        rgb[1] = cross;
+        rgb[2] = py ? c : diag;
+    } else {
+        const float horiz = (v(0, -1) + v(0, 1)) * 0.5f;
+        const float vert = (v(-1, 0) + v(1, 0)) * 0.5f;
+        rgb[0] = py ? vert : horiz;
+        rgb[1] = c;
+        rgb[2] = py ? horiz : vert;
+    }
+}
+
+/* The noise feature 128*sqrt(noise[0]), noise[0] from the quantizer as
+ * DemosaicParams::with_white_balance computes it (demosaic.rs:86-101). */
+__device__ inline float feature_of(const NetFrame &f)
+{
+    if (!f.qstatus)
+        return f.feature;
+    const double q = (double)(f.qmat0 * f.qstatus->max_qscale);
+    const float n0 = (float)((q * q * 5.0 + 64.0 / 3.0) / 150994944.0);
+    return 128.0f * sqrtf(n0);
+}
+
+/* One packed cell: 32 bf16 channels. */
+__device__ inline void pack_cell(const NetFrame &f, int y, int x, uint16_t *dst)
+{
+    const uint16_t r = f2bf(norm_at(f, y, x)), g1 = f2bf(norm_at(f, y, x + 1));
+    const uint16_t g2 = f2bf(norm_at(f, y + 1, x)), b = f2bf(norm_at(f, y + 1, x + 1));
+    uint4 *d = (uint4 *)dst;
+    d[0] = make_uint4(pack2(r, g1), pack2(g2, b), pack2(f2bf(feature_of(f)), 0), 0);
+    d[1] = make_uint4(0, 0, 0, 0);
+    d[2] = make_uint4(0, 0, 0, 0);
+    d[3] = make_uint4(0, 0, 0, 0);
+}
+
+__global__ void k_pack_crops(const NetFrame *__restrict__ frames, const Crop *__restrict__ crops,
+                             const uint32_t *__restrict__ step, uint32_t steps, int n, int h2,
+                             int w2, uint16_t *__restrict__ x0)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t cells = (size_t)h2 * w2;
+    if (i >= cells * n)
+        return;
+    const int k = (int)(i / cells), c = (int)(i - (size_t)k * cells);
+    const Crop cr = crops[(size_t)(*step % steps) * n + k];
+    const NetFrame &f = frames[cr.frame];
+    const int ci = c / w2, cj = c - ci * w2;
+    pack_cell(f, (int)cr.y0 + 2 * ci, (int)cr.x0 + 2 * cj, x0 + i * C);
+}
+
+__global__ void k_pack_frame(NetFrame f, uint16_t *__restrict__ x0)
+{
+    const int w2 = (int)f.width / 2, h2 = (int)f.height / 2;
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)h2 * w2)
+        return;
+    const int ci = (int)(i / w2), cj = (int)(i - (size_t)ci * w2);
+    pack_cell(f, 2 * ci, 2 * cj, x0 + i * C);
+}
+
+/* ------------------------------------------------------------------ */
+/* Loss: mean squared error between out + bilinear and the target over
+ * the cells at least `margin` from the crop's edge, and its gradient with
+ * respect to out (bf16). One lane per cell. */
+
+__global__ __launch_bounds__(256) void k_loss(const NetFrame *__restrict__ frames,
+                                              const Crop *__restrict__ crops,
+                                              const uint32_t *__restrict__ step, uint32_t steps,
+                                              int n, int h2, int w2, int margin,
+                                              const float *__restrict__ out,
+                                              uint16_t *__restrict__ dout,
+                                              float *__restrict__ loss_part)
+{
+    __shared__ float s_sum[256];
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t cells = (size_t)h2 * w2;
+    float sq = 0.0f;
+    if (i < cells * n) {
+        const int k = (int)(i / cells), c = (int)(i - (size_t)k * cells);
+        const int ci = c / w2, cj = c - ci * w2;
+        float d[C];
+#pragma unroll
+        for (int ch = 0; ch < C; ch++)
+            d[ch] = 0.0f;
+        if (ci >= margin && ci < h2 - margin && cj >= margin && cj < w2 - margin) {
+            const Crop cr = crops[(size_t)(*step % steps) * n + k];
+            const NetFrame &f = frames[cr.frame];
+            const size_t count = (size_t)n * 3 * (size_t)(2 * (h2 - 2 * margin)) *
+                                 (size_t)(2 * (w2 - 2 * margin));
+            const float scale = 2.0f / (float)count;
+            const size_t plane = (size_t)f.width * f.height;
+#pragma unroll
+            for (int dy = 0; dy < 2; dy++)
+#pragma unroll
+                for (int dx = 0; dx < 2; dx++) {
+                    float base[3];
+                    const int y = 2 * ci + dy, x = 2 * cj + dx;
+                    bilinear(f, (int)cr.y0, (int)cr.x0, 2 * h2, 2 * w2, y, x, base);
+                    const size_t at = (size_t)((int)cr.y0 + y) * f.width + (size_t)((int)cr.x0 + x);
+#pragma unroll
+                    for (int col = 0; col < 3; col++) {
+                        const int ch = 3 * (2 * dy + dx) + col;
+                        const float pred = out[i * C + ch] + base[col];
+                        const float diff = pred - f.target[col * plane + at];
+                        sq += diff * diff;
+                        d[ch] = diff * scale;
+                    }
+                }
+        }
+        uint4 *dst = (uint4 *)(dout + i * C);
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+            dst[q] = make_uint4(pack2(f2bf(d[8 * q + 0]), f2bf(d[8 * q + 1])),
+                                pack2(f2bf(d[8 * q + 2]), f2bf(d[8 * q + 3])),
+                                pack2(f2bf(d[8 * q + 4]), f2bf(d[8 * q + 5])),
+                                pack2(f2bf(d[8 * q + 6]), f2bf(d[8 * q + 7])));
+    }
+    s_sum[threadIdx.x] = sq;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s)
+            s_sum[threadIdx.x] += s_sum[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        loss_part[blockIdx.x] = s_sum[0];
+}
+
+__global__ void k_loss_total(const float *__restrict__ part, uint32_t n, float inv_count,
+                             float *__restrict__ loss)
+{
+    if (threadIdx.x != 0)
+        return;
+    double s = 0.0;
+    for (uint32_t i = 0; i < n; i++)
+        s += part[i];
+    *loss = (float)(s * (double)inv_count);
+}
+
+/* ------------------------------------------------------------------ */
+/* Inference output: out + bilinear, then white balance with the clip at 1
+ * and the output matrix as develop's last steps do them (chroma_row_with
+ * and matrix_planes_impl, demosaic.rs:648-720), written in sensor order.
+ * One lane per cell. */
+
+__global__ __launch_bounds__(256) void k_compose(NetFrame f, Finish fin,
+                                                 const float *__restrict__ out,
+                                                 float *__restrict__ pr, float *__restrict__ pg,
+                                                 float *__restrict__ pb)
+{
+    const int w2 = (int)f.width / 2, h2 = (int)f.height / 2;
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)h2 * w2)
+        return;
+    const int ci = (int)(i / w2), cj = (int)(i - (size_t)ci * w2);
+#pragma unroll
+    for (int dy = 0; dy < 2; dy++)
+#pragma unroll
+        for (int dx = 0; dx < 2; dx++) {
+            const int y = 2 * ci + dy, x = 2 * cj + dx;
+            float base[3];
+            bilinear(f, 0, 0, (int)f.height, (int)f.width, y, x, base);
+            const int ch = 3 * (2 * dy + dx);
+            float r = fminf((out[i * C + ch + 0] + base[0]) * fin.wb[0], 1.0f);
+            float g = fminf(out[i * C + ch + 1] + base[1], 1.0f);
+            float b = fminf((out[i * C + ch + 2] + base[2]) * fin.wb[1], 1.0f);
+            if (fin.has_matrix) {
+                const float r2 = fmaf(b, fin.m[2], fmaf(r, fin.m[0], g * fin.m[1]));
+                const float g2 = fmaf(b, fin.m[5], fmaf(r, fin.m[3], g * fin.m[4]));
+                const float b2 = fmaf(b, fin.m[8], fmaf(r, fin.m[6], g * fin.m[7]));
+                r = r2;
+                g = g2;
+                b = b2;
+            }
+            const uint32_t sy = f.flip_y ? f.height - 1 - (uint32_t)y : (uint32_t)y;
+            const uint32_t sx = f.flip_x ? f.width - 1 - (uint32_t)x : (uint32_t)x;
+            const size_t at = (size_t)sy * f.width + sx;
+            pr[at] = r;
+            pg[at] = g;
+            pb[at] = b;
+        }
+}
+
+/* ------------------------------------------------------------------ */
+
+hipError_t launch_weights(const float *params, uint16_t *wf, uint16_t *wt, hipStream_t s)
+{
+    const uint32_t n = LAYERS * LAYER_W;
+    hipLaunchKernelGGL(k_weights, dim3((n + 255) / 256), dim3(256), 0, s, params, wf, wt);
+    return hipGetLastError();
+}
+
+hipError_t launch_conv(int epi, const uint16_t *in, int n, int h, int w, const uint16_t *wgt,
+                       const float *bias, const uint16_t *resid, uint16_t *out, uint16_t *aux,
+                       float *out_f32, hipStream_t s)
+{
+    if (n <= 0 || h <= 0 || w <= 0)
+        return hipSuccess;
+    const dim3 grid((w + CT_COLS - 1) / CT_COLS, (h + CT_ROWS - 1) / CT_ROWS, n);
+    switch (epi) {
+    case EPI_RELU:
+        hipLaunchKernelGGL(k_conv<EPI_RELU>, grid, dim3(256), 0, s, in, h, w, wgt, bias, resid,
+                           out, aux, out_f32);
+        break;
+    case EPI_RESID_RELU:
+        hipLaunchKernelGGL(k_conv<EPI_RESID_RELU>, grid, dim3(256), 0, s, in, h, w, wgt, bias,
+                           resid, out, aux, out_f32);
+        break;
+    case EPI_LINEAR_F32:
+        hipLaunchKernelGGL(k_conv<EPI_LINEAR_F32>, grid, dim3(256), 0, s, in, h, w, wgt, bias,
+                           resid, out, aux, out_f32);
+        break;
+    case EPI_LINEAR:
+        hipLaunchKernelGGL(k_conv<EPI_LINEAR>, grid, dim3(256), 0, s, in, h, w, wgt, bias, resid,
+                           out, aux, out_f32);
+        break;
+    case EPI_ADD:
+        hipLaunchKernelGGL(k_conv<EPI_ADD>, grid, dim3(256), 0, s, in, h, w, wgt, bias, resid,
+                           out, aux, out_f32);
+        break;
+    default:
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+size_t wgrad_tiles(int n, int h, int w)
+{
+    return (size_t)n * ((h + GT_ROWS - 1) / GT_ROWS) * ((w + GT_COLS - 1) / GT_COLS);
+}
+
+hipError_t launch_wgrad(const uint16_t *x, const uint16_t *g, int n, int h, int w, float *part,
+                        float *grad, hipStream_t s)
+{
+    const dim3 grid((w + GT_COLS - 1) / GT_COLS, (h + GT_ROWS - 1) / GT_ROWS, n);
+    hipLaunchKernelGGL(k_wgrad, grid, dim3(GT_THREADS), 0, s, x, g, h, w, part);
+    const uint32_t tiles = (uint32_t)wgrad_tiles(n, h, w);
+    hipLaunchKernelGGL(k_reduce, dim3((LAYER_PARAMS + 255) / 256), dim3(256), 0, s, part, tiles,
+                       grad);
+    return hipGetLastError();
+}
+
+hipError_t launch_relu_grad(const uint16_t *dh, const uint16_t *mask, uint16_t *dz, size_t count,
+                            hipStream_t s)
+{
+    hipLaunchKernelGGL(k_relu_grad, dim3((uint32_t)((count + 255) / 256)), dim3(256), 0, s, dh,
+                       mask, dz, count);
+    return hipGetLastError();
+}
+
+hipError_t launch_adam(float *params, const float *grads, float *m, float *v, uint32_t *step,
+                       float lr, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_adam, dim3((PARAMS + 255) / 256), dim3(256), 0, s, params, grads, m, v,
+                       step, lr);
+    return hipGetLastError();
+}
+
+hipError_t launch_step_inc(uint32_t *step, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_step_inc, dim3(1), dim3(64), 0, s, step);
+    return hipGetLastError();
+}
+
+hipError_t launch_gate_clear(uint32_t *flag, uint32_t value, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_set, dim3(1), dim3(64), 0, s, flag, value);
+    return hipGetLastError();
+}
+
+hipError_t launch_noise_gate(const NetFrame &f, float lo, float hi, uint32_t *use_plain,
+                             hipStream_t s)
+{
+    hipLaunchKernelGGL(k_noise_gate, dim3(1), dim3(64), 0, s, f, lo, hi, use_plain);
+    return hipGetLastError();
+}
+
+hipError_t launch_pack_crops(const NetFrame *frames, const Crop *crops, const uint32_t *step,
+                             uint32_t steps, int n, int h2, int w2, uint16_t *x0, hipStream_t s)
+{
+    const size_t cells = (size_t)n * h2 * w2;
+    hipLaunchKernelGGL(k_pack_crops, dim3((uint32_t)((cells + 255) / 256)), dim3(256), 0, s,
+                       frames, crops, step, steps, n, h2, w2, x0);
+    return hipGetLastError();
+}
+
+hipError_t launch_loss(const NetFrame *frames, const Crop *crops, const uint32_t *step,
+                       uint32_t steps, int n, int h2, int w2, int margin, const float *out,
+                       uint16_t *dout, float *loss_part, float *loss, hipStream_t s)
+{
+    const size_t cells = (size_t)n * h2 * w2;
+    const uint32_t blocks = (uint32_t)((cells + 255) / 256);
+    hipLaunchKernelGGL(k_loss, dim3(blocks), dim3(256), 0, s, frames, crops, step, steps, n, h2,
+                       w2, margin, out, dout, loss_part);
+    const double count = (double)n * 3.0 * (2.0 * (h2 - 2 * margin)) * (2.0 * (w2 - 2 * margin));
+    hipLaunchKernelGGL(k_loss_total, dim3(1), dim3(64), 0, s, loss_part, blocks,
+                       (float)(1.0 / count), loss);
+    return hipGetLastError();
+}
+
+hipError_t launch_pack_frame(const NetFrame &f, uint16_t *x0, hipStream_t s)
+{
+    const size_t cells = (size_t)(f.width / 2) * (f.height / 2);
+    hipLaunchKernelGGL(k_pack_frame, dim3((uint32_t)((cells + 255) / 256)), dim3(256), 0, s, f,
+                       x0);
+    return hipGetLastError();
+}
+
+hipError_t launch_compose(const NetFrame &f, const Finish &fin, const float *out, float *r,
+                          float *g, float *b, hipStream_t s)
+{
+    const size_t cells = (size_t)(f.width / 2) * (f.height / 2);
+    hipLaunchKernelGGL(k_compose, dim3((uint32_t)((cells + 255) / 256)), dim3(256), 0, s, f, fin,
+                       out, r, g, b);
+    return hipGetLastError();
+}
+
+} // namespace net
+} // namespace prr

@@ -1,0 +1,204 @@
+"""GPU checks of the learned demosaic's HIP kernels against the oracle
+(oracle/net_oracle.py), through the C ABI of include/proresraw_learned.h.
+
+Each layer is checked on the inputs the GPU itself produced (so rounding
+differences do not compound): the matrix cores sum in float32 in their own
+order and the oracle in float64, so activations may differ by a bf16
+rounding step; packing, the loss gradient and the Adam step are exact."""
+import numpy as np
+import pytest
+
+from oracle import net_oracle as NO
+
+pytestmark = pytest.mark.gpu
+
+CELLS, MARGIN = 24, 4
+
+
+@pytest.fixture(scope="module")
+def env(gpu):
+    import torch
+    from proresraw_rs_amd import learned as L
+    from proresraw_rs_amd import synth as S
+    frames = []
+    for i, q in enumerate((15, 45)):
+        sc = S.scene(128, 96, 0, seed=500 + i, version=0, alignment=2, qscale=(q, q),
+                     white_level=4095)
+        frames.append(L.frame_pair(S.encode_scene(sc)))
+    return torch, L, frames
+
+
+def crop_host(frame, y0, x0, cells):
+    """Normalized mosaic, target and bilinear base of one crop (numpy)."""
+    h = frame.header
+    samples = frame.samples.cpu().numpy().view(np.uint16).reshape(h.coded_height(), h.coded_width())
+    p = frame.params
+    norm = NO.normalized(samples[:h.height, :h.width], list(h.curve), p.black, p.one_over_range)
+    win = norm[y0:y0 + 2 * cells, x0:x0 + 2 * cells]
+    win = NO.preclip(win, 1.0, 1.0)
+    target = frame.target.cpu().numpy()[:, y0:y0 + 2 * cells, x0:x0 + 2 * cells]
+    return win, target, NO.bilinear(win)
+
+
+def run_step(L, net, frames, params, crops, lr=0.0):
+    net.set_params(params)
+    net.train_setup(frames, crops, CELLS, MARGIN)
+    net.train_step(lr)
+    return {k: net.read(k, l) for k, l in
+            [(L.X0, 0), (L.OUT, 0), (L.DOUT, 0), (L.GRAD, 0), (L.LOSS, 0)]}
+
+
+def bf16_close(got, want, what, steps=2):
+    """Equal up to `steps` bf16 rounding steps of the larger magnitude."""
+    got, want = np.asarray(got, np.float64), np.asarray(want, np.float64)
+    tol = steps * 2.0 ** -8 * np.maximum(np.abs(want), np.abs(got)) + 1e-30
+    bad = np.abs(got - want) > tol
+    assert bad.mean() < 1e-3, f"{what}: {bad.mean():.2e} of values off, worst {np.abs(got - want).max()}"
+
+
+def test_forward_layers_loss_and_gradients_match_oracle(env):
+    torch, L, frames = env
+    net = L.Net()
+    params = L.init_params(3)
+    # Larger residual and output weights than the training start, so every
+    # layer contributes measurably.
+    for l in range(1, 8):
+        params[l * NO.LAYER_PARAMS:l * NO.LAYER_PARAMS + 9216] *= 3.0
+    params[7 * NO.LAYER_PARAMS + 9216:7 * NO.LAYER_PARAMS + 9216 + 12] = 0.01
+    crops = np.array([[[0, 10, 20], [1, 40, 36]]], np.uint32)
+    t = run_step(L, net, frames, params, crops)
+    n = crops.shape[1]
+    wins, targets, bases = zip(*[crop_host(frames[c[0]], c[1], c[2], CELLS) for c in crops[0]])
+
+    # Packing is exact.
+    x0 = np.stack([NO.pack(w, L.noise_feature(frames[c[0]].quantizer)) for w, c in zip(wins, crops[0])])
+    assert np.array_equal(t[L.X0], x0)
+
+    # Each layer against the oracle on the GPU's own input.
+    ws = [NO.bf16(w.astype(np.float32)) for w, _ in NO.split(params)]
+    bs = [b for _, b in NO.split(params)]
+    h = [net.read(L.H, l) for l in range(7)]
+    r = [None] + [net.read(L.R, l) for l in range(1, 7)]
+    bf16_close(h[0], NO.bf16(np.maximum(NO.conv(x0, ws[0], bs[0]), 0).astype(np.float32)), "h0")
+    for l in range(1, 7):
+        rr = NO.bf16(np.maximum(NO.conv(h[l - 1], ws[l], bs[l]), 0).astype(np.float32))
+        bf16_close(r[l], rr, f"r{l}")
+        bf16_close(h[l], NO.bf16((h[l - 1] + r[l]).astype(np.float32)), f"h{l}", steps=0)
+    out = NO.conv(h[6], ws[7], bs[7])
+    assert np.allclose(t[L.OUT], out, rtol=1e-4, atol=1e-5 * np.abs(out).max())
+
+    # Loss and its gradient at out are exact given out.
+    loss, dout = NO.loss_and_dout(t[L.OUT], np.stack(bases), np.stack(targets), MARGIN)
+    assert np.array_equal(t[L.DOUT], dout)
+    assert abs(float(t[L.LOSS][0]) - loss) <= 1e-5 * loss
+
+    # Gradients: the oracle backward over the GPU's tensors.
+    cache = {"x0": x0, "h": h, "r": r, "w": ws}
+    grad = NO.backward(cache, t[L.DOUT])
+    for l in range(NO.LAYERS):
+        g = grad[l * NO.LAYER_PARAMS:(l + 1) * NO.LAYER_PARAMS]
+        gg = t[L.GRAD][l * NO.LAYER_PARAMS:(l + 1) * NO.LAYER_PARAMS]
+        scale = np.abs(g).max()
+        assert scale > 0
+        cos = float(np.dot(g, gg) / (np.linalg.norm(g) * np.linalg.norm(gg)))
+        assert cos > 0.9999, (l, cos)
+        assert np.abs(g - gg).max() <= 0.02 * scale, (l, np.abs(g - gg).max(), scale)
+
+
+def test_adam_step_is_exact(env):
+    torch, L, frames = env
+    net = L.Net()
+    params = L.init_params(4)
+    crops = np.array([[[0, 0, 0], [1, 20, 16]]], np.uint32)
+    t = run_step(L, net, frames, params, crops, lr=1e-3)
+    want, _, _ = NO.adam(params, t[L.GRAD], np.zeros_like(params), np.zeros_like(params), 1)
+    assert np.array_equal(net.params(), want)
+    assert int(net.read(L.STEP)[0]) == 1
+
+
+def test_graph_replay_equals_eager_and_training_is_deterministic(env):
+    torch, L, frames = env
+    rng = np.random.default_rng(1)
+    crops = L.random_crops(rng, frames, 40, 2, CELLS)
+    finals = []
+    for use_graph in (False, True, True):
+        net = L.Net(L.init_params(5))
+        net.train_setup(frames, crops, CELLS, MARGIN)
+        stream = torch.cuda.Stream()
+        with torch.cuda.stream(stream):
+            if use_graph:
+                net.train_step(1e-3, stream)
+                g = torch.cuda.CUDAGraph()
+                with torch.cuda.graph(g, stream=stream):
+                    net.train_step(1e-3, stream)
+                for _ in range(39):
+                    g.replay()
+                    torch.cuda.synchronize()  # eager host work between replays
+            else:
+                for _ in range(40):
+                    net.train_step(1e-3, stream)
+        stream.synchronize()
+        assert int(net.read(L.STEP)[0]) == 40
+        finals.append(net.params())
+    assert np.array_equal(finals[0], finals[1])
+    assert np.array_equal(finals[1], finals[2])
+
+
+def test_training_reduces_the_loss(env):
+    torch, L, frames = env
+    net = L.Net(L.init_params(6))
+    hist = L.train(net, frames, steps=300, batch=4, cells=CELLS, margin=MARGIN, lr=1e-3, seed=2,
+                   log_every=25, log=lambda *_: None)
+    first, last = hist[0][1], np.mean([l for _, l in hist[-3:]])
+    print("loss history", hist)
+    assert last < 0.8 * first, hist
+
+
+def develop_pair(torch, L, frame, stage, net, **kw):
+    from proresraw_rs_amd import api as A
+    h = frame.header
+    n = h.width * h.height
+    planes = [torch.zeros(n, dtype=torch.float32, device="cuda") for _ in range(3)]
+    net.develop(h, frame.samples, frame.params, stage, planes, **kw)
+    torch.cuda.synchronize()
+    got = np.stack([p.cpu().numpy() for p in planes])
+    fr = A.Frame(h, frame.samples.cpu().numpy().view(np.uint16), [], 0)
+    plain = np.stack(A.develop_planar(fr, frame.params, stage).planes)
+    return got, plain
+
+
+def test_fallbacks_give_plain_develop_exactly(env):
+    torch, L, frames = env
+    from proresraw_rs_amd.api import Stage
+    net = L.Net(L.init_params(7))
+    # Noise feature outside the trained range.
+    net.set_range(10.0, 11.0)
+    got, plain = develop_pair(torch, L, frames[0], Stage.WhiteBalanced, net, check_tiles=0)
+    assert net.last_check()[0]
+    assert np.array_equal(got, plain)
+    # A sampled-tile floor no network reaches.
+    net.set_range(0.0, 10.0)
+    got, plain = develop_pair(torch, L, frames[1], Stage.Rec2020, net, check_tiles=4, min_db=500.0)
+    used, worst = net.last_check()
+    assert used and worst < 500.0
+    assert np.array_equal(got, plain)
+
+
+def test_zero_network_is_the_bilinear_base(env):
+    torch, L, frames = env
+    from proresraw_rs_amd.api import Stage
+    net = L.Net(np.zeros(NO.PARAMS, np.float32))
+    net.set_range(0.0, 10.0)
+    f = frames[0]
+    got, plain = develop_pair(torch, L, f, Stage.Demosaiced, net, check_tiles=4, min_db=0.0)
+    used, worst = net.last_check()
+    assert not used
+    h = f.header
+    samples = f.samples.cpu().numpy().view(np.uint16).reshape(h.coded_height(), h.coded_width())
+    norm = NO.normalized(samples[:h.height, :h.width], list(h.curve), f.params.black,
+                         f.params.one_over_range)
+    base = NO.bilinear(NO.preclip(norm, 1.0, 1.0))
+    want = np.minimum(base, np.float32(1.0)).reshape(3, -1)
+    assert np.array_equal(got, want)
+    # The sampled-tile PSNR the kernel reports is the PSNR of those tiles.
+    assert 20.0 < worst < 80.0
