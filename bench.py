@@ -37,6 +37,8 @@ def parse_args():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-frames", type=int, default=64)
+    ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--learned-frames", type=int, default=32)
     return ap.parse_args()
 
 
@@ -156,6 +158,7 @@ def main():
                 "decode_tiles_per_s": round(info.nb_tiles / (t_decode_ms * 1e-3), 1)}
 
     batched = None if args.no_batch else batched_bench(args)
+    learned = None if args.no_learned else learned_bench(args, value)
 
     cpu = None
     if not args.no_cpu:
@@ -180,6 +183,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "batched": batched,
+        "batched_learned": learned,
     }
     if cpu:
         line["gpu_over_cpu"] = round(value / cpu["value"], 1)
@@ -231,6 +235,9 @@ def batched_bench(args):
     s.synchronize()
     for slot in b.slots:
         slot.check()
+    tuned = b.autotune(s)
+    b.run(s)
+    s.synchronize()
     g = torch.cuda.CUDAGraph()
     with torch.cuda.stream(s):
         with torch.cuda.graph(g, stream=s):
@@ -250,7 +257,114 @@ def batched_bench(args):
             "full_develop_frames": n_full, "ms_per_batch": round(el * 1e3, 3),
             "gsensels": round(sensels / 1e9, 4), "tile_groups": info["tile_groups"],
             "persistent_grid": info["grid"], "lds_bytes": info["lds_bytes"],
-            "note": "learned demosaic not yet in this path"}
+            "autotuned": {"spw": tuned["spw"], "chunk": tuned["chunk"], "waves": tuned["waves"],
+                          "cached": tuned["cached"],
+                          "trial_ms": [round(t["ms"], 3) for t in tuned.get("trials", [])]}}
+
+
+def learned_bench(args, plain_value):
+    """The end-to-end batched + learned configuration: 32 frames alternating
+    8192x4320 (v1, alignment 4, qscale 15) and 5760x3240 (v0, alignment 3,
+    qscale 45), RGGB. One persistent batched decode (entropy decode + IDCT +
+    linearize) of all frames, then per frame the learned demosaic + white
+    balance + Rec.2020 matrix with the on-device fallback to plain develop
+    (noise outside the trained range, or any of 8 sampled 64x64 tiles below
+    45 dB against plain develop). The same batch with plain develop is timed
+    beside it. Two distinct packets of each size are encoded and cycled."""
+    import torch
+
+    import proresraw_rs_amd as P
+    from proresraw_rs_amd import _native as N
+    from proresraw_rs_amd import learned as L
+    from proresraw_rs_amd import synth as S
+    from proresraw_rs_amd.batch import Batch
+    from proresraw_rs_amd.workloads import seeded_curve
+
+    kinds = [dict(size=(8192, 4320), version=1, alignment=4, q=15, vendor=b"aprh"),
+             dict(size=(5760, 3240), version=0, alignment=3, q=45, vendor=b"aprn")]
+    distinct = []
+    for i in range(4):
+        k = kinds[i % 2]
+        sc = S.scene(*k["size"], 0, seed=3000 + i, version=k["version"],
+                     alignment=k["alignment"], qscale=(k["q"], k["q"]),
+                     curve=seeded_curve(40 + i) if k["version"] == 1 else None)
+        distinct.append(S.encode_scene(sc))
+    b = Batch()
+    frames = []
+    sensels = 0
+    for j in range(args.learned_frames):
+        pk = distinct[j % 4]
+        slot = b.add_full(pk, P.Idct.Apple, linear=True)
+        h = slot.header
+        sensels += h.width * h.height
+        planes = [torch.empty(h.width * h.height, dtype=torch.float32, device="cuda")
+                  for _ in range(3)]
+        frames.append((slot, P.DemosaicParams.new(h, 0), planes))
+    params, lo, hi = L.load_weights()
+    net = L.Net(params)
+    net.set_range(lo, hi)
+    lib, ctx = N.lib(), P.context()
+    s = torch.cuda.Stream()
+    hs = ctypes.c_void_p(s.cuda_stream)
+    err = N.PrrError()
+
+    def learned_step():
+        b.run(s)
+        for slot, prm, planes in frames:
+            net.develop(slot.header, slot.samples, prm, P.Stage.Rec2020, planes, status=slot.status,
+                        check_tiles=8, min_db=45.0, stream=s)
+
+    def plain_step():
+        b.run(s)
+        for slot, prm, planes in frames:
+            rc = lib.prr_develop_frame_device(ctx.handle, ctypes.byref(slot.header._raw),
+                                              slot.samples.data_ptr(), slot.status.data_ptr(), 0,
+                                              0.0, 0.0, int(P.Stage.Rec2020), planes[0].data_ptr(),
+                                              planes[1].data_ptr(), planes[2].data_ptr(), hs,
+                                              ctypes.byref(err))
+            if rc:
+                raise P.Error(err)
+
+    with torch.cuda.stream(s):
+        b.run(s)
+        s.synchronize()
+        for slot, _, _ in frames:
+            slot.check()
+        b.autotune(s)
+        # Fallbacks and sampled-tile PSNR, frame by frame (eager, once).
+        fallbacks, worst = 0, float("inf")
+        b.run(s)
+        for slot, prm, planes in frames:
+            net.develop(slot.header, slot.samples, prm, P.Stage.Rec2020, planes,
+                        status=slot.status, check_tiles=8, min_db=45.0, stream=s)
+            used, psnr = net.last_check()
+            fallbacks += int(used)
+            worst = min(worst, psnr)
+        results = {}
+        for name, fn in (("learned", learned_step), ("plain", plain_step)):
+            fn()
+            s.synchronize()
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g, stream=s):
+                fn()
+            g.replay()
+            s.synchronize()
+            reps = 3
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                g.replay()
+            s.synchronize()
+            results[name] = (time.perf_counter() - t0) / reps
+    v_learned = sensels / results["learned"] / 1e9
+    v_plain = sensels / results["plain"] / 1e9
+    return {"value": round(v_learned, 3), "unit": "Gsensels/s", "frames": args.learned_frames,
+            "gsensels": round(sensels / 1e9, 4), "ms_per_batch": round(results["learned"] * 1e3, 3),
+            "batched_plain_value": round(v_plain, 3),
+            "vs_batched_plain": round(v_learned / v_plain, 3),
+            "vs_plain_gpu_path": round(v_learned / plain_value, 3),
+            "fallback_frames": fallbacks, "min_sampled_tile_psnr_db": round(worst, 2),
+            "workload": "32 frames alternating 8192x4320 v1 a4 q15 and 5760x3240 v0 a3 q45, "
+                        "batched decode+linearize, learned demosaic+WB+Rec2020 with fallback"}
 
 
 def cpu_baseline(packet, seconds):

@@ -243,9 +243,11 @@ int prr_batch_create(prr_context *ctx, prr_batch **out, prr_error *err);
 /* Add a frame (at most 256 per batch); `hdr`/`info` from prr_parse_frame_info. */
 int prr_batch_add(prr_batch *batch, const prr_header *hdr, const prr_frame_info *info,
                   const prr_batch_frame *frame, prr_error *err);
-/* Launch shape: streams per workgroup (16/32/64, before adding frames),
- * queue entries taken per atomic grab, waves per CU of the persistent
- * launch (0 = as many as fit). */
+/* Launch shape: streams per workgroup (16/32/64; frames already added are
+ * laid out again), queue entries taken per atomic grab, waves per CU of the
+ * persistent launch (0 = as many as fit); 0 keeps the current spw / chunk.
+ * Takes effect at the next prr_batch_run, which then uploads the frame
+ * table again (capture a graph only after that run). */
 int prr_batch_set_config(prr_batch *batch, uint32_t streams_per_wg, uint32_t chunk,
                          uint32_t waves_per_cu, prr_error *err);
 /* Enqueue the whole batch on `stream`. The first call after adding frames

@@ -85,7 +85,9 @@ int prr_net_develop_device(prr_net *net, const prr_header *hdr, const uint16_t *
                            int stage, uint32_t check_tiles, float min_db, float *d_r, float *d_g,
                            float *d_b, void *stream, prr_error *err);
 /* After the stream has run the last prr_net_develop_device: whether it
- * fell back to plain develop, and the lowest sampled-tile PSNR (dB). */
+ * fell back to plain develop, and the lowest sampled-tile PSNR (dB).
+ * A prr_net holds one set of activation, check and flag buffers: calls on
+ * it must be ordered (one stream), or they race. */
 int prr_net_last_check(prr_net *net, uint32_t *used_plain, float *min_psnr, prr_error *err);
 
 #ifdef __cplusplus

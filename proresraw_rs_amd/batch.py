@@ -2,6 +2,9 @@
 version, alignment and qscale in one persistent launch (prr_batch_* of the
 C ABI). torch only allocates device memory and provides the stream."""
 import ctypes
+import hashlib
+import json
+import os
 
 import numpy as np
 
@@ -132,6 +135,74 @@ class Batch:
         err = N.PrrError()
         self._check(self.lib.prr_batch_run(self.handle, ctypes.c_void_p(s.cuda_stream),
                                            ctypes.byref(err)), err)
+
+    def signature(self) -> str:
+        """What the best launch shape depends on: every frame's size,
+        alignment, mode and tile count."""
+        parts = []
+        for s in self.slots:
+            h = s.header
+            parts.append(f"{h.width}x{h.height}a{h.alignment}k{s.kind}s{s.scale or 0}"
+                         f"t{s.info.nb_tiles}")
+        return hashlib.sha1(";".join(parts).encode()).hexdigest()[:16]
+
+    def time_run(self, stream=None, reps=3):
+        """Mean milliseconds of `reps` runs after one untimed run (which also
+        uploads the frame table after a configuration change)."""
+        torch = _torch()
+        s = stream if stream is not None else torch.cuda.current_stream()
+        self.run(s)
+        s.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(s)
+        for _ in range(reps):
+            self.run(s)
+        e1.record(s)
+        e1.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    def autotune(self, stream=None, spws=(16, 32, 64), chunks=(1, 2, 4, 8), waves=(0,),
+                 reps=3, cache=None):
+        """Pick streams per workgroup, queue chunk and waves per CU by timing
+        each candidate on this batch; the choice is kept in a JSON cache
+        (PRR_AUTOTUNE_CACHE, else ~/.cache/proresraw_amd/batch_tuning.json)
+        under the batch's signature and reused without timing. Returns
+        {"spw", "chunk", "waves", "ms", "cached", "trials"}."""
+        path = cache or os.environ.get("PRR_AUTOTUNE_CACHE") or os.path.join(
+            os.path.expanduser("~"), ".cache", "proresraw_amd", "batch_tuning.json")
+        key = self.signature()
+        table = {}
+        if os.path.exists(path):
+            try:
+                with open(path) as f:
+                    table = json.load(f)
+            except (OSError, ValueError):
+                table = {}
+        if key in table:
+            best = dict(table[key])
+            self.set_config(best["spw"], best["chunk"], best["waves"])
+            best["cached"] = True
+            return best
+        trials = []
+        for spw in spws:
+            for chunk in chunks:
+                for w in waves:
+                    self.set_config(spw, chunk, w)
+                    trials.append({"spw": spw, "chunk": chunk, "waves": w,
+                                   "ms": self.time_run(stream, reps)})
+        best = dict(min(trials, key=lambda t: t["ms"]))
+        self.set_config(best["spw"], best["chunk"], best["waves"])
+        table[key] = best
+        try:
+            os.makedirs(os.path.dirname(path), exist_ok=True)
+            tmp = path + ".tmp"
+            with open(tmp, "w") as f:
+                json.dump(table, f, indent=1)
+            os.replace(tmp, path)
+        except OSError:
+            pass
+        best.update(cached=False, trials=trials)
+        return best
 
     def info(self):
         n, g, l = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()

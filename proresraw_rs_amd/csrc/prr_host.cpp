@@ -744,9 +744,16 @@ extern "C" int prr_develop_into(prr_context *ctx, const prr_header *hdr, const u
 /* ------------------------------------------------------------------ */
 /* Batches: many frames, one persistent decode launch. */
 
+/* What a frame's tile-group layout depends on, kept so the layout can be
+ * redone when the streams per workgroup change. */
+struct BatchShape {
+    uint32_t per_block, alignment, nb_tiles;
+};
+
 struct prr_batch {
     prr_context *ctx = nullptr;
     std::vector<BatchFrame> frames;
+    std::vector<BatchShape> shapes;
     std::vector<BatchDevelop> jobs;
     std::vector<size_t> offset_base; /* per frame, in u64 entries */
     size_t offsets_total = 0;
@@ -794,6 +801,26 @@ static uint32_t frame_spw(uint32_t want, uint32_t per_block, uint32_t amax, uint
     }
 }
 
+/* Tile groups of every frame for the batch's streams per workgroup: each
+ * frame's spw (shrunk until its LDS fits), first queue item and the
+ * batch's LDS size. */
+static void relayout(prr_batch *b)
+{
+    b->n_items = 0;
+    b->lds = 0;
+    for (size_t i = 0; i < b->frames.size(); i++) {
+        BatchFrame &bf = b->frames[i];
+        const BatchShape &sh = b->shapes[i];
+        size_t lds;
+        bf.spw = frame_spw(b->spw, sh.per_block, sh.alignment, b->ctx->launch.lds_limit, lds);
+        const uint32_t T = bf.spw / 4;
+        bf.first_item = b->n_items;
+        b->n_items += (sh.nb_tiles + T - 1) / T;
+        b->lds = std::max(b->lds, lds);
+    }
+    b->uploaded = false;
+}
+
 extern "C" int prr_batch_add(prr_batch *b, const prr_header *hdr, const prr_frame_info *info,
                              const prr_batch_frame *fr, prr_error *err)
 {
@@ -824,20 +851,15 @@ extern "C" int prr_batch_add(prr_batch *b, const prr_header *hdr, const prr_fram
     } else {
         return fail(err, PRR_E_INVALID_ARGUMENT, "kind", (uint64_t)fr->kind);
     }
-    size_t lds;
-    bf.spw = frame_spw(b->spw, per_block, hdr->alignment, b->ctx->launch.lds_limit, lds);
     bf.flags = fr->d_tile_flags;
     bf.status = fr->d_status;
-    uint32_t T = bf.spw / 4;
-    uint32_t groups = (info->nb_tiles + T - 1) / T;
-    if (groups >= (1u << 20))
+    if ((info->nb_tiles + 3) / 4 >= (1u << 20))
         return fail(err, PRR_E_INVALID_ARGUMENT, "too many tile groups in one frame");
-    bf.first_item = b->n_items;
-    b->n_items += groups;
     b->offset_base.push_back(b->offsets_total);
     b->offsets_total += (size_t)info->nb_tiles + 1;
-    b->lds = std::max(b->lds, lds);
     b->frames.push_back(bf);
+    b->shapes.push_back({per_block, hdr->alignment, info->nb_tiles});
+    relayout(b);
     if (fr->kind == PRR_BATCH_DEVELOP) {
         if (!fr->d_r || (fr->d_g == nullptr) != (fr->d_b == nullptr))
             return fail(err, PRR_E_INVALID_ARGUMENT, "planes");
@@ -862,8 +884,6 @@ extern "C" int prr_batch_set_config(prr_batch *b, uint32_t streams_per_wg, uint3
 {
     if (!b)
         return fail(err, PRR_E_INVALID_ARGUMENT, "batch");
-    if (!b->frames.empty() && streams_per_wg && streams_per_wg != b->spw)
-        return fail(err, PRR_E_INVALID_ARGUMENT, "set streams_per_wg before adding frames");
     if (streams_per_wg) {
         if (streams_per_wg != 16 && streams_per_wg != 32 && streams_per_wg != 64)
             return fail(err, PRR_E_INVALID_ARGUMENT, "streams_per_wg", streams_per_wg);
@@ -872,7 +892,7 @@ extern "C" int prr_batch_set_config(prr_batch *b, uint32_t streams_per_wg, uint3
     if (chunk)
         b->chunk = std::min<uint32_t>(chunk, 64);
     b->waves_per_cu = waves_per_cu;
-    b->uploaded = false;
+    relayout(b);
     return PRR_OK;
 }
 

@@ -7,6 +7,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <tuple>
 #include <vector>
 
 #include "../../include/proresraw_learned.h"
@@ -55,8 +57,11 @@ struct prr_net {
     Buf frames, crops, x0, h[7], r[7], out, dout, dh[2], dz, part, loss_part, loss;
     uint32_t n_frames = 0, steps = 0, batch = 0, cells = 0, margin = 0;
     /* inference */
-    Buf ix0, ia, ib, iout, tiles, scratch, psnr, flag;
-    uint32_t tiles_w = 0, tiles_h = 0, n_tiles = 0, last_k = 0;
+    Buf ix0, ia, ib, iout, scratch, psnr, flag;
+    /* Sampled-tile lists per (width, height, count), uploaded on first use
+     * so that later calls (and graph capture) copy nothing. */
+    std::map<std::tuple<uint32_t, uint32_t, uint32_t>, Buf> tiles;
+    uint32_t last_k = 0;
 };
 
 static int grow(prr_net *net, Buf &b, size_t bytes, prr_error *err)
@@ -131,7 +136,9 @@ extern "C" void prr_net_destroy(prr_net *net)
                               &net->step, &net->frames, &net->crops, &net->x0, &net->out,
                               &net->dout, &net->dh[0], &net->dh[1], &net->dz, &net->part,
                               &net->loss_part, &net->loss, &net->ix0, &net->ia, &net->ib,
-                              &net->iout, &net->tiles, &net->scratch, &net->psnr, &net->flag};
+                              &net->iout, &net->scratch, &net->psnr, &net->flag};
+    for (auto &kv : net->tiles)
+        all.push_back(&kv.second);
     for (int l = 0; l < 7; l++) {
         all.push_back(&net->h[l]);
         all.push_back(&net->r[l]);
@@ -421,18 +428,19 @@ extern "C" int prr_net_develop_device(prr_net *net, const prr_header *hdr,
     GROW(net->ib, 2 * cells * C);
     GROW(net->iout, 4 * cells * C);
     const uint32_t k = std::min<uint32_t>(check_tiles, 64);
-    GROW(net->tiles, sizeof(uint32_t) * 2 * 64);
     GROW(net->scratch, sizeof(float) * 3 * 4096 * 64);
     GROW(net->psnr, sizeof(float) * 64);
-    if (k && (net->tiles_w != w || net->tiles_h != h || net->n_tiles != k)) {
-        std::vector<uint32_t> t = sample_tiles(w, h, k);
-        NET_TRY(hipDeviceSynchronize(), "hipDeviceSynchronize");
-        NET_TRY(hipMemcpy(net->tiles.p, t.data(), sizeof(uint32_t) * t.size(),
-                          hipMemcpyHostToDevice),
-                "hipMemcpy tiles");
-        net->tiles_w = w;
-        net->tiles_h = h;
-        net->n_tiles = k;
+    const uint32_t *d_tiles = nullptr;
+    if (k) {
+        Buf &tb = net->tiles[std::make_tuple(w, h, k)];
+        if (!tb.p) {
+            std::vector<uint32_t> t = sample_tiles(w, h, k);
+            GROW(tb, sizeof(uint32_t) * t.size());
+            NET_TRY(hipDeviceSynchronize(), "hipDeviceSynchronize");
+            NET_TRY(hipMemcpy(tb.p, t.data(), sizeof(uint32_t) * t.size(), hipMemcpyHostToDevice),
+                    "hipMemcpy tiles");
+        }
+        d_tiles = tb.as<uint32_t>();
     }
     uint32_t *flag = net->flag.as<uint32_t>();
     const uint16_t *wf = net->wf.as<uint16_t>();
@@ -457,7 +465,7 @@ extern "C" int prr_net_develop_device(prr_net *net, const prr_header *hdr,
                         net->iout.as<float>(), s),
             "conv kernel");
     NET_TRY(launch_compose(f, fin, net->iout.as<float>(), d_r, d_g, d_b, s), "compose kernel");
-    NET_TRY(launch_check(d, d_r, d_g, d_b, net->tiles.as<uint32_t>(), k, net->scratch.as<float>(),
+    NET_TRY(launch_check(d, d_r, d_g, d_b, d_tiles, k, net->scratch.as<float>(),
                          min_db, net->psnr.as<float>(), flag, s),
             "check kernels");
     d.gate = flag;

@@ -202,3 +202,35 @@ def test_zero_network_is_the_bilinear_base(env):
     assert np.array_equal(got, want)
     # The sampled-tile PSNR the kernel reports is the PSNR of those tiles.
     assert 20.0 < worst < 80.0
+
+
+def test_trained_network_passes_the_floor_and_beats_bilinear(env):
+    """The committed weights on 5760x3240 frames of seeds outside the
+    training set: at least 45 dB against plain develop at
+    Stage::Demosaiced over the whole frame, a clear gain over the bilinear
+    base alone (the method without the network); with the sampled-tile
+    check on, the output is the network's exactly when every sampled tile
+    reaches 45 dB, else plain develop."""
+    torch, L, _ = env
+    from proresraw_rs_amd import synth as S
+    from proresraw_rs_amd.api import Stage
+    params, lo, hi = L.load_weights()
+    net = L.Net(params)
+    net.set_range(lo, hi)
+    zero = L.Net(np.zeros(NO.PARAMS, np.float32))
+    zero.set_range(0.0, 100.0)
+    for q, pattern in ((15, 0), (45, 3)):
+        sc = S.scene(5760, 3240, pattern, seed=987654 + q, version=0, alignment=3, qscale=(q, q),
+                     white_level=4095)
+        f = L.frame_pair(S.encode_scene(sc))
+        got, plain = develop_pair(torch, L, f, Stage.Demosaiced, net, check_tiles=0)
+        base, _ = develop_pair(torch, L, f, Stage.Demosaiced, zero, check_tiles=0)
+        p_net, p_base = L.psnr(got, plain), L.psnr(base, plain)
+        checked, _ = develop_pair(torch, L, f, Stage.Demosaiced, net, check_tiles=8, min_db=45.0)
+        used, worst = net.last_check()
+        print(f"q{q} pattern {pattern}: learned {p_net:.2f} dB, bilinear {p_base:.2f} dB, "
+              f"sampled worst {worst:.2f} dB, fallback {used}")
+        assert p_net >= 45.0
+        assert p_net >= p_base + 4.0
+        assert used == (worst < 45.0)
+        assert np.array_equal(checked, plain if used else got)
