@@ -59,6 +59,11 @@ int prr_net_set_range(prr_net *net, float feature_lo, float feature_hi, prr_erro
 int prr_net_train_setup(prr_net *net, const prr_net_frame *frames, uint32_t n_frames,
                         const uint32_t *crops, uint32_t steps, uint32_t batch, uint32_t cells,
                         uint32_t margin, prr_error *err);
+/* Learning-rate schedule of the following steps: the rate given to
+ * prr_net_train_step decays along a cosine to lr_min over decay_steps
+ * steps (0, the default, keeps it constant); computed on the device from
+ * the step count. */
+int prr_net_set_schedule(prr_net *net, float lr_min, uint32_t decay_steps, prr_error *err);
 /* One step: pack, forward, loss, backward, Adam (lr) unless lr is 0, step
  * count + 1. Enqueued on `stream`; allocates nothing. */
 int prr_net_train_step(prr_net *net, float lr, void *stream, prr_error *err);
@@ -78,8 +83,11 @@ int prr_net_read(prr_net *net, int what, uint32_t layer, void *host, size_t byte
  * of width*height floats. The noise feature comes from `params`, or from
  * d_status (its max_qscale) when that is not NULL. The output is the plain
  * develop when the feature is outside the trained range, or when any of
- * `check_tiles` sampled 64x64 tiles (0 = none) has a PSNR against plain
- * develop below min_db; the choice is made on the device. */
+ * `check_tiles` sampled 64x64 tiles (0 = none) has a PSNR (peak 1) below
+ * min_db between the network's demosaic and the plain one, both white
+ * balanced and clipped as the stage says but before the output matrix
+ * (which is exact and the same for both); the choice is made on the
+ * device. */
 int prr_net_develop_device(prr_net *net, const prr_header *hdr, const uint16_t *d_samples,
                            const prr_demosaic_params *params, const prr_device_status *d_status,
                            int stage, uint32_t check_tiles, float min_db, float *d_r, float *d_g,

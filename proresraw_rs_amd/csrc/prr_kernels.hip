@@ -1325,51 +1325,6 @@ __global__ __launch_bounds__(256) void k_develop_sampled(DevelopDesc d, const ui
                  2 * DTH * by2, 64);
 }
 
-/* PSNR (peak 1) of the planes against the sampled plain tiles; any tile
- * below min_db sets *use_plain. One workgroup per tile. */
-__global__ __launch_bounds__(256) void k_check_psnr(uint32_t width, uint32_t height,
-                                                    const float *r, const float *g,
-                                                    const float *b, const uint32_t *tiles,
-                                                    const float *scratch, float min_db,
-                                                    float *psnr, uint32_t *use_plain)
-{
-    __shared__ double s_sum[256];
-    __shared__ uint32_t s_n[256];
-    const uint32_t k = blockIdx.x;
-    const uint32_t x0 = DTW * tiles[2 * k], y0 = 2 * DTH * tiles[2 * k + 1];
-    const float *base = scratch + (size_t)k * 3 * 4096;
-    double sum = 0.0;
-    uint32_t n = 0;
-    for (uint32_t i = threadIdx.x; i < 4096; i += 256) {
-        const uint32_t y = y0 + i / 64, x = x0 + i % 64;
-        if (y >= height || x >= width)
-            continue;
-        const size_t at = (size_t)y * width + x;
-        const double e0 = (double)r[at] - base[i], e1 = (double)g[at] - base[4096 + i],
-                     e2 = (double)b[at] - base[8192 + i];
-        sum += e0 * e0 + e1 * e1 + e2 * e2;
-        n += 3;
-    }
-    s_sum[threadIdx.x] = sum;
-    s_n[threadIdx.x] = n;
-    __syncthreads();
-    for (uint32_t s = 128; s > 0; s >>= 1) {
-        if (threadIdx.x < s) {
-            s_sum[threadIdx.x] += s_sum[threadIdx.x + s];
-            s_n[threadIdx.x] += s_n[threadIdx.x + s];
-        }
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        const double mse = s_n[0] ? s_sum[0] / s_n[0] : 0.0;
-        const float db = mse > 0.0 ? (float)(-10.0 * log10(mse)) : 200.0f;
-        psnr[k] = db;
-        if (!(db >= min_db))
-            atomicOr(use_plain, 1u);
-    }
-}
-
-
 /* Load a struct the whole wave agrees on into scalar registers. */
 template <class T>
 __device__ inline T load_uniform(const T *p)
@@ -1639,15 +1594,12 @@ hipError_t launch_batch(const BatchFrame *d_frames, uint32_t n_frames, uint32_t 
 const void *batch_decode_kernel() { return (const void *)k_batch_decode; }
 
 namespace net {
-hipError_t launch_check(const DevelopDesc &d, const float *r, const float *g, const float *b,
-                        const uint32_t *tiles, uint32_t n_tiles, float *scratch, float min_db,
-                        float *psnr, uint32_t *use_plain, hipStream_t s)
+hipError_t launch_develop_sampled(const DevelopDesc &d, const uint32_t *tiles, uint32_t n_tiles,
+                                  float *scratch, hipStream_t s)
 {
     if (n_tiles == 0)
         return hipSuccess;
     hipLaunchKernelGGL(k_develop_sampled, dim3(2 * n_tiles), dim3(256), 0, s, d, tiles, scratch);
-    hipLaunchKernelGGL(k_check_psnr, dim3(n_tiles), dim3(256), 0, s, d.width, d.height, r, g, b,
-                       tiles, scratch, min_db, psnr, use_plain);
     return hipGetLastError();
 }
 } // namespace net

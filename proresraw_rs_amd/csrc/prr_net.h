@@ -72,8 +72,10 @@ hipError_t launch_wgrad(const uint16_t *x, const uint16_t *g, int n, int h, int 
                         float *grad, hipStream_t s);
 hipError_t launch_relu_grad(const uint16_t *dh, const uint16_t *mask, uint16_t *dz, size_t count,
                             hipStream_t s);
+/* Adam at lr, decayed along a cosine to lr_min over `decay` steps (0 =
+ * constant), the step count read on the device. */
 hipError_t launch_adam(float *params, const float *grads, float *m, float *v, uint32_t *step,
-                       float lr, hipStream_t s);
+                       float lr, float lr_min, uint32_t decay, hipStream_t s);
 /* Training: crops[(step % steps) * n + k] (step read from *step) packed
  * into x0. */
 hipError_t launch_pack_crops(const NetFrame *frames, const Crop *crops, const uint32_t *step,
@@ -85,13 +87,18 @@ hipError_t launch_loss(const NetFrame *frames, const Crop *crops, const uint32_t
 hipError_t launch_pack_frame(const NetFrame &f, uint16_t *x0, hipStream_t s);
 hipError_t launch_compose(const NetFrame &f, const Finish &fin, const float *out, float *r,
                           float *g, float *b, hipStream_t s);
-/* Sampled-tile check: PSNR of the network's planes against plain develop
- * over 64x64 tiles (tiles[2k], tiles[2k+1] = column of 64, row of 64)
- * through scratch (n_tiles * 3 * 4096 floats); sets *use_plain when any is
- * below min_db. */
-hipError_t launch_check(const DevelopDesc &d, const float *r, const float *g, const float *b,
-                        const uint32_t *tiles, uint32_t n_tiles, float *scratch, float min_db,
-                        float *psnr, uint32_t *use_plain, hipStream_t s);
+/* Plain develop of sampled 64x64 tiles (tiles[2k], tiles[2k+1] = column
+ * of 64, row of 64) into scratch (n_tiles * 3 * 4096 floats). */
+hipError_t launch_develop_sampled(const DevelopDesc &d, const uint32_t *tiles, uint32_t n_tiles,
+                                  float *scratch, hipStream_t s);
+/* Sampled-tile check of the demosaic: the plain develop of each tile at
+ * the stage's white balance and clip without the output matrix (d must
+ * have has_matrix = 0) against the network's output finished the same way;
+ * PSNR per tile into psnr, *use_plain set when any is below min_db. */
+hipError_t launch_check(const DevelopDesc &d, const NetFrame &f, const Finish &fin,
+                        const float *out, const uint32_t *tiles, uint32_t n_tiles,
+                        float *scratch, float min_db, float *psnr, uint32_t *use_plain,
+                        hipStream_t s);
 hipError_t launch_gate_clear(uint32_t *flag, uint32_t value, hipStream_t s);
 /* Sets *use_plain when the frame's noise feature lies outside [lo, hi]. */
 hipError_t launch_noise_gate(const NetFrame &f, float lo, float hi, uint32_t *use_plain,

@@ -275,13 +275,21 @@ __global__ void k_relu_grad(const uint16_t *__restrict__ dh, const uint16_t *__r
 }
 
 /* Adam (betas 0.9 / 0.999, eps 1e-8); the step count is read on the
- * device so a captured graph replays correctly. */
+ * device so a captured graph replays correctly. With decay > 0 the rate
+ * follows lr_min + (lr - lr_min) * (1 + cos(pi * min(step, decay) / decay)) / 2. */
 __global__ void k_adam(float *__restrict__ p, const float *__restrict__ g, float *__restrict__ m,
-                       float *__restrict__ v, const uint32_t *__restrict__ step, float lr)
+                       float *__restrict__ v, const uint32_t *__restrict__ step, float lr0,
+                       float lr_min, uint32_t decay)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= PARAMS)
         return;
+    float lr = lr0;
+    if (decay) {
+        const double u = (double)min(*step, decay) / (double)decay;
+        lr = (float)((double)lr_min +
+                     ((double)lr0 - (double)lr_min) * 0.5 * (1.0 + cos(3.141592653589793 * u)));
+    }
     const double t = (double)(*step + 1);
     const float c1 = (float)(1.0 - pow(0.9, t));
     const float c2 = (float)(1.0 - pow(0.999, t));
@@ -542,7 +550,77 @@ __global__ __launch_bounds__(256) void k_compose(NetFrame f, Finish fin,
         }
 }
 
+/* Sampled-tile check: per tile, the network's output at each sensel
+ * (out + bilinear, white balance and clip at 1, no matrix) against the
+ * plain develop in scratch; PSNR with peak 1. One workgroup per tile. */
+__global__ __launch_bounds__(256) void k_check_psnr(NetFrame f, Finish fin,
+                                                    const float *__restrict__ out,
+                                                    const uint32_t *__restrict__ tiles,
+                                                    const float *__restrict__ scratch,
+                                                    float min_db, float *__restrict__ psnr,
+                                                    uint32_t *__restrict__ use_plain)
+{
+    __shared__ double s_sum[256];
+    __shared__ uint32_t s_n[256];
+    const uint32_t k = blockIdx.x;
+    const uint32_t x0 = 64 * tiles[2 * k], y0 = 64 * tiles[2 * k + 1];
+    const float *plain = scratch + (size_t)k * 3 * 4096;
+    const int w2 = (int)f.width / 2;
+    double sum = 0.0;
+    uint32_t n = 0;
+    for (uint32_t i = threadIdx.x; i < 4096; i += 256) {
+        const uint32_t sy = y0 + i / 64, sx = x0 + i % 64;
+        if (sy >= f.height || sx >= f.width)
+            continue;
+        const int y = (int)(f.flip_y ? f.height - 1 - sy : sy);
+        const int x = (int)(f.flip_x ? f.width - 1 - sx : sx);
+        float base[3];
+        bilinear(f, 0, 0, (int)f.height, (int)f.width, y, x, base);
+        const size_t cell = (size_t)(y >> 1) * w2 + (x >> 1);
+        const int ch = 3 * (2 * (y & 1) + (x & 1));
+        const float r = fminf((out[cell * C + ch + 0] + base[0]) * fin.wb[0], 1.0f);
+        const float g = fminf(out[cell * C + ch + 1] + base[1], 1.0f);
+        const float b = fminf((out[cell * C + ch + 2] + base[2]) * fin.wb[1], 1.0f);
+        const double e0 = (double)r - plain[i], e1 = (double)g - plain[4096 + i],
+                     e2 = (double)b - plain[8192 + i];
+        sum += e0 * e0 + e1 * e1 + e2 * e2;
+        n += 3;
+    }
+    s_sum[threadIdx.x] = sum;
+    s_n[threadIdx.x] = n;
+    __syncthreads();
+    for (uint32_t s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            s_sum[threadIdx.x] += s_sum[threadIdx.x + s];
+            s_n[threadIdx.x] += s_n[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double mse = s_n[0] ? s_sum[0] / s_n[0] : 0.0;
+        const float db = mse > 0.0 ? (float)(-10.0 * log10(mse)) : 200.0f;
+        psnr[k] = db;
+        if (!(db >= min_db))
+            atomicOr(use_plain, 1u);
+    }
+}
+
 /* ------------------------------------------------------------------ */
+
+hipError_t launch_check(const DevelopDesc &d, const NetFrame &f, const Finish &fin,
+                        const float *out, const uint32_t *tiles, uint32_t n_tiles,
+                        float *scratch, float min_db, float *psnr, uint32_t *use_plain,
+                        hipStream_t s)
+{
+    if (n_tiles == 0)
+        return hipSuccess;
+    hipError_t e = launch_develop_sampled(d, tiles, n_tiles, scratch, s);
+    if (e != hipSuccess)
+        return e;
+    hipLaunchKernelGGL(k_check_psnr, dim3(n_tiles), dim3(256), 0, s, f, fin, out, tiles, scratch,
+                       min_db, psnr, use_plain);
+    return hipGetLastError();
+}
 
 hipError_t launch_weights(const float *params, uint16_t *wf, uint16_t *wt, hipStream_t s)
 {
@@ -610,10 +688,10 @@ hipError_t launch_relu_grad(const uint16_t *dh, const uint16_t *mask, uint16_t *
 }
 
 hipError_t launch_adam(float *params, const float *grads, float *m, float *v, uint32_t *step,
-                       float lr, hipStream_t s)
+                       float lr, float lr_min, uint32_t decay, hipStream_t s)
 {
     hipLaunchKernelGGL(k_adam, dim3((PARAMS + 255) / 256), dim3(256), 0, s, params, grads, m, v,
-                       step, lr);
+                       step, lr, lr_min, decay);
     return hipGetLastError();
 }
 

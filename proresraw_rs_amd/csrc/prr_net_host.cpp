@@ -56,6 +56,8 @@ struct prr_net {
     /* training */
     Buf frames, crops, x0, h[7], r[7], out, dout, dh[2], dz, part, loss_part, loss;
     uint32_t n_frames = 0, steps = 0, batch = 0, cells = 0, margin = 0;
+    float lr_min = 0.0f;
+    uint32_t decay = 0;
     /* inference */
     Buf ix0, ia, ib, iout, scratch, psnr, flag;
     /* Sampled-tile lists per (width, height, count), uploaded on first use
@@ -331,11 +333,22 @@ extern "C" int prr_net_train_step(prr_net *net, float lr, void *stream, prr_erro
     NET_TRY(launch_relu_grad(net->dh[a].as<uint16_t>(), H(0), dz, count, s), "relu grad");
     NET_TRY(launch_wgrad(net->x0.as<uint16_t>(), dz, n, hw, hw, part, G(0), s), "wgrad");
     if (lr > 0.0f) {
-        NET_TRY(launch_adam(params, grads, net->m.as<float>(), net->v.as<float>(), step, lr, s),
+        NET_TRY(launch_adam(params, grads, net->m.as<float>(), net->v.as<float>(), step, lr,
+                            net->lr_min, net->decay, s),
                 "adam kernel");
         NET_TRY(launch_weights(params, wf, wt, s), "weights kernel");
     }
     NET_TRY(launch_step_inc(step, s), "step kernel");
+    return PRR_OK;
+}
+
+extern "C" int prr_net_set_schedule(prr_net *net, float lr_min, uint32_t decay_steps,
+                                    prr_error *err)
+{
+    if (!net || !(lr_min >= 0.0f))
+        return net_fail(err, PRR_E_INVALID_ARGUMENT, "net or lr_min");
+    net->lr_min = lr_min;
+    net->decay = decay_steps;
     return PRR_OK;
 }
 
@@ -465,7 +478,9 @@ extern "C" int prr_net_develop_device(prr_net *net, const prr_header *hdr,
                         net->iout.as<float>(), s),
             "conv kernel");
     NET_TRY(launch_compose(f, fin, net->iout.as<float>(), d_r, d_g, d_b, s), "compose kernel");
-    NET_TRY(launch_check(d, d_r, d_g, d_b, d_tiles, k, net->scratch.as<float>(),
+    DevelopDesc dc = d;
+    dc.has_matrix = 0;
+    NET_TRY(launch_check(dc, f, fin, net->iout.as<float>(), d_tiles, k, net->scratch.as<float>(),
                          min_db, net->psnr.as<float>(), flag, s),
             "check kernels");
     d.gate = flag;

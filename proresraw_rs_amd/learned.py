@@ -107,18 +107,22 @@ def frame_pair(packet: bytes, device="cuda") -> TrainFrame:
 
 
 def scene_packets(n: int, size, seed: int, qscales=(15, 45), workers: int = 16,
-                  pattern: int = 0) -> List[tuple]:
+                  pattern: int = 0, curves: bool = False) -> List[tuple]:
     """(packet, qscale) of n seeded RGGB scenes (config-0 recipe: 8
     gradients, 200 discs, Poisson-Gaussian noise, v0, default curve,
-    white 4095), qscale drawn uniformly per scene. Encoded on the host in
+    white 4095), qscale drawn uniformly per scene; with curves, every other
+    scene is v1 with a seeded 8-point curve instead. Encoded on the host in
     parallel (the encoder releases the GIL)."""
+    from .workloads import seeded_curve
     w, h = size
     rng = np.random.default_rng(seed)
     qs = rng.integers(qscales[0], qscales[1] + 1, size=n)
 
     def one(i):
-        sc = S.scene(w, h, pattern, seed=seed * 1000003 + i + 1, version=0, alignment=2,
-                     qscale=(int(qs[i]), int(qs[i])), white_level=4095, threads=1)
+        custom = curves and i % 2 == 1
+        sc = S.scene(w, h, pattern, seed=seed * 1000003 + i + 1, version=1 if custom else 0,
+                     alignment=2, qscale=(int(qs[i]), int(qs[i])), white_level=4095,
+                     curve=seeded_curve(seed * 7919 + i) if custom else None, threads=1)
         return S.encode_scene(sc), int(qs[i])
 
     with concurrent.futures.ThreadPoolExecutor(max_workers=workers) as ex:
@@ -173,6 +177,9 @@ class Net:
         self._call("prr_net_train_setup", arr, len(frames), crops.ctypes.data, steps, batch,
                    cells, margin)
         self.batch, self.cells = batch, cells
+
+    def set_schedule(self, lr_min: float, decay_steps: int):
+        self._call("prr_net_set_schedule", ctypes.c_float(lr_min), decay_steps)
 
     def train_step(self, lr: float, stream=None):
         torch = _torch()
@@ -233,13 +240,15 @@ def random_crops(rng, frames: List[TrainFrame], steps: int, batch: int, cells: i
 
 def train(net: Net, frames: List[TrainFrame], steps: int, batch: int = 16, cells: int = 144,
           margin: int = 8, lr: float = 1e-3, seed: int = 0, log_every: int = 1000,
-          log=print, use_graph: bool = True):
+          log=print, use_graph: bool = True, lr_min: Optional[float] = None):
     """Adam over random crops of 2*cells sensels (loss on the cells at least
     `margin` from the crop edge); one step captured in a graph and replayed.
-    Returns the list of (step, loss)."""
+    With lr_min, the rate decays from lr to lr_min along a cosine over the
+    run. Returns the list of (step, loss)."""
     torch = _torch()
     rng = np.random.default_rng(seed)
     net.train_setup(frames, random_crops(rng, frames, steps, batch, cells), cells, margin)
+    net.set_schedule(lr_min if lr_min is not None else 0.0, steps if lr_min is not None else 0)
     stream = torch.cuda.Stream()
     history = []
     t0 = time.time()

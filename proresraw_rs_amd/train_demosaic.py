@@ -3,16 +3,20 @@
     python -m proresraw_rs_amd.train_demosaic --out DIR [--scenes 2000]
         [--steps 20000] [--eval 50] [--eval-size 5760x3240]
 
-1. seeded synthetic training scenes (RGGB, v0, qscale uniform in [15, 45],
+1. seeded synthetic training scenes (RGGB, qscale uniform in [15, 45],
    8 gradients + 200 discs, Poisson-Gaussian noise), each decoded and
    developed at Stage::Demosaiced by the plain GPU path (bit-exact to the
    crate) into (mosaic, target) pairs kept on the device;
-2. Adam (lr 1e-3), batches of 16 random crops of 288x288 sensels (loss on
-   the central 256x256), one step captured in a graph and replayed;
+2. Adam (lr 1e-3, constant; --lr-min decays it along a cosine over the
+   run), batches of 16 random crops of 288x288 sensels (loss on the central
+   256x256), one step captured in a graph and replayed (--curves: every
+   other scene v1 with a seeded custom curve);
 3. held-out frames of other seeds: PSNR and max abs error of the learned
    develop against plain develop, per qscale, at Stage::Demosaiced and
-   Stage::WhiteBalanced (gains 2.0 / 1.6), next to the bilinear base alone
-   (the method without the network), and the time of both develops.
+   Stage::WhiteBalanced (gains 2.0 / 1.6) and Stage::Rec2020 (header
+   gains and matrix), next to the bilinear base alone (the method without
+   the network), the time of both develops, and the sampled-tile check
+   (8 tiles, 45 dB) the end-to-end path would apply.
 
 Writes DIR/demosaic_net.npz (parameters, trained feature range, loss
 history) and DIR/eval.json.
@@ -83,10 +87,16 @@ def evaluate(torch, net, zero, packets, stages, log=log):
                 mse = float((d * d).mean())
                 row[name] = {"psnr": (-10.0 * math.log10(mse)) if mse > 0 else float("inf"),
                              "max_abs": float(d.abs().max())}
+            checked = [torch.empty(n, dtype=torch.float32, device="cuda") for _ in range(3)]
+            net.develop(h, samples, params, stage, checked, check_tiles=8, min_db=45.0,
+                        stream=stream)
+            used, worst = net.last_check()
+            row["check"] = {"fallback": used, "worst_tile_psnr": worst}
             rows.append(row)
         log(f"eval frame {k} q{q}: " + ", ".join(
             f"{r['stage']} learned {r['learned']['psnr']:.2f} dB (max {r['learned']['max_abs']:.3f}) "
-            f"bilinear {r['bilinear']['psnr']:.2f} dB" for r in rows[-len(stages):]))
+            f"bilinear {r['bilinear']['psnr']:.2f} dB tiles>={r['check']['worst_tile_psnr']:.1f}"
+            f"{' FALLBACK' if r['check']['fallback'] else ''}" for r in rows[-len(stages):]))
     return rows
 
 
@@ -95,8 +105,9 @@ def summarize(rows):
     for r in rows:
         key = f"{r['stage']}/q{r['qscale']}"
         s = out.setdefault(key, {"frames": 0, "learned_psnr": [], "bilinear_psnr": [],
-                                 "learned_max_abs": 0.0})
+                                 "learned_max_abs": 0.0, "fallbacks": 0})
         s["frames"] += 1
+        s["fallbacks"] += int(r["check"]["fallback"])
         s["learned_psnr"].append(r["learned"]["psnr"])
         s["bilinear_psnr"].append(r["bilinear"]["psnr"])
         s["learned_max_abs"] = max(s["learned_max_abs"], r["learned"]["max_abs"])
@@ -116,6 +127,8 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=20000)
     ap.add_argument("--batch", type=int, default=16)
     ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--lr-min", type=float, default=0.0)
+    ap.add_argument("--curves", action="store_true")
     ap.add_argument("--eval", type=int, default=50)
     ap.add_argument("--eval-size", default="5760x3240")
     ap.add_argument("--seed", type=int, default=1)
@@ -126,7 +139,8 @@ def main(argv=None):
     os.makedirs(args.out, exist_ok=True)
     sw, sh = (int(v) for v in args.scene_size.split("x"))
     t0 = time.time()
-    packets = L.scene_packets(args.scenes, (sw, sh), args.seed, workers=args.workers)
+    packets = L.scene_packets(args.scenes, (sw, sh), args.seed, workers=args.workers,
+                              curves=args.curves)
     log(f"{args.scenes} training scenes encoded in {time.time() - t0:.1f} s")
     t0 = time.time()
     frames = [L.frame_pair(p) for p, _ in packets]
@@ -137,11 +151,13 @@ def main(argv=None):
     lo, hi = L.noise_feature(15), L.noise_feature(45)
     t0 = time.time()
     hist = L.train(net, frames, args.steps, batch=args.batch, cells=144, margin=8, lr=args.lr,
-                   seed=args.seed, log_every=1000, log=log)
+                   seed=args.seed, log_every=1000, log=log,
+                   lr_min=args.lr_min if args.lr_min > 0 else None)
     train_s = time.time() - t0
     params = net.params()
     L.save_weights(os.path.join(args.out, "demosaic_net.npz"), params, lo, hi,
-                   steps=args.steps, batch=args.batch, lr=args.lr, scenes=args.scenes,
+                   steps=args.steps, batch=args.batch, lr=args.lr, lr_min=args.lr_min,
+                   curves=args.curves, scenes=args.scenes,
                    loss_history=np.array(hist, np.float64))
     log(f"trained {args.steps} steps in {train_s:.1f} s")
     del frames
@@ -151,10 +167,12 @@ def main(argv=None):
     zero.set_range(0.0, 100.0)
     ew, eh = (int(v) for v in args.eval_size.split("x"))
     t0 = time.time()
-    held = L.scene_packets(args.eval, (ew, eh), args.seed + 7919, workers=args.workers)
+    held = L.scene_packets(args.eval, (ew, eh), args.seed + 7919, workers=args.workers,
+                           curves=args.curves)
     log(f"{args.eval} held-out frames encoded in {time.time() - t0:.1f} s")
     rows = evaluate(torch, net, zero, held,
-                    [(Stage.Demosaiced, None), (Stage.WhiteBalanced, (2.0, 1.6))])
+                    [(Stage.Demosaiced, None), (Stage.WhiteBalanced, (2.0, 1.6)),
+                     (Stage.Rec2020, None)])
     report = {"train_seconds": train_s, "steps": args.steps, "batch": args.batch,
               "scenes": args.scenes, "scene_size": args.scene_size, "eval_size": args.eval_size,
               "final_loss": hist[-1][1] if hist else None, "summary": summarize(rows),
