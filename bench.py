@@ -39,11 +39,51 @@ def parse_args():
     ap.add_argument("--batch-frames", type=int, default=64)
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--learned-frames", type=int, default=32)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device to device 10 times (the reference reading "
+                         "of the HBM byte counters) and exit")
     return ap.parse_args()
+
+
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_plain4096x2160.txt")
+
+
+def measured_traffic(kernel, width, height):
+    """HBM bytes per launch of `kernel` from the committed PMC summary
+    (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by what the
+    counter reads for a copy of known size), when it covers this frame
+    size; else None."""
+    if (width, height) != (4096, 2160) or not os.path.exists(TRAFFIC_FILE):
+        return None
+    for line in open(TRAFFIC_FILE):
+        f = line.split()
+        if not f or f[0] != kernel:
+            continue
+        kv = dict(x.split("=") for x in f[1:])
+        return int(1024 * (float(kv["fetch_kib"]) * float(kv["fetch_scale"]) +
+                           float(kv["write_kib"]) * float(kv["write_scale"])))
+    return None
+
+
+def calibrate_copy(mib):
+    """Ten device-to-device copies of `mib` MiB: each moves mib MiB in and
+    mib MiB out, the known bytes the FETCH_SIZE / WRITE_SIZE counters are
+    scaled against."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.arange(n, dtype=torch.int32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(10):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibrate_copy_mib": mib, "copies": 10, "ok": bool((dst == src).all())}))
 
 
 def main():
     args = parse_args()
+    if args.calibrate_copy:
+        calibrate_copy(args.calibrate_copy)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are out of scope "
                                    "(frames shard across GPUs with no exchange)"}))
@@ -151,7 +191,9 @@ def main():
     dev_bytes = 14.0 * npix
     achieved = dev_bytes / (t_develop_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": 8000.0, "unit": "GB/s",
-                "frac": round(achieved / 8000.0, 4), "traffic": None,
+                "frac": round(achieved / 8000.0, 4),
+                "traffic": measured_traffic("k_develop", w, h),
+                "traffic_source": os.path.relpath(TRAFFIC_FILE, ROOT),
                 "kernel": "k_develop", "kernel_ms": round(t_develop_ms, 4),
                 "bytes_per_launch": int(dev_bytes),
                 "decode_ms": round(t_decode_ms, 4),
