@@ -263,8 +263,8 @@ struct DevBuf {
 struct prr_context {
     int device = 0;
     hipStream_t stream = nullptr;
-    DecodeLaunch launch{16, 65536};
-    DevBuf packet, offsets, status, flags, samples, out0, out1, out2;
+    DecodeLaunch launch{16, 65536, 256};
+    DevBuf packet, offsets, queue, status, flags, samples, out0, out1, out2;
     std::vector<uint8_t> host_flags;
 };
 
@@ -394,13 +394,14 @@ extern "C" int prr_decode_frame_device(prr_context *ctx, const prr_header *hdr,
     hipStream_t s = (hipStream_t)stream;
     HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
     rc = ensure(ctx, ctx->offsets, sizeof(uint64_t) * ((size_t)info->nb_tiles + 1), err);
+    if (!rc)
+        rc = ensure(ctx, ctx->queue, sizeof(uint32_t) * std::max<size_t>(info->nb_tiles, 1), err);
     if (rc)
         return rc;
     FrameDesc f = frame_desc(hdr, info, d_packet, len, (uint64_t *)ctx->offsets.ptr);
-    HIP_TRY(launch_status_init(d_status, s), "status kernel");
-    HIP_TRY(launch_scan(f, d_status, s), "scan kernel");
-    HIP_TRY(launch_decode(f, idct, d_samples, d_linear, d_tile_flags, d_status, ctx->launch, s),
-            "decode kernel");
+    HIP_TRY(launch_decode(f, idct, d_samples, d_linear, d_tile_flags, d_status,
+                          (uint32_t *)ctx->queue.ptr, ctx->launch, s),
+            "decode kernels");
     return PRR_OK;
 }
 
@@ -421,14 +422,14 @@ extern "C" int prr_decode_frame_reduced_device(prr_context *ctx, const prr_heade
     hipStream_t s = (hipStream_t)stream;
     HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
     rc = ensure(ctx, ctx->offsets, sizeof(uint64_t) * ((size_t)info->nb_tiles + 1), err);
+    if (!rc)
+        rc = ensure(ctx, ctx->queue, sizeof(uint32_t) * std::max<size_t>(info->nb_tiles, 1), err);
     if (rc)
         return rc;
     FrameDesc f = frame_desc(hdr, info, d_packet, len, (uint64_t *)ctx->offsets.ptr);
-    HIP_TRY(launch_status_init(d_status, s), "status kernel");
-    HIP_TRY(launch_scan(f, d_status, s), "scan kernel");
-    HIP_TRY(launch_decode_reduced(f, scale, greens, d_rgb, d_tile_flags, d_status, ctx->launch,
-                                  s),
-            "reduced kernel");
+    HIP_TRY(launch_decode_reduced(f, scale, greens, d_rgb, d_tile_flags, d_status,
+                                  (uint32_t *)ctx->queue.ptr, ctx->launch, s),
+            "reduced kernels");
     return PRR_OK;
 }
 

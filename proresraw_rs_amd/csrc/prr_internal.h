@@ -86,20 +86,21 @@ struct BatchDevelop {
 struct DecodeLaunch {
     uint32_t streams_per_wg; /* 16, 32 or 64 lanes decoding streams */
     uint32_t lds_limit;      /* bytes of LDS a workgroup may use */
+    uint32_t solo_tiles;     /* longest tiles decoded one per workgroup */
 };
 
-hipError_t launch_scan(const FrameDesc &f, prr_device_status *status, hipStream_t s);
+/* Status reset + size-table scan + tile order (k_prepare), then the
+ * decode; `queue` is device scratch of nb_tiles uint32. */
 hipError_t launch_decode(const FrameDesc &f, int ffmpeg, uint16_t *samples, uint16_t *linear,
-                         uint8_t *flags, prr_device_status *status, const DecodeLaunch &cfg,
-                         hipStream_t s);
+                         uint8_t *flags, prr_device_status *status, uint32_t *queue,
+                         const DecodeLaunch &cfg, hipStream_t s);
 hipError_t launch_decode_reduced(const FrameDesc &f, int shift, int average, uint16_t *rgb,
-                                 uint8_t *flags, prr_device_status *status,
+                                 uint8_t *flags, prr_device_status *status, uint32_t *queue,
                                  const DecodeLaunch &cfg, hipStream_t s);
 hipError_t launch_linearize(const uint16_t *samples, uint32_t stride, uint32_t width,
                             uint32_t height, const uint16_t *curve, uint16_t *out,
                             hipStream_t s);
 hipError_t launch_develop(const DevelopDesc &d, float *r, float *g, float *b, hipStream_t s);
-hipError_t launch_status_init(prr_device_status *status, hipStream_t s);
 hipError_t launch_batch(const BatchFrame *d_frames, uint32_t n_frames, uint32_t n_items,
                         uint32_t *d_queue, uint32_t *d_head, const BatchDevelop *d_jobs,
                         uint32_t n_jobs, uint32_t max_w, uint32_t max_h, uint32_t grid,

@@ -1,8 +1,8 @@
 /* HIP kernels for gfx950 (MI355X): ProRes RAW decode and develop.
  *
- *  k_status_init  per-frame status word (no hipMemsetAsync: graph safe)
- *  k_scan         size table -> tile offsets, max_qscale, first OOB tile
- *                 (parse_frame, decoder.rs:104-116)
+ *  k_prepare      status reset (no hipMemsetAsync: graph safe), size table
+ *                 -> tile offsets, max_qscale, first OOB tile (parse_frame,
+ *                 decoder.rs:104-116), tile groups ordered longest first
  *  k_decode<F>    one wave per group of tiles: one lane per component
  *                 stream runs the entropy decoder (bits.rs,
  *                 coefficients.rs) over the packet into LDS coefficient
@@ -531,8 +531,20 @@ struct Group {
     TileInfo *tiles;     /* up to 16 */
     int16_t *coefs;
     uint16_t *stage;     /* 4096 entries */
-    uint32_t t0, t_end, T;
+    uint32_t n, T;       /* tiles in the group, capacity */
 };
+
+/* The tiles of one group: list[0 .. count) when list is set (tiles in the
+ * order a queue chose), else t0 .. t0 + count - 1. */
+struct TileSet {
+    const uint32_t *list;
+    uint32_t t0, count;
+};
+
+__device__ inline uint32_t tile_at(const TileSet &ts, uint32_t i)
+{
+    return ts.list ? ts.list[i] : ts.t0 + i;
+}
 
 /* Carve the dynamic LDS. Layout: [table 544 B][curve 36 B][qmat 64 B at
  * 592][tiles 16 x 20 B at 656][coefs][stage 8 KB]. */
@@ -560,14 +572,13 @@ __host__ inline size_t group_lds_bytes(uint32_t spw, uint32_t stride)
  * 4T decodes stream l & 3 of the group's tile l >> 2 whose streams are
  * listed in `want` (bit s set = decode stream s). Returns the lane's
  * overrun flag. */
-__device__ inline bool front_end(const FrameDesc &f, Group &g, uint32_t group, uint32_t spw,
+__device__ inline bool front_end(const FrameDesc &f, Group &g, const TileSet &ts, uint32_t spw,
                                  uint32_t stride, uint32_t log2_coefs,
                                  uint32_t want, uint8_t *flags, prr_device_status *status)
 {
     const uint32_t lane = threadIdx.x;
     g.T = spw >> 2;
-    g.t0 = group * g.T;
-    g.t_end = min(g.t0 + g.T, f.nb_tiles);
+    g.n = ts.count;
 
     for (uint32_t i = lane; i < K_N; i += 64) {
         uint32_t v;
@@ -603,8 +614,8 @@ __device__ inline bool front_end(const FrameDesc &f, Group &g, uint32_t group, u
 
     /* Tile::parse (tile.rs:126-154), by the four lanes of each tile. */
     const uint32_t tl = lane >> 2, s = lane & 3;
-    const uint32_t t = g.t0 + tl;
-    bool have = lane < 4 * g.T && t < g.t_end;
+    bool have = lane < 4 * g.T && tl < g.n;
+    const uint32_t t = have ? tile_at(ts, tl) : 0u;
     uint64_t toff = 0, tend = 0;
     bool valid = false;
     uint32_t hl = 0, sz[3] = {0, 0, 0}, qs = 0;
@@ -663,18 +674,43 @@ __device__ inline bool front_end(const FrameDesc &f, Group &g, uint32_t group, u
 
 /* ------------------------------------------------------------------ */
 
-__global__ void k_status_init(prr_device_status *status)
+/* Exclusive prefix sum of one value per lane over a 1024-lane workgroup:
+ * within each wave with shuffles, then across the 16 waves through `sums`
+ * (32 entries of LDS). Returns the lane's exclusive prefix; `total` gets
+ * the sum of all. */
+__device__ inline uint64_t block_exclusive(uint64_t v, uint64_t *sums, uint64_t &total)
 {
-    if (threadIdx.x == 0) {
-        status->oob_tile = 0xFFFFFFFFu;
-        status->max_qscale = 0;
-        status->n_skipped = 0;
-        status->overrun = 0;
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint64_t incl = v;
+#pragma unroll
+    for (uint32_t d = 1; d < 64; d <<= 1) {
+        const uint64_t u = __shfl_up(incl, d);
+        if (lane >= d)
+            incl += u;
     }
+    if (lane == 63)
+        sums[wave] = incl;
+    __syncthreads();
+    if (wave == 0) {
+        uint64_t w = lane < 16 ? sums[lane] : 0;
+#pragma unroll
+        for (uint32_t d = 1; d < 16; d <<= 1) {
+            const uint64_t u = __shfl_up(w, d);
+            if (lane >= d)
+                w += u;
+        }
+        if (lane < 16)
+            sums[16 + lane] = w;
+    }
+    __syncthreads();
+    total = sums[31];
+    return (incl - v) + (wave ? sums[16 + wave - 1] : 0);
 }
 
-/* Size-table walk of parse_frame (decoder.rs:104-116) as a block scan by
- * one 1024-lane workgroup. */
+/* Size-table walk of parse_frame (decoder.rs:104-116) by one 1024-lane
+ * workgroup, each lane over a run of `per` consecutive tiles: the tile
+ * offsets (a prefix sum of the size table), the first tile that runs past
+ * the packet, and the largest qscale byte. */
 __device__ inline void scan_frame(const FrameDesc &f, prr_device_status *status, uint64_t *sums)
 {
     const uint32_t n = f.nb_tiles, t = threadIdx.x;
@@ -684,19 +720,12 @@ __device__ inline void scan_frame(const FrameDesc &f, prr_device_status *status,
     uint64_t sum = 0;
     for (uint32_t i = lo; i < hi; i++)
         sum += be16_at(tab + 2 * (size_t)i);
-    sums[t] = sum;
-    __syncthreads();
-    for (uint32_t d = 1; d < 1024; d <<= 1) {
-        uint64_t v = t >= d ? sums[t - d] : 0;
-        __syncthreads();
-        sums[t] += v;
-        __syncthreads();
-    }
-    uint64_t off = (uint64_t)f.table_off + 2ull * n + (t ? sums[t - 1] : 0);
-    uint32_t mq = 0, oob = 0xFFFFFFFFu;
+    uint64_t total;
+    uint64_t off = (uint64_t)f.table_off + 2ull * n + block_exclusive(sum, sums, total);
+    uint32_t oob = 0xFFFFFFFFu, mq = 0;
     for (uint32_t i = lo; i < hi; i++) {
-        uint32_t size = be16_at(tab + 2 * (size_t)i);
-        uint32_t q = off + 1 < f.len ? f.packet[off + 1] : 0u;
+        const uint32_t size = be16_at(tab + 2 * (size_t)i);
+        const uint32_t q = off + 1 < f.len ? f.packet[off + 1] : 0u;
         mq = q > mq ? q : mq;
         if (off + size > f.len && oob == 0xFFFFFFFFu)
             oob = i;
@@ -704,23 +733,151 @@ __device__ inline void scan_frame(const FrameDesc &f, prr_device_status *status,
         off += size;
     }
     if (t == 1023)
-        f.offsets[n] = (uint64_t)f.table_off + 2ull * n + sums[1023];
+        f.offsets[n] = (uint64_t)f.table_off + 2ull * n + total;
     if (oob != 0xFFFFFFFFu)
         atomicMin(&status->oob_tile, oob);
     if (mq)
         atomicMax(&status->max_qscale, mq);
 }
 
-__global__ __launch_bounds__(1024) void k_scan(FrameDesc f, prr_device_status *status)
+/* scan_frame for frames of at most 1024 * PREP_PER tiles, with a lane's
+ * sizes kept in registers so the size table is read once and every
+ * qscale byte is requested before any is waited for; also adds each
+ * tile's size to gbytes[tile / T], its group's coded bytes, when gbytes is
+ * not NULL. */
+enum { PREP_PER = 8, PREP_SMALL = 1024 * PREP_PER };
+__device__ inline void scan_frame_small(const FrameDesc &f, prr_device_status *status,
+                                        uint64_t *sums, uint32_t *gbytes, uint32_t T)
 {
-    __shared__ uint64_t sums[1024];
-    scan_frame(f, status, sums);
+    const uint32_t n = f.nb_tiles, t = threadIdx.x;
+    const uint32_t per = (n + 1023) / 1024;
+    const uint32_t lo = min(t * per, n), hi = min(lo + per, n);
+    const uint8_t *tab = f.packet + f.table_off;
+    uint32_t sz[PREP_PER];
+    uint64_t sum = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < PREP_PER; k++) {
+        sz[k] = lo + k < hi ? be16_at(tab + 2 * (size_t)(lo + k)) : 0u;
+        sum += sz[k];
+    }
+    uint64_t total;
+    const uint64_t first = (uint64_t)f.table_off + 2ull * n + block_exclusive(sum, sums, total);
+    /* Every qscale load is issued (at a clamped, in-packet address) before
+     * any is used. */
+    uint64_t offs[PREP_PER];
+    uint32_t q[PREP_PER];
+    {
+        uint64_t off = first;
+#pragma unroll
+        for (uint32_t k = 0; k < PREP_PER; k++) {
+            offs[k] = off;
+            off += sz[k];
+            q[k] = f.packet[min(offs[k] + 1, f.len - 1)];
+        }
+    }
+    uint32_t oob = 0xFFFFFFFFu, mq = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < PREP_PER; k++) {
+        const uint32_t i = lo + k;
+        if (i < hi) {
+            const uint64_t off = offs[k];
+            const uint32_t qk = off + 1 < f.len ? q[k] : 0u;
+            mq = qk > mq ? qk : mq;
+            if (off + sz[k] > f.len && oob == 0xFFFFFFFFu)
+                oob = i;
+            f.offsets[i] = off;
+            if (gbytes)
+                atomicAdd(&gbytes[i / T], sz[k]);
+        }
+    }
+    if (t == 1023)
+        f.offsets[n] = (uint64_t)f.table_off + 2ull * n + total;
+    /* One atomic per wave. */
+#pragma unroll
+    for (uint32_t d = 32; d > 0; d >>= 1) {
+        oob = min(oob, (uint32_t)__shfl_xor((int)oob, (int)d));
+        mq = max(mq, (uint32_t)__shfl_xor((int)mq, (int)d));
+    }
+    if ((t & 63) == 0) {
+        if (oob != 0xFFFFFFFFu)
+            atomicMin(&status->oob_tile, oob);
+        if (mq)
+            atomicMax(&status->max_qscale, mq);
+    }
+}
+
+/* Tile groups ordered by coded length, longest first, into queue[0 ..
+ * groups): a bucket sort on a half-octave log scale of each group's bytes
+ * (`bytes_of(g)`), by one 1024-lane workgroup. Waves that start first get
+ * the longest chains, and the waves that share a SIMD at the end of the
+ * launch are the short ones. */
+template <class Bytes>
+__device__ inline void order_groups(uint32_t groups, uint32_t *queue, uint32_t *hist,
+                                    uint32_t *base, Bytes bytes_of)
+{
+    constexpr uint32_t NB = 64;
+    const uint32_t tid = threadIdx.x;
+    auto bucket_of = [&](uint32_t g) {
+        const uint64_t bytes = bytes_of(g);
+        const float l2 = bytes ? __log2f((float)bytes) : 0.0f;
+        int b = (int)(2.0f * l2);
+        b = b < 0 ? 0 : b > (int)NB - 1 ? (int)NB - 1 : b;
+        return (uint32_t)(NB - 1 - b);
+    };
+    for (uint32_t g = tid; g < groups; g += 1024)
+        atomicAdd(&hist[bucket_of(g)], 1u);
+    __syncthreads();
+    if (tid == 0) {
+        uint32_t run = 0;
+        for (uint32_t b = 0; b < NB; b++) {
+            base[b] = run;
+            run += hist[b];
+        }
+    }
+    __syncthreads();
+    for (uint32_t g = tid; g < groups; g += 1024)
+        queue[atomicAdd(&base[bucket_of(g)], 1u)] = g;
+}
+
+/* Status reset, size-table scan and the order of the frame's groups of T
+ * tiles (T = 1: the tiles themselves, longest first). */
+__global__ __launch_bounds__(1024) void k_prepare(FrameDesc f, prr_device_status *status,
+                                                  uint32_t *queue, uint32_t T)
+{
+    __shared__ uint64_t sums[32];
+    __shared__ uint32_t hist[64], base[64];
+    __shared__ uint32_t gbytes[PREP_SMALL];
+    const uint32_t tid = threadIdx.x, n = f.nb_tiles, groups = (n + T - 1) / T;
+    if (tid == 0) {
+        status->oob_tile = 0xFFFFFFFFu;
+        status->max_qscale = 0;
+        status->n_skipped = 0;
+        status->overrun = 0;
+    }
+    if (tid < 64)
+        hist[tid] = 0;
+    if (n <= PREP_SMALL) {
+        for (uint32_t g = tid; g < groups; g += 1024)
+            gbytes[g] = 0;
+        __syncthreads();
+        scan_frame_small(f, status, sums, gbytes, T);
+        __syncthreads();
+        order_groups(groups, queue, hist, base, [&](uint32_t g) { return (uint64_t)gbytes[g]; });
+    } else {
+        __syncthreads();
+        scan_frame(f, status, sums);
+        __syncthreads();
+        order_groups(groups, queue, hist, base, [&](uint32_t g) {
+            const uint32_t t0 = g * T, t1 = min(t0 + T, n);
+            return f.offsets[t1] - f.offsets[t0];
+        });
+    }
 }
 
 /* Full decode (decoder.rs:226-307) of tile group `group` (spw/4
  * consecutive tiles) by one 64-lane wave. */
 template <int FFMPEG>
-__device__ inline void decode_group_full(const FrameDesc &f, uint32_t group, uint16_t *samples,
+__device__ inline void decode_group_full(const FrameDesc &f, const TileSet &ts, uint16_t *samples,
                                          uint16_t *linear, uint8_t *flags,
                                          prr_device_status *status, uint32_t spw,
                                          uint32_t *smem)
@@ -728,7 +885,7 @@ __device__ inline void decode_group_full(const FrameDesc &f, uint32_t group, uin
     const uint32_t amax = f.alignment;
     const uint32_t stride = stream_stride(64, amax);
     Group g = carve(smem, spw, stride);
-    front_end(f, g, group, spw, stride, 6, 0xFu, flags, status);
+    front_end(f, g, ts, spw, stride, 6, 0xFu, flags, status);
 
     const uint32_t lane = threadIdx.x;
     const uint32_t blocks_max = 1u << amax;
@@ -742,7 +899,7 @@ __device__ inline void decode_group_full(const FrameDesc &f, uint32_t group, uin
         const uint32_t ls = slot >> amax, b = slot & (blocks_max - 1);
         const uint32_t tl = ls >> 2, s = ls & 3;
         const uint32_t tc = tl - (c0 >> amax) / 4;
-        if (g.t0 + tl < g.t_end) {
+        if (tl < g.n) {
             TileInfo ti = g.tiles[tl];
             if (ti.valid == 1 && b < (1u << ti.log2)) {
                 const int16_t *src = g.coefs + ls * stride + b;
@@ -777,7 +934,7 @@ __device__ inline void decode_group_full(const FrameDesc &f, uint32_t group, uin
             const uint32_t tcc = i >> sh_tile;
             const uint32_t row = (i >> sh_row) & 15u, dcol = i & ((1u << sh_row) - 1);
             const uint32_t tli = first_tile + tcc;
-            if (tli >= g.T || g.t0 + tli >= g.t_end)
+            if (tli >= g.n)
                 continue;
             TileInfo ti = g.tiles[tli];
             if (ti.valid == 2 || dcol >= (8u << ti.log2))
@@ -800,7 +957,7 @@ __device__ inline void decode_group_full(const FrameDesc &f, uint32_t group, uin
 /* Reduced decode (reduced.rs:147-273) of one tile group: streams 3, 0,
  * (1,) 2 as R, G, (G,) B; shift 1/2/3 for Half/Quarter/Eighth. */
 template <uint32_t shift>
-__device__ inline void decode_group_reduced(const FrameDesc &f, uint32_t group, uint16_t *rgb,
+__device__ inline void decode_group_reduced(const FrameDesc &f, const TileSet &ts, uint16_t *rgb,
                                             uint8_t *flags, prr_device_status *status,
                                             uint32_t spw, uint32_t average,
                                             uint32_t *smem)
@@ -811,7 +968,7 @@ __device__ inline void decode_group_reduced(const FrameDesc &f, uint32_t group, 
     const uint32_t stride = stream_stride(per_block, amax);
     Group g = carve(smem, spw, stride);
     const uint32_t want = average ? 0xFu : 0xDu;
-    front_end(f, g, group, spw, stride, log2_coefs, want, flags, status);
+    front_end(f, g, ts, spw, stride, log2_coefs, want, flags, status);
 
     const uint32_t lane = threadIdx.x;
     const uint32_t blocks_max = 1u << amax;
@@ -825,7 +982,7 @@ __device__ inline void decode_group_reduced(const FrameDesc &f, uint32_t group, 
         const uint32_t ls = slot >> amax, b = slot & (blocks_max - 1);
         const uint32_t tl = ls >> 2, s = ls & 3;
         const uint32_t tc = tl - (c0 >> amax) / 4;
-        if (g.t0 + tl < g.t_end && ((want >> s) & 1u)) {
+        if (tl < g.n && ((want >> s) & 1u)) {
             TileInfo ti = g.tiles[tl];
             if (ti.valid == 1 && b < (1u << ti.log2)) {
                 const int16_t *src = g.coefs + ls * stride + b;
@@ -867,7 +1024,7 @@ __device__ inline void decode_group_reduced(const FrameDesc &f, uint32_t group, 
         for (uint32_t dcol = lane; dcol < dw_per_row; dcol += 64) {
             const uint32_t tcc = tr / side, row = tr % side; /* side is a constant */
             const uint32_t tli = first_tile + tcc;
-            if (tli >= g.T || g.t0 + tli >= g.t_end)
+            if (tli >= g.n)
                 continue;
             TileInfo ti = g.tiles[tli];
             const uint32_t tw = side << ti.log2;
@@ -901,22 +1058,39 @@ __device__ inline void decode_group_reduced(const FrameDesc &f, uint32_t group, 
     }
 }
 
+/* The tiles of workgroup `item` of a single-frame launch over `order`
+ * (the frame's tiles, longest first): the first `solo` workgroups take one
+ * tile each, so the longest chains end with the shortest tail (one tile's
+ * transforms and stores), and the rest take spw / 4 tiles each. */
+__device__ inline TileSet item_tiles(const uint32_t *order, uint32_t n, uint32_t solo,
+                                     uint32_t spw, uint32_t item)
+{
+    if (item < solo)
+        return TileSet{order + item, 0, 1};
+    const uint32_t T = spw >> 2, start = solo + (item - solo) * T;
+    return TileSet{order + start, 0, min(T, n - start)};
+}
+
 template <int FFMPEG>
 __global__ __launch_bounds__(64) void k_decode(FrameDesc f, uint16_t *samples, uint16_t *linear,
                                                 uint8_t *flags, prr_device_status *status,
-                                                uint32_t spw)
+                                                uint32_t spw, const uint32_t *order,
+                                                uint32_t solo)
 {
     extern __shared__ uint32_t smem[];
-    decode_group_full<FFMPEG>(f, blockIdx.x, samples, linear, flags, status, spw, smem);
+    const TileSet ts = item_tiles(order, f.nb_tiles, solo, spw, blockIdx.x);
+    decode_group_full<FFMPEG>(f, ts, samples, linear, flags, status, spw, smem);
 }
 
 template <uint32_t shift>
 __global__ __launch_bounds__(64) void k_reduced(FrameDesc f, uint16_t *rgb, uint8_t *flags,
                                                  prr_device_status *status, uint32_t spw,
-                                                 uint32_t average)
+                                                 uint32_t average, const uint32_t *order,
+                                                 uint32_t solo)
 {
     extern __shared__ uint32_t smem[];
-    decode_group_reduced<shift>(f, blockIdx.x, rgb, flags, status, spw, average, smem);
+    const TileSet ts = item_tiles(order, f.nb_tiles, solo, spw, blockIdx.x);
+    decode_group_reduced<shift>(f, ts, rgb, flags, status, spw, average, smem);
 }
 
 /* Frame::linearize_into (decoder.rs:197-216), two values per lane. */
@@ -1051,10 +1225,6 @@ __device__ inline float estimate_pre(const float *xx, const float *gg, const flo
 
 enum { DTW = 64, DTH = 32, NORM_W = DTW + 6, NORM_H = DTH + 6, GRN_W = DTW + 4, GRN_H = DTH + 2 };
 
-/* One 64x32 tile of output per 256-lane workgroup: normalized code values
- * with a 3-sensel halo and green-pass values with a 1-sensel halo live in
- * LDS; each lane handles a red/blue + green pair of sensels so every lane
- * does the same work. */
 /* LDS of one develop tile. Each green-pass cell holds its red/blue value,
  * its green, and the two noise deviations every chroma estimate that reads
  * the cell needs: sqrt(variance(rb)) and sqrt(variance(g)) at the chroma
@@ -1077,227 +1247,274 @@ __device__ inline void noise_from_quantizer(uint32_t quantizer, uint32_t range, 
     noise[2] = (float)(64.0 / 3.0 / range2);
 }
 
-/* Output sensel (x, y) goes to (y - oy) * ostride + (x - ox) of the planes
- * (ox = oy = 0, ostride = width for a whole frame). */
-__device__ inline void develop_tile(const DevelopDesc &dd, float *out_r, float *out_g,
-                                    float *out_b, uint32_t bx, uint32_t by, DevelopLds &lds,
-                                    uint32_t ox, uint32_t oy, uint32_t ostride)
+/* Per-frame constants of the develop passes. */
+struct DevK {
+    float n0, n1, n2; /* noise model */
+    float bs;         /* black in normalized units: the green pass offset */
+    float coff;       /* black_over_range: the chroma pass offset */
+    uint32_t red_row, red_col;
+    uint32_t w, h;
+};
+
+__device__ __forceinline__ DevK dev_consts(const DevelopDesc &d)
 {
-    auto &norm = lds.norm;
-    auto &green = lds.green;
-    auto &curve = lds.curve;
-    const DevelopDesc &d = dd;
     float noise[3] = {d.noise[0], d.noise[1], d.noise[2]};
     if (d.qstatus)
         noise_from_quantizer(d.qmat0 * d.qstatus->max_qscale, d.range, noise);
+    DevK k;
+    k.n0 = noise[0];
+    k.n1 = noise[1];
+    k.n2 = noise[2];
+    k.bs = (float)d.black * d.one_over_range;
+    k.coff = d.black_over_range;
+    k.red_row = (uint32_t)(d.cfa_phase >> 1);
+    k.red_col = (uint32_t)(d.cfa_phase & 1);
+    k.w = d.width;
+    k.h = d.height;
+    return k;
+}
+
+/* Normalized value of a code value (Conversion::new folded table,
+ * demosaic.rs:293-298). */
+__device__ __forceinline__ float norm_of(const DevelopDesc &d, const uint32_t *curve, uint32_t code)
+{
+    const int32_t lin = (int32_t)linearize(curve, code);
+    return d.one_over_range * (float)(lin - d.black);
+}
+
+/* Green pass (green_row_impl, demosaic.rs:481-541) of one red/blue sensel
+ * (gx, gy): `norm(y, x)` reads normalized values at picture coordinates
+ * (rows and columns outside the picture hold clamped copies). Returns the
+ * cell (rb, green, sqrt(variance(rb)), sqrt(variance(green))) at the
+ * chroma offset. */
+template <class Norm>
+__device__ __forceinline__ float4 green_rb(const DevK &k, int32_t gx, int32_t gy, float clip,
+                                           Norm norm)
+{
+    const float n0 = k.n0, n1 = k.n1, n2 = k.n2, bs = k.bs;
+    const float centre = norm(gy, gx);
+    const float c = fminf(centre, clip);
+    float hh[4] = {norm(gy, gx - 1), norm(gy, gx + 1), norm(gy, gx - 2), norm(gy, gx + 2)};
+    float vv[4] = {norm(gy - 1, gx), norm(gy + 1, gx), norm(gy - 2, gx), norm(gy + 2, gx)};
+    hh[0] = fminf(hh[0], 1.0f);
+    hh[1] = fminf(hh[1], 1.0f);
+    hh[2] = fminf(hh[2], clip);
+    hh[3] = fminf(hh[3], clip);
+    vv[0] = fminf(vv[0], 1.0f);
+    vv[1] = fminf(vv[1], 1.0f);
+    vv[2] = fminf(vv[2], clip);
+    vv[3] = fminf(vv[3], clip);
+    mirror4(hh, (uint32_t)gx, k.w, c);
+    mirror4(vv, (uint32_t)gy, k.h, c);
+    float var_h[4], var_v[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        var_h[q] = variance(hh[q], bs, n0, n1, n2);
+        var_v[q] = variance(vv[q], bs, n0, n1, n2);
+    }
+    const float sums0 = hh[0] + hh[1], sums1 = vv[0] + vv[1];
+    const float sums2 = hh[2] + hh[3], sums3 = vv[2] + vv[3];
+    float ratio0 = 0.0f, ratio1 = 0.0f;
+    if (c > sqrtf(variance(c, bs, n0, n1, n2))) {
+        const float t0 = c * 2.0f + sums2, t1 = c * 2.0f + sums3;
+        const float t2 = c * -2.0f + sums2, t3 = c * -2.0f + sums3;
+#pragma unroll
+        for (int dir = 0; dir < 2; dir++) {
+            const float d0 = dir ? vv[0] : hh[0], d1 = dir ? vv[1] : hh[1];
+            const float d2 = dir ? vv[2] : hh[2], d3 = dir ? vv[3] : hh[3];
+            const float sd0 = sqrtf(dir ? var_v[0] : var_h[0]);
+            const float sd1 = sqrtf(dir ? var_v[1] : var_h[1]);
+            const float sd2 = sqrtf(dir ? var_v[2] : var_h[2]);
+            const float sd3 = sqrtf(dir ? var_v[3] : var_h[3]);
+            float a = d0 - d1, b = d2 - d3;
+            bool any = a * b > 0.0f || (d2 - c) * (d3 - c) > 0.0f || sd0 + sd1 > fabsf(a) ||
+                       sd2 + sd3 > fabsf(b);
+            bool all = d0 > sd0 && d1 > sd1 && d2 > sd2 && d3 > sd3;
+            if (any && all) {
+                if (dir == 0)
+                    ratio0 = t2 / (t0 * 2.0f);
+                else
+                    ratio1 = t3 / (t1 * 2.0f);
+            }
+        }
+    }
+    const float gh = sums0 * 0.5f * (1.0f - ratio0);
+    const float gv = sums1 * 0.5f * (1.0f - ratio1);
+    const float dh0 = hh[0] - gh, dh1 = hh[1] - gh, dh2 = hh[2] - c, dh3 = hh[3] - c;
+    const float dv0 = vv[0] - gv, dv1 = vv[1] - gv, dv2 = vv[2] - c, dv3 = vv[3] - c;
+    const float eh = dot4(dh0, dh1, dh2, dh3, dh0, dh1, dh2, dh3) + (var_h[0] + var_h[1]);
+    const float ev = dot4(dv0, dv1, dv2, dv3, dv0, dv1, dv2, dv3) + (var_v[0] + var_v[1]);
+    const float gout = gv + (gh - gv) * ev / (eh + ev);
+    return make_float4(c, gout, sqrtf(variance(c, k.coff, n0, n1, n2)),
+                       sqrtf(variance(gout, k.coff, n0, n1, n2)));
+}
+
+/* The green-pass cell of a green sensel: its own value as green. */
+__device__ __forceinline__ float4 green_g(const DevK &k, float centre)
+{
+    return make_float4(0.0f, centre, 0.0f, sqrtf(variance(centre, k.coff, k.n0, k.n1, k.n2)));
+}
+
+/* Chroma pass (chroma_row_with, demosaic.rs:648-688), white balance and
+ * the output matrix (matrix_planes_impl, demosaic.rs:711-720) of the pair
+ * of pixels (xb, y), (xb + 1, y), xb even: `green(y, x)` reads green-pass
+ * cells at picture coordinates. res[i] is pixel xb + i. */
+template <class Green>
+__device__ __forceinline__ void chroma_pair(const DevelopDesc &d, const DevK &k, uint32_t y,
+                                            uint32_t xb, Green green, float (&res)[2][3])
+{
+    const uint32_t w = k.w, h = k.h;
+    const float n0 = k.n0, n1 = k.n1, n2 = k.n2, off = k.coff;
+    const uint32_t ym = y > 0 ? y - 1 : y + 1, yp = y < h - 1 ? y + 1 : y - 1;
+    const bool on_red_row = (y & 1u) == k.red_row;
+    const uint32_t rb_col = on_red_row ? k.red_col : 1 - k.red_col;
+    auto finish = [&](float rr, float gval, float bb, float *out) {
+        float r = fminf(rr * d.wb[0], 1.0f);
+        float g = fminf(gval, 1.0f);
+        float b = fminf(bb * d.wb[1], 1.0f);
+        if (d.has_matrix) {
+            const float r2 = fmaf(b, d.m[2], fmaf(r, d.m[0], g * d.m[1]));
+            const float g2 = fmaf(b, d.m[5], fmaf(r, d.m[3], g * d.m[4]));
+            const float b2 = fmaf(b, d.m[8], fmaf(r, d.m[6], g * d.m[7]));
+            r = r2;
+            g = g2;
+            b = b2;
+        }
+        out[0] = r;
+        out[1] = g;
+        out[2] = b;
+    };
+    float res_rb[3], res_g[3];
+    {
+        /* The red or blue pixel: diagonal neighbours. */
+        const uint32_t x = xb + rb_col;
+        const uint32_t xm = x > 0 ? x - 1 : x + 1, xp = x < w - 1 ? x + 1 : x - 1;
+        const float4 mid = green(y, x);
+        const float4 a = green(ym, xm), b = green(yp, xm), c2 = green(ym, xp), e = green(yp, xp);
+        const float xx[4] = {a.x, b.x, c2.x, e.x};
+        const float gg[4] = {a.y, b.y, c2.y, e.y};
+        const float sx[4] = {a.z, b.z, c2.z, e.z};
+        const float sg[4] = {a.w, b.w, c2.w, e.w};
+        const float est = estimate_pre(xx, gg, sx, sg, mid.y, mid.w, off, n0, n1, n2);
+        finish(on_red_row ? mid.x : est, mid.y, on_red_row ? est : mid.x, res_rb);
+    }
+    {
+        /* The green pixel: horizontal and vertical neighbour pairs. */
+        const uint32_t x = xb + (1 - rb_col);
+        const uint32_t xm = x > 0 ? x - 1 : x + 1, xp = x < w - 1 ? x + 1 : x - 1;
+        const float4 mid = green(y, x);
+        const float4 l = green(y, xm), r = green(y, xp);
+        const float4 u = green(ym, x), dn = green(yp, x);
+        const float hx[4] = {l.x, r.x, l.x, r.x}, hg[4] = {l.y, r.y, l.y, r.y};
+        const float hsx[4] = {l.z, r.z, l.z, r.z}, hsg[4] = {l.w, r.w, l.w, r.w};
+        const float vx[4] = {u.x, dn.x, u.x, dn.x}, vg[4] = {u.y, dn.y, u.y, dn.y};
+        const float vsx[4] = {u.z, dn.z, u.z, dn.z}, vsg[4] = {u.w, dn.w, u.w, dn.w};
+        const float eh = estimate_pre(hx, hg, hsx, hsg, mid.y, mid.w, off, n0, n1, n2);
+        const float ev = estimate_pre(vx, vg, vsx, vsg, mid.y, mid.w, off, n0, n1, n2);
+        finish(on_red_row ? eh : ev, mid.y, on_red_row ? ev : eh, res_g);
+    }
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        res[0][c] = rb_col == 0 ? res_rb[c] : res_g[c];
+        res[1][c] = rb_col == 0 ? res_g[c] : res_rb[c];
+    }
+}
+
+/* Store the pair (xb, xb + 1) at `at` (the index of xb): planar with
+ * float2 stores, or interleaved RGB when out_g is NULL. */
+__device__ __forceinline__ void store_pair(float *out_r, float *out_g, float *out_b, size_t at,
+                                           uint32_t xb, uint32_t w, const float (&res)[2][3])
+{
+    if (out_g) {
+        if (xb + 1 < w) {
+            *(float2 *)(out_r + at) = make_float2(res[0][0], res[1][0]);
+            *(float2 *)(out_g + at) = make_float2(res[0][1], res[1][1]);
+            *(float2 *)(out_b + at) = make_float2(res[0][2], res[1][2]);
+        } else if (xb < w) {
+            out_r[at] = res[0][0];
+            out_g[at] = res[0][1];
+            out_b[at] = res[0][2];
+        }
+    } else {
+#pragma unroll
+        for (uint32_t k = 0; k < 2; k++)
+            if (xb + k < w) {
+                out_r[3 * (at + k) + 0] = res[k][0];
+                out_r[3 * (at + k) + 1] = res[k][1];
+                out_r[3 * (at + k) + 2] = res[k][2];
+            }
+    }
+}
+
+/* One 64x32 tile of output per 256-lane workgroup: normalized code values
+ * with a 3-sensel halo and green-pass cells with a 1-sensel halo live in
+ * LDS; each lane handles a red/blue + green pair of sensels so every lane
+ * does the same work. Output sensel (x, y) goes to (y - oy) * ostride +
+ * (x - ox) of the planes (ox = oy = 0, ostride = width for a whole frame). */
+__device__ inline void develop_tile(const DevelopDesc &d, float *out_r, float *out_g,
+                                    float *out_b, uint32_t bx, uint32_t by, DevelopLds &lds,
+                                    uint32_t ox, uint32_t oy, uint32_t ostride)
+{
+    const DevK k = dev_consts(d);
     const uint32_t tid = threadIdx.x;
-    const uint32_t w = d.width, h = d.height;
+    const uint32_t w = k.w, h = k.h;
     const int32_t x0 = (int32_t)(bx * DTW), y0 = (int32_t)(by * DTH);
     if (tid < 9)
-        curve[tid] = tid < 8 ? d.curve[tid] : 65536u;
+        lds.curve[tid] = tid < 8 ? d.curve[tid] : 65536u;
     __syncthreads();
 
-    /* Normalized values (Conversion::new folded table, demosaic.rs:293-298),
-     * rows and columns clamped to the picture. */
+    /* Normalized values, rows and columns clamped to the picture. */
     for (uint32_t i = tid; i < NORM_H * NORM_W; i += 256) {
         int32_t ly = (int32_t)(i / NORM_W), lx = (int32_t)(i - (uint32_t)ly * NORM_W);
         int32_t y = y0 - 3 + ly, x = x0 - 3 + lx;
         y = y < 0 ? 0 : y > (int32_t)h - 1 ? (int32_t)h - 1 : y;
         x = x < 0 ? 0 : x > (int32_t)w - 1 ? (int32_t)w - 1 : x;
-        uint32_t code = d.samples[(size_t)y * d.stride + (uint32_t)x];
-        int32_t lin = (int32_t)linearize(curve, code);
-        norm[ly][lx] = d.one_over_range * (float)(lin - d.black);
+        lds.norm[ly][lx] = norm_of(d, lds.curve, d.samples[(size_t)y * d.stride + (uint32_t)x]);
     }
     __syncthreads();
 
-    /* Green pass (green_row_impl, demosaic.rs:481-541). Each lane takes one
-     * (green, red/blue) pair of a row and does the green sensel, then the
-     * red/blue one, so every lane of a wave runs the same code whatever the
-     * Bayer phase of its row. */
-    const float n0 = noise[0], n1 = noise[1], n2 = noise[2];
-    const float bs = (float)d.black * d.one_over_range;
-    const float coff = d.black_over_range; /* the chroma pass offset */
-    const uint32_t red_row = (uint32_t)(d.cfa_phase >> 1), red_col = (uint32_t)(d.cfa_phase & 1);
+    /* Green pass. Each lane takes one (green, red/blue) pair of a row and
+     * does the green sensel, then the red/blue one, so every lane of a wave
+     * runs the same code whatever the Bayer phase of its row. */
+    auto norm = [&](int32_t y, int32_t x) { return lds.norm[y - (y0 - 3)][x - (x0 - 3)]; };
     for (uint32_t i = tid; i < GRN_H * (GRN_W / 2); i += 256) {
         const int32_t ly = (int32_t)(i / (GRN_W / 2));
         const int32_t lp = (int32_t)(i - (uint32_t)ly * (GRN_W / 2));
         const int32_t gy = y0 - 1 + ly;
         if (gy < 0 || gy >= (int32_t)h)
             continue;
-        const bool phase = ((uint32_t)gy & 1u) == red_row;
-        const int32_t first = (int32_t)(phase ? red_col : 1 - red_col);
+        const bool phase = ((uint32_t)gy & 1u) == k.red_row;
+        const int32_t first = (int32_t)(phase ? k.red_col : 1 - k.red_col);
         const float clip = phase ? d.clip[0] : d.clip[1];
-        const int32_t ny = gy - (y0 - 3);
         auto in_range = [&](int32_t gx) {
             return gx >= x0 - 1 && gx <= x0 + DTW && gx >= 0 && gx < (int32_t)w;
         };
         {
             const int32_t gx = x0 - 2 + 2 * lp + (1 - first);
-            if (in_range(gx)) {
-                const float centre = norm[ny][gx - (x0 - 3)];
-                green[ly][gx - (x0 - 2)] =
-                    make_float4(0.0f, centre, 0.0f, sqrtf(variance(centre, coff, n0, n1, n2)));
-            }
+            if (in_range(gx))
+                lds.green[ly][gx - (x0 - 2)] = green_g(k, norm(gy, gx));
         }
         const int32_t gx = x0 - 2 + 2 * lp + first;
         if (!in_range(gx))
             continue;
-        const int32_t nx = gx - (x0 - 3);
-        const float centre = norm[ny][nx];
-        {
-            const float c = fminf(centre, clip);
-            float hh[4] = {norm[ny][nx - 1], norm[ny][nx + 1], norm[ny][nx - 2], norm[ny][nx + 2]};
-            float vv[4] = {norm[ny - 1][nx], norm[ny + 1][nx], norm[ny - 2][nx], norm[ny + 2][nx]};
-            hh[0] = fminf(hh[0], 1.0f);
-            hh[1] = fminf(hh[1], 1.0f);
-            hh[2] = fminf(hh[2], clip);
-            hh[3] = fminf(hh[3], clip);
-            vv[0] = fminf(vv[0], 1.0f);
-            vv[1] = fminf(vv[1], 1.0f);
-            vv[2] = fminf(vv[2], clip);
-            vv[3] = fminf(vv[3], clip);
-            mirror4(hh, (uint32_t)gx, w, c);
-            mirror4(vv, (uint32_t)gy, h, c);
-            float var_h[4], var_v[4];
-#pragma unroll
-            for (int q = 0; q < 4; q++) {
-                var_h[q] = variance(hh[q], bs, n0, n1, n2);
-                var_v[q] = variance(vv[q], bs, n0, n1, n2);
-            }
-            const float sums0 = hh[0] + hh[1], sums1 = vv[0] + vv[1];
-            const float sums2 = hh[2] + hh[3], sums3 = vv[2] + vv[3];
-            float ratio0 = 0.0f, ratio1 = 0.0f;
-            if (c > sqrtf(variance(c, bs, n0, n1, n2))) {
-                const float t0 = c * 2.0f + sums2, t1 = c * 2.0f + sums3;
-                const float t2 = c * -2.0f + sums2, t3 = c * -2.0f + sums3;
-#pragma unroll
-                for (int dir = 0; dir < 2; dir++) {
-                    const float d0 = dir ? vv[0] : hh[0], d1 = dir ? vv[1] : hh[1];
-                    const float d2 = dir ? vv[2] : hh[2], d3 = dir ? vv[3] : hh[3];
-                    const float sd0 = sqrtf(dir ? var_v[0] : var_h[0]);
-                    const float sd1 = sqrtf(dir ? var_v[1] : var_h[1]);
-                    const float sd2 = sqrtf(dir ? var_v[2] : var_h[2]);
-                    const float sd3 = sqrtf(dir ? var_v[3] : var_h[3]);
-                    float a = d0 - d1, b = d2 - d3;
-                    bool any = a * b > 0.0f || (d2 - c) * (d3 - c) > 0.0f ||
-                               sd0 + sd1 > fabsf(a) || sd2 + sd3 > fabsf(b);
-                    bool all = d0 > sd0 && d1 > sd1 && d2 > sd2 && d3 > sd3;
-                    if (any && all) {
-                        if (dir == 0)
-                            ratio0 = t2 / (t0 * 2.0f);
-                        else
-                            ratio1 = t3 / (t1 * 2.0f);
-                    }
-                }
-            }
-            const float gh = sums0 * 0.5f * (1.0f - ratio0);
-            const float gv = sums1 * 0.5f * (1.0f - ratio1);
-            const float dh0 = hh[0] - gh, dh1 = hh[1] - gh, dh2 = hh[2] - c, dh3 = hh[3] - c;
-            const float dv0 = vv[0] - gv, dv1 = vv[1] - gv, dv2 = vv[2] - c, dv3 = vv[3] - c;
-            const float eh = dot4(dh0, dh1, dh2, dh3, dh0, dh1, dh2, dh3) + (var_h[0] + var_h[1]);
-            const float ev = dot4(dv0, dv1, dv2, dv3, dv0, dv1, dv2, dv3) + (var_v[0] + var_v[1]);
-            const float gout = gv + (gh - gv) * ev / (eh + ev);
-            green[ly][gx - (x0 - 2)] =
-                make_float4(c, gout, sqrtf(variance(c, coff, n0, n1, n2)),
-                            sqrtf(variance(gout, coff, n0, n1, n2)));
-        }
+        lds.green[ly][gx - (x0 - 2)] = green_rb(k, gx, gy, clip, norm);
     }
     __syncthreads();
 
-    /* Chroma pass (chroma_row_with, demosaic.rs:648-688) and the output
-     * matrix (matrix_planes_impl, demosaic.rs:711-720). Each lane takes the
+    /* Chroma pass, white balance and output matrix: each lane takes the
      * red/blue pixel and the green pixel of one pair. */
-    const float off = d.black_over_range;
+    auto green = [&](uint32_t y, uint32_t x) {
+        return lds.green[(int32_t)y - (y0 - 1)][(int32_t)x - (x0 - 2)];
+    };
     for (uint32_t i = tid; i < DTH * (DTW / 2); i += 256) {
         const uint32_t ly = i / (DTW / 2), lp = i - ly * (DTW / 2);
         const uint32_t y = (uint32_t)y0 + ly;
-        if (y >= h)
-            continue;
-        const uint32_t ym = y > 0 ? y - 1 : y + 1, yp = y < h - 1 ? y + 1 : y - 1;
-        const bool on_red_row = (y & 1u) == red_row;
-        const uint32_t rb_col = on_red_row ? red_col : 1 - red_col;
-        const int32_t gr_m = (int32_t)ym - (y0 - 1), gr_c = (int32_t)y - (y0 - 1),
-                      gr_p = (int32_t)yp - (y0 - 1);
         const uint32_t xb = (uint32_t)x0 + 2 * lp;
-        if (xb >= w)
+        if (y >= h || xb >= w)
             continue;
-        auto finish = [&](float rr, float gval, float bb, float *res) {
-            float r = fminf(rr * d.wb[0], 1.0f);
-            float g = fminf(gval, 1.0f);
-            float b = fminf(bb * d.wb[1], 1.0f);
-            if (d.has_matrix) {
-                const float r2 = fmaf(b, d.m[2], fmaf(r, d.m[0], g * d.m[1]));
-                const float g2 = fmaf(b, d.m[5], fmaf(r, d.m[3], g * d.m[4]));
-                const float b2 = fmaf(b, d.m[8], fmaf(r, d.m[6], g * d.m[7]));
-                r = r2;
-                g = g2;
-                b = b2;
-            }
-            res[0] = r;
-            res[1] = g;
-            res[2] = b;
-        };
-        float res_rb[3], res_g[3];
-        {
-            /* The red or blue pixel: diagonal neighbours. */
-            const uint32_t x = xb + rb_col;
-            const uint32_t xm = x > 0 ? x - 1 : x + 1, xp = x < w - 1 ? x + 1 : x - 1;
-            const int32_t cm = (int32_t)xm - (x0 - 2), cc = (int32_t)x - (x0 - 2),
-                          cp = (int32_t)xp - (x0 - 2);
-            const float4 mid = green[gr_c][cc];
-            const float4 a = green[gr_m][cm], b = green[gr_p][cm], c2 = green[gr_m][cp],
-                         e = green[gr_p][cp];
-            const float xx[4] = {a.x, b.x, c2.x, e.x};
-            const float gg[4] = {a.y, b.y, c2.y, e.y};
-            const float sx[4] = {a.z, b.z, c2.z, e.z};
-            const float sg[4] = {a.w, b.w, c2.w, e.w};
-            const float est = estimate_pre(xx, gg, sx, sg, mid.y, mid.w, off, n0, n1, n2);
-            finish(on_red_row ? mid.x : est, mid.y, on_red_row ? est : mid.x, res_rb);
-        }
-        {
-            /* The green pixel: horizontal and vertical neighbour pairs. */
-            const uint32_t x = xb + (1 - rb_col);
-            const uint32_t xm = x > 0 ? x - 1 : x + 1, xp = x < w - 1 ? x + 1 : x - 1;
-            const int32_t cm = (int32_t)xm - (x0 - 2), cc = (int32_t)x - (x0 - 2),
-                          cp = (int32_t)xp - (x0 - 2);
-            const float4 mid = green[gr_c][cc];
-            const float4 l = green[gr_c][cm], r = green[gr_c][cp];
-            const float4 u = green[gr_m][cc], dn = green[gr_p][cc];
-            const float hx[4] = {l.x, r.x, l.x, r.x}, hg[4] = {l.y, r.y, l.y, r.y};
-            const float hsx[4] = {l.z, r.z, l.z, r.z}, hsg[4] = {l.w, r.w, l.w, r.w};
-            const float vx[4] = {u.x, dn.x, u.x, dn.x}, vg[4] = {u.y, dn.y, u.y, dn.y};
-            const float vsx[4] = {u.z, dn.z, u.z, dn.z}, vsg[4] = {u.w, dn.w, u.w, dn.w};
-            const float eh = estimate_pre(hx, hg, hsx, hsg, mid.y, mid.w, off, n0, n1, n2);
-            const float ev = estimate_pre(vx, vg, vsx, vsg, mid.y, mid.w, off, n0, n1, n2);
-            finish(on_red_row ? eh : ev, mid.y, on_red_row ? ev : eh, res_g);
-        }
         float res[2][3];
-#pragma unroll
-        for (int c = 0; c < 3; c++) {
-            res[0][c] = rb_col == 0 ? res_rb[c] : res_g[c];
-            res[1][c] = rb_col == 0 ? res_g[c] : res_rb[c];
-        }
-        const uint32_t xs[2] = {xb, xb + 1};
-        const size_t at = (size_t)(y - oy) * ostride + (xs[0] - ox);
-        if (out_g) {
-            if (xs[1] < w) {
-                *(float2 *)(out_r + at) = make_float2(res[0][0], res[1][0]);
-                *(float2 *)(out_g + at) = make_float2(res[0][1], res[1][1]);
-                *(float2 *)(out_b + at) = make_float2(res[0][2], res[1][2]);
-            } else if (xs[0] < w) {
-                out_r[at] = res[0][0];
-                out_g[at] = res[0][1];
-                out_b[at] = res[0][2];
-            }
-        } else {
-#pragma unroll
-            for (uint32_t k = 0; k < 2; k++)
-                if (xs[k] < w) {
-                    out_r[3 * (at + k) + 0] = res[k][0];
-                    out_r[3 * (at + k) + 1] = res[k][1];
-                    out_r[3 * (at + k) + 2] = res[k][2];
-                }
-        }
+        chroma_pair(d, k, y, xb, green, res);
+        store_pair(out_r, out_g, out_b, (size_t)(y - oy) * ostride + (xb - ox), xb, w, res);
     }
 }
 
@@ -1346,7 +1563,7 @@ __device__ inline T load_uniform(const T *p)
  * status; the last workgroup also resets the queue head. */
 __global__ __launch_bounds__(1024) void k_batch_scan(const BatchFrame *frames, uint32_t *head)
 {
-    __shared__ uint64_t sums[1024];
+    __shared__ uint64_t sums[32];
     const BatchFrame &bf = frames[blockIdx.x];
     if (threadIdx.x == 0) {
         bf.status->oob_tile = 0xFFFFFFFFu;
@@ -1357,7 +1574,10 @@ __global__ __launch_bounds__(1024) void k_batch_scan(const BatchFrame *frames, u
             *head = 0;
     }
     __syncthreads();
-    scan_frame(bf.f, bf.status, sums);
+    if (bf.f.nb_tiles <= PREP_SMALL)
+        scan_frame_small(bf.f, bf.status, sums, nullptr, 1);
+    else
+        scan_frame(bf.f, bf.status, sums);
 }
 
 /* Order the tile groups of the batch by coded length, longest first: a
@@ -1439,7 +1659,8 @@ __global__ __launch_bounds__(64) void k_batch_decode(const BatchFrame *__restric
         for (uint32_t i = start; i < stop; i++) {
             const uint32_t item = __builtin_amdgcn_readfirstlane(queue[i]);
             const BatchFrame bf = load_uniform(frames + (item >> 20));
-            const uint32_t g = item & 0xFFFFFu;
+            const uint32_t T = bf.spw >> 2, t0 = (item & 0xFFFFFu) * T;
+            const TileSet g{nullptr, t0, min(T, bf.f.nb_tiles - t0)};
             switch (bf.mode) {
             case BATCH_FULL_APPLE:
                 decode_group_full<0>(bf.f, g, bf.samples, bf.linear, bf.flags, bf.status, bf.spw,
@@ -1480,18 +1701,6 @@ __global__ __launch_bounds__(256) void k_batch_develop(const BatchDevelop *jobs)
 /* ------------------------------------------------------------------ */
 /* Launchers. */
 
-hipError_t launch_status_init(prr_device_status *status, hipStream_t s)
-{
-    hipLaunchKernelGGL(k_status_init, dim3(1), dim3(64), 0, s, status);
-    return hipGetLastError();
-}
-
-hipError_t launch_scan(const FrameDesc &f, prr_device_status *status, hipStream_t s)
-{
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, f, status);
-    return hipGetLastError();
-}
-
 static uint32_t pick_spw(const DecodeLaunch &cfg, uint32_t per_block, uint32_t amax, size_t &lds)
 {
     uint32_t spw = cfg.streams_per_wg;
@@ -1503,45 +1712,59 @@ static uint32_t pick_spw(const DecodeLaunch &cfg, uint32_t per_block, uint32_t a
     }
 }
 
+/* Each decode launch is preceded by k_prepare: status reset, size-table
+ * scan, and the frame's tiles ordered longest first into `order`
+ * (nb_tiles entries). Workgroups: `solo` single tiles, then groups of
+ * spw / 4. */
+static uint32_t single_frame_items(uint32_t n, uint32_t spw, uint32_t solo)
+{
+    const uint32_t T = spw / 4;
+    return solo + (n - solo + T - 1) / T;
+}
+
 hipError_t launch_decode(const FrameDesc &f, int ffmpeg, uint16_t *samples, uint16_t *linear,
-                         uint8_t *flags, prr_device_status *status, const DecodeLaunch &cfg,
-                         hipStream_t s)
+                         uint8_t *flags, prr_device_status *status, uint32_t *order,
+                         const DecodeLaunch &cfg, hipStream_t s)
 {
     size_t lds;
-    uint32_t spw = pick_spw(cfg, 64, f.alignment, lds);
-    uint32_t T = spw / 4;
-    uint32_t groups = (f.nb_tiles + T - 1) / T;
-    if (groups == 0)
-        return hipSuccess;
+    const uint32_t spw = pick_spw(cfg, 64, f.alignment, lds);
+    const uint32_t n = f.nb_tiles, solo = min(n, cfg.solo_tiles);
+    const uint32_t items = single_frame_items(n, spw, solo);
+    hipLaunchKernelGGL(k_prepare, dim3(1), dim3(1024), 0, s, f, status, order, 1u);
+    if (items == 0)
+        return hipGetLastError();
+    const uint32_t *o = order;
     if (ffmpeg)
-        hipLaunchKernelGGL(k_decode<1>, dim3(groups), dim3(64), lds, s, f, samples, linear, flags,
-                           status, spw);
+        hipLaunchKernelGGL(k_decode<1>, dim3(items), dim3(64), lds, s, f, samples, linear, flags,
+                           status, spw, o, solo);
     else
-        hipLaunchKernelGGL(k_decode<0>, dim3(groups), dim3(64), lds, s, f, samples, linear, flags,
-                           status, spw);
+        hipLaunchKernelGGL(k_decode<0>, dim3(items), dim3(64), lds, s, f, samples, linear, flags,
+                           status, spw, o, solo);
     return hipGetLastError();
 }
 
 hipError_t launch_decode_reduced(const FrameDesc &f, int shift, int average, uint16_t *rgb,
-                                 uint8_t *flags, prr_device_status *status,
+                                 uint8_t *flags, prr_device_status *status, uint32_t *order,
                                  const DecodeLaunch &cfg, hipStream_t s)
 {
-    uint32_t per_block = shift == 1 ? 64u : shift == 2 ? 16u : 4u;
+    const uint32_t per_block = shift == 1 ? 64u : shift == 2 ? 16u : 4u;
     size_t lds;
-    uint32_t spw = pick_spw(cfg, per_block, f.alignment, lds);
-    uint32_t T = spw / 4;
-    uint32_t groups = (f.nb_tiles + T - 1) / T;
-    if (groups == 0)
-        return hipSuccess;
+    const uint32_t spw = pick_spw(cfg, per_block, f.alignment, lds);
+    const uint32_t n = f.nb_tiles, solo = min(n, cfg.solo_tiles);
+    const uint32_t items = single_frame_items(n, spw, solo);
+    hipLaunchKernelGGL(k_prepare, dim3(1), dim3(1024), 0, s, f, status, order, 1u);
+    if (items == 0)
+        return hipGetLastError();
+    const uint32_t *o = order;
     if (shift == 1)
-        hipLaunchKernelGGL(k_reduced<1>, dim3(groups), dim3(64), lds, s, f, rgb, flags, status,
-                           spw, (uint32_t)average);
+        hipLaunchKernelGGL(k_reduced<1>, dim3(items), dim3(64), lds, s, f, rgb, flags, status,
+                           spw, (uint32_t)average, o, solo);
     else if (shift == 2)
-        hipLaunchKernelGGL(k_reduced<2>, dim3(groups), dim3(64), lds, s, f, rgb, flags, status,
-                           spw, (uint32_t)average);
+        hipLaunchKernelGGL(k_reduced<2>, dim3(items), dim3(64), lds, s, f, rgb, flags, status,
+                           spw, (uint32_t)average, o, solo);
     else
-        hipLaunchKernelGGL(k_reduced<3>, dim3(groups), dim3(64), lds, s, f, rgb, flags, status,
-                           spw, (uint32_t)average);
+        hipLaunchKernelGGL(k_reduced<3>, dim3(items), dim3(64), lds, s, f, rgb, flags, status,
+                           spw, (uint32_t)average, o, solo);
     return hipGetLastError();
 }
 
@@ -1592,6 +1815,7 @@ hipError_t launch_batch(const BatchFrame *d_frames, uint32_t n_frames, uint32_t 
 }
 
 const void *batch_decode_kernel() { return (const void *)k_batch_decode; }
+
 
 namespace net {
 hipError_t launch_develop_sampled(const DevelopDesc &d, const uint32_t *tiles, uint32_t n_tiles,

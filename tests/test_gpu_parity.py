@@ -108,3 +108,31 @@ def test_develop_random_frames_all_stages(gpu):
         oprm = O.params(h, q, 1.8, 1.3)
         for stage in (0, 1, 2):
             check_develop(f, h, s, prm, oprm, stage)
+
+
+def test_frames_beyond_the_register_scan(gpu):
+    """More than 8192 tiles: the size-table scan takes its general path
+    (one frame decode, one reduced decode, one batched frame)."""
+    from proresraw_rs_amd.batch import Batch
+    import torch
+    sc = S.scene(2048, 1088, 1, seed=21, version=1, alignment=0, qscale=(15, 45),
+                 corrupt_fraction=0.02, threads=4)
+    pk = S.encode_scene(sc)
+    hdr, info = P.parse_frame_info(pk)
+    say(f"2048x1088 a0: {info.nb_tiles} tiles")
+    assert info.nb_tiles > 8192
+    f = P.decode_frame(pk)
+    h, s, sk, mq = O.decode_frame(pk)
+    assert f.skipped_tiles == sk and len(sk) > 0 and f.max_qscale == mq
+    np.testing.assert_array_equal(f.samples, s)
+    r = P.decode_frame_reduced_with(pk, P.Scale.Half, P.Greens.Average)
+    h, rgb, sk2, mq2 = O.decode_frame_reduced(pk, 1, True)
+    assert r.skipped_tiles == sk2 and r.max_qscale == mq2
+    np.testing.assert_array_equal(r.samples, rgb)
+    b = Batch()
+    slot = b.add_full(pk, P.Idct.Apple)
+    st = torch.cuda.Stream()
+    b.run(st)
+    st.synchronize()
+    assert slot.skipped_tiles() == sk and slot.status_host()[1] == mq
+    np.testing.assert_array_equal(slot.samples.cpu().numpy().view(np.uint16), s)
