@@ -63,6 +63,13 @@ struct Finish {
     float m[9];
 };
 
+/* Inference convolution modes (k_conv_infer) and the channels of the
+ * network output the composition reads. */
+enum { IM_HEAD = 0, IM_RES = 1, IM_TAIL = 2, OUT_CH = 12 };
+hipError_t launch_conv_infer(int mode, const uint16_t *in, int h, int w, const uint16_t *wgt,
+                             const float *bias, uint16_t *out, float *out12, const uint32_t *skip,
+                             uint32_t grid, hipStream_t s);
+const void *conv_infer_kernel();
 hipError_t launch_weights(const float *params, uint16_t *wf, uint16_t *wt, hipStream_t s);
 hipError_t launch_conv(int epi, const uint16_t *in, int n, int h, int w, const uint16_t *wgt,
                        const float *bias, const uint16_t *resid, uint16_t *out, uint16_t *aux,
@@ -83,22 +90,30 @@ hipError_t launch_pack_crops(const NetFrame *frames, const Crop *crops, const ui
 hipError_t launch_loss(const NetFrame *frames, const Crop *crops, const uint32_t *step,
                        uint32_t steps, int n, int h2, int w2, int margin, const float *out,
                        uint16_t *dout, float *loss_part, float *loss, hipStream_t s);
-/* Inference over a whole frame (even width and height). */
-hipError_t launch_pack_frame(const NetFrame &f, uint16_t *x0, hipStream_t s);
+/* Inference over a whole frame (even width and height); nothing when
+ * *skip is set. */
+hipError_t launch_pack_frame(const NetFrame &f, uint16_t *x0, const uint32_t *skip, hipStream_t s);
+/* The sampled-tile windows (WIN x WIN cells at most) packed as n images
+ * of sy x sx cells for the training convolution. */
+enum { WIN = 48 };
+hipError_t launch_pack_windows(const NetFrame &f, const uint32_t *tiles, int n, int sy, int sx,
+                               uint16_t *x0, hipStream_t s);
 hipError_t launch_compose(const NetFrame &f, const Finish &fin, const float *out, float *r,
-                          float *g, float *b, hipStream_t s);
+                          float *g, float *b, const uint32_t *skip, hipStream_t s);
 /* Plain develop of sampled 64x64 tiles (tiles[2k], tiles[2k+1] = column
  * of 64, row of 64) into scratch (n_tiles * 3 * 4096 floats). */
 hipError_t launch_develop_sampled(const DevelopDesc &d, const uint32_t *tiles, uint32_t n_tiles,
                                   float *scratch, hipStream_t s);
 /* Sampled-tile check of the demosaic: the plain develop of each tile at
  * the stage's white balance and clip without the output matrix (d must
- * have has_matrix = 0) against the network's output finished the same way;
- * PSNR per tile into psnr, *use_plain set when any is below min_db. */
+ * have has_matrix = 0) against the network's output on the tile's window
+ * (wout: n_tiles images of sy x sx cells, 32 fp32 channels) finished the
+ * same way; PSNR per tile into psnr, *use_plain set when any is below
+ * min_db. */
 hipError_t launch_check(const DevelopDesc &d, const NetFrame &f, const Finish &fin,
-                        const float *out, const uint32_t *tiles, uint32_t n_tiles,
-                        float *scratch, float min_db, float *psnr, uint32_t *use_plain,
-                        hipStream_t s);
+                        const float *wout, int sy, int sx, const uint32_t *tiles,
+                        uint32_t n_tiles, float *scratch, float min_db, float *psnr,
+                        uint32_t *use_plain, hipStream_t s);
 hipError_t launch_gate_clear(uint32_t *flag, uint32_t value, hipStream_t s);
 /* Sets *use_plain when the frame's noise feature lies outside [lo, hi]. */
 hipError_t launch_noise_gate(const NetFrame &f, float lo, float hi, uint32_t *use_plain,

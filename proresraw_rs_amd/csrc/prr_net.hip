@@ -22,6 +22,8 @@
  */
 #include "prr_net.h"
 
+#include <algorithm>
+
 namespace prr {
 namespace net {
 
@@ -178,6 +180,150 @@ __global__ __launch_bounds__(256) void k_conv(const uint16_t *__restrict__ in, i
                     o[i] = f2bf(v[i]);
             }
             *(uint2 *)(out + px * C + co) = make_uint2(pack2(o[0], o[1]), pack2(o[2], o[3]));
+        }
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* Inference convolution: the arithmetic of k_conv (the same MFMAs in the
+ * same order, so the same bits), in a persistent workgroup that keeps the
+ * layer's weights in registers (A operands of all 9 taps, loaded once),
+ * walks the frame's 4 x 64 tiles in a grid stride, and loads the next
+ * tile's input into registers while the current one is on the matrix
+ * cores. Vertically adjacent tiles are a multiple of 8 tiles apart, so
+ * with workgroups spread round robin over the 8 L2s a tile's halo rows are
+ * usually its neighbours' rows in the same L2.
+ *   IM_HEAD  h = relu(conv(x0) + b)
+ *   IM_RES   h' = h + relu(conv(h) + b) with h read from the staged tile
+ *   IM_TAIL  out[px][0..11] = conv(h) + b in fp32, the 12 channels the
+ *            composition reads (output-channel tile 0 only)
+ * A non-zero *skip (the fallback flag) makes the launch do nothing. */
+
+enum { IT_SLOTS = (CT_ROWS + 2) * (CT_COLS + 2) * 4, IT_PER = (IT_SLOTS + 255) / 256 };
+
+template <int MODE>
+__global__ __launch_bounds__(256) void k_conv_infer(const uint16_t *__restrict__ in, int H, int W,
+                                                    const uint16_t *__restrict__ wgt,
+                                                    const float *__restrict__ bias,
+                                                    uint16_t *__restrict__ out,
+                                                    float *__restrict__ out12,
+                                                    const uint32_t *__restrict__ skip)
+{
+    __shared__ __attribute__((aligned(16))) uint16_t s_in[CT_ROWS + 2][CT_COLS + 2][CP];
+    if (skip && *skip)
+        return;
+    constexpr int MT = MODE == IM_TAIL ? 1 : 2;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int r16 = lane & 15, kq = (lane >> 4) * 8;
+    const int tiles_x = (W + CT_COLS - 1) / CT_COLS;
+    const int n_tiles = tiles_x * ((H + CT_ROWS - 1) / CT_ROWS);
+
+    bf16x8 wa[TAPS][MT];
+#pragma unroll
+    for (int t = 0; t < TAPS; t++)
+#pragma unroll
+        for (int mt = 0; mt < MT; mt++)
+            wa[t][mt] = *(const bf16x8 *)(wgt + ((size_t)(t * C + 16 * mt + r16) * C + kq));
+    float bv[MT][4];
+#pragma unroll
+    for (int mt = 0; mt < MT; mt++)
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+            bv[mt][i] = bias[16 * mt + (lane >> 4) * 4 + i];
+
+    uint4 pre[IT_PER];
+    auto fetch = [&](int tile) {
+        const int ty = tile / tiles_x, tx = tile - ty * tiles_x;
+        const int x0 = tx * CT_COLS, y0 = ty * CT_ROWS;
+#pragma unroll
+        for (int k = 0; k < IT_PER; k++) {
+            const int i = tid + 256 * k;
+            const int q = i & 3, p = i >> 2;
+            const int ly = p / (CT_COLS + 2), lx = p - ly * (CT_COLS + 2);
+            const int y = y0 + ly - 1, x = x0 + lx - 1;
+            uint4 v = make_uint4(0, 0, 0, 0);
+            if (i < IT_SLOTS && y >= 0 && y < H && x >= 0 && x < W)
+                v = ((const uint4 *)(in + ((size_t)y * W + x) * C))[q];
+            pre[k] = v;
+        }
+    };
+    int tile = blockIdx.x;
+    if (tile < n_tiles)
+        fetch(tile);
+    for (; tile < n_tiles; tile += gridDim.x) {
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < IT_PER; k++) {
+            const int i = tid + 256 * k;
+            if (i < IT_SLOTS) {
+                const int q = i & 3, p = i >> 2;
+                const int ly = p / (CT_COLS + 2), lx = p - ly * (CT_COLS + 2);
+                *(uint4 *)&s_in[ly][lx][8 * q] = pre[k];
+            }
+        }
+        __syncthreads();
+        const int ty = tile / tiles_x, tx = tile - ty * tiles_x;
+        const int x0 = tx * CT_COLS, y0 = ty * CT_ROWS;
+        if (tile + (int)gridDim.x < n_tiles)
+            fetch(tile + gridDim.x);
+
+        f32x4 acc[MT][4];
+#pragma unroll
+        for (int mt = 0; mt < MT; mt++)
+#pragma unroll
+            for (int nt = 0; nt < 4; nt++)
+                acc[mt][nt] = (f32x4){0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int t = 0; t < TAPS; t++) {
+            const int ky = t / 3, kx = t % 3;
+            bf16x8 b[4];
+#pragma unroll
+            for (int nt = 0; nt < 4; nt++)
+                b[nt] = *(const bf16x8 *)&s_in[wave + ky][16 * nt + r16 + kx][kq];
+#pragma unroll
+            for (int mt = 0; mt < MT; mt++)
+#pragma unroll
+                for (int nt = 0; nt < 4; nt++)
+                    acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa[t][mt], b[nt],
+                                                                          acc[mt][nt], 0, 0, 0);
+        }
+
+        const int y = y0 + wave;
+        if (y >= H)
+            continue;
+#pragma unroll
+        for (int nt = 0; nt < 4; nt++) {
+            const int x = x0 + 16 * nt + r16;
+            if (x >= W)
+                continue;
+            const size_t px = (size_t)y * W + x;
+#pragma unroll
+            for (int mt = 0; mt < MT; mt++) {
+                const int co = 16 * mt + (lane >> 4) * 4;
+                float v[4];
+#pragma unroll
+                for (int i = 0; i < 4; i++)
+                    v[i] = acc[mt][nt][i] + bv[mt][i];
+                if (MODE == IM_TAIL) {
+                    if (co < OUT_CH)
+                        *(float4 *)(out12 + px * OUT_CH + co) = make_float4(v[0], v[1], v[2], v[3]);
+                    continue;
+                }
+                uint16_t o[4];
+                if (MODE == IM_HEAD) {
+#pragma unroll
+                    for (int i = 0; i < 4; i++)
+                        o[i] = f2bf(fmaxf(v[i], 0.0f));
+                } else {
+                    const uint2 rv = *(const uint2 *)&s_in[wave + 1][16 * nt + r16 + 1][co];
+                    const uint16_t rs[4] = {(uint16_t)rv.x, (uint16_t)(rv.x >> 16), (uint16_t)rv.y,
+                                            (uint16_t)(rv.y >> 16)};
+#pragma unroll
+                    for (int i = 0; i < 4; i++)
+                        o[i] = f2bf(bf2f(rs[i]) + bf2f(f2bf(fmaxf(v[i], 0.0f))));
+                }
+                *(uint2 *)(out + px * C + co) = make_uint2(pack2(o[0], o[1]), pack2(o[2], o[3]));
+            }
         }
     }
 }
@@ -417,14 +563,49 @@ __global__ void k_pack_crops(const NetFrame *__restrict__ frames, const Crop *__
     pack_cell(f, (int)cr.y0 + 2 * ci, (int)cr.x0 + 2 * cj, x0 + i * C);
 }
 
-__global__ void k_pack_frame(NetFrame f, uint16_t *__restrict__ x0)
+__global__ void k_pack_frame(NetFrame f, uint16_t *__restrict__ x0,
+                             const uint32_t *__restrict__ skip)
 {
+    if (skip && *skip)
+        return;
     const int w2 = (int)f.width / 2, h2 = (int)f.height / 2;
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (size_t)h2 * w2)
         return;
     const int ci = (int)(i / w2), cj = (int)(i - (size_t)ci * w2);
     pack_cell(f, 2 * ci, 2 * cj, x0 + i * C);
+}
+
+/* The window of S_y x S_x cells (at most WIN x WIN) around sampled 64x64
+ * tile (bx, by): 8 cells of context on each side (the network's receptive
+ * field), shifted to stay inside the frame. Where the window meets the
+ * frame's edge, the convolution's zero padding at the window's edge is the
+ * frame's own; elsewhere the tile's cells are 8 or more cells from the
+ * window's edge, so the network's output on the tile equals the
+ * whole-frame output. */
+__device__ inline void window_origin(const NetFrame &f, uint32_t bx, uint32_t by, int sy, int sx,
+                                     int &wy, int &wx)
+{
+    /* The tile's first cell in the network's (RGGB-order) coordinates. */
+    const int h = (int)f.height, w = (int)f.width;
+    const int ly = f.flip_y ? (h - min(64 * (int)by + 64, h)) / 2 : 32 * (int)by;
+    const int lx = f.flip_x ? (w - min(64 * (int)bx + 64, w)) / 2 : 32 * (int)bx;
+    wy = min(max(ly - 8, 0), h / 2 - sy);
+    wx = min(max(lx - 8, 0), w / 2 - sx);
+}
+
+__global__ void k_pack_windows(NetFrame f, const uint32_t *__restrict__ tiles, int n, int sy,
+                               int sx, uint16_t *__restrict__ x0)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t cells = (size_t)sy * sx;
+    if (i >= cells * n)
+        return;
+    const int k = (int)(i / cells), c = (int)(i - (size_t)k * cells);
+    int wy, wx;
+    window_origin(f, tiles[2 * k], tiles[2 * k + 1], sy, sx, wy, wx);
+    const int ci = c / sx, cj = c - ci * sx;
+    pack_cell(f, 2 * (wy + ci), 2 * (wx + cj), x0 + i * C);
 }
 
 /* ------------------------------------------------------------------ */
@@ -510,13 +691,16 @@ __global__ void k_loss_total(const float *__restrict__ part, uint32_t n, float i
 /* Inference output: out + bilinear, then white balance with the clip at 1
  * and the output matrix as develop's last steps do them (chroma_row_with
  * and matrix_planes_impl, demosaic.rs:648-720), written in sensor order.
- * One lane per cell. */
+ * One lane per cell; nothing when *skip is set (the frame falls back). */
 
 __global__ __launch_bounds__(256) void k_compose(NetFrame f, Finish fin,
                                                  const float *__restrict__ out,
                                                  float *__restrict__ pr, float *__restrict__ pg,
-                                                 float *__restrict__ pb)
+                                                 float *__restrict__ pb,
+                                                 const uint32_t *__restrict__ skip)
 {
+    if (skip && *skip)
+        return;
     const int w2 = (int)f.width / 2, h2 = (int)f.height / 2;
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (size_t)h2 * w2)
@@ -530,9 +714,9 @@ __global__ __launch_bounds__(256) void k_compose(NetFrame f, Finish fin,
             float base[3];
             bilinear(f, 0, 0, (int)f.height, (int)f.width, y, x, base);
             const int ch = 3 * (2 * dy + dx);
-            float r = fminf((out[i * C + ch + 0] + base[0]) * fin.wb[0], 1.0f);
-            float g = fminf(out[i * C + ch + 1] + base[1], 1.0f);
-            float b = fminf((out[i * C + ch + 2] + base[2]) * fin.wb[1], 1.0f);
+            float r = fminf((out[i * OUT_CH + ch + 0] + base[0]) * fin.wb[0], 1.0f);
+            float g = fminf(out[i * OUT_CH + ch + 1] + base[1], 1.0f);
+            float b = fminf((out[i * OUT_CH + ch + 2] + base[2]) * fin.wb[1], 1.0f);
             if (fin.has_matrix) {
                 const float r2 = fmaf(b, fin.m[2], fmaf(r, fin.m[0], g * fin.m[1]));
                 const float g2 = fmaf(b, fin.m[5], fmaf(r, fin.m[3], g * fin.m[4]));
@@ -554,7 +738,7 @@ __global__ __launch_bounds__(256) void k_compose(NetFrame f, Finish fin,
  * (out + bilinear, white balance and clip at 1, no matrix) against the
  * plain develop in scratch; PSNR with peak 1. One workgroup per tile. */
 __global__ __launch_bounds__(256) void k_check_psnr(NetFrame f, Finish fin,
-                                                    const float *__restrict__ out,
+                                                    const float *__restrict__ wout, int sy, int sx,
                                                     const uint32_t *__restrict__ tiles,
                                                     const float *__restrict__ scratch,
                                                     float min_db, float *__restrict__ psnr,
@@ -565,18 +749,20 @@ __global__ __launch_bounds__(256) void k_check_psnr(NetFrame f, Finish fin,
     const uint32_t k = blockIdx.x;
     const uint32_t x0 = 64 * tiles[2 * k], y0 = 64 * tiles[2 * k + 1];
     const float *plain = scratch + (size_t)k * 3 * 4096;
-    const int w2 = (int)f.width / 2;
+    int wy, wx;
+    window_origin(f, tiles[2 * k], tiles[2 * k + 1], sy, sx, wy, wx);
+    const float *out = wout + (size_t)k * sy * sx * C;
     double sum = 0.0;
     uint32_t n = 0;
     for (uint32_t i = threadIdx.x; i < 4096; i += 256) {
-        const uint32_t sy = y0 + i / 64, sx = x0 + i % 64;
-        if (sy >= f.height || sx >= f.width)
+        const uint32_t py = y0 + i / 64, px = x0 + i % 64; /* sensor order */
+        if (py >= f.height || px >= f.width)
             continue;
-        const int y = (int)(f.flip_y ? f.height - 1 - sy : sy);
-        const int x = (int)(f.flip_x ? f.width - 1 - sx : sx);
+        const int y = (int)(f.flip_y ? f.height - 1 - py : py);
+        const int x = (int)(f.flip_x ? f.width - 1 - px : px);
         float base[3];
         bilinear(f, 0, 0, (int)f.height, (int)f.width, y, x, base);
-        const size_t cell = (size_t)(y >> 1) * w2 + (x >> 1);
+        const size_t cell = (size_t)((y >> 1) - wy) * sx + ((x >> 1) - wx);
         const int ch = 3 * (2 * (y & 1) + (x & 1));
         const float r = fminf((out[cell * C + ch + 0] + base[0]) * fin.wb[0], 1.0f);
         const float g = fminf(out[cell * C + ch + 1] + base[1], 1.0f);
@@ -607,20 +793,51 @@ __global__ __launch_bounds__(256) void k_check_psnr(NetFrame f, Finish fin,
 
 /* ------------------------------------------------------------------ */
 
+hipError_t launch_pack_windows(const NetFrame &f, const uint32_t *tiles, int n, int sy, int sx,
+                               uint16_t *x0, hipStream_t s)
+{
+    const size_t cells = (size_t)n * sy * sx;
+    hipLaunchKernelGGL(k_pack_windows, dim3((uint32_t)((cells + 255) / 256)), dim3(256), 0, s, f,
+                       tiles, n, sy, sx, x0);
+    return hipGetLastError();
+}
+
 hipError_t launch_check(const DevelopDesc &d, const NetFrame &f, const Finish &fin,
-                        const float *out, const uint32_t *tiles, uint32_t n_tiles,
-                        float *scratch, float min_db, float *psnr, uint32_t *use_plain,
-                        hipStream_t s)
+                        const float *wout, int sy, int sx, const uint32_t *tiles,
+                        uint32_t n_tiles, float *scratch, float min_db, float *psnr,
+                        uint32_t *use_plain, hipStream_t s)
 {
     if (n_tiles == 0)
         return hipSuccess;
     hipError_t e = launch_develop_sampled(d, tiles, n_tiles, scratch, s);
     if (e != hipSuccess)
         return e;
-    hipLaunchKernelGGL(k_check_psnr, dim3(n_tiles), dim3(256), 0, s, f, fin, out, tiles, scratch,
-                       min_db, psnr, use_plain);
+    hipLaunchKernelGGL(k_check_psnr, dim3(n_tiles), dim3(256), 0, s, f, fin, wout, sy, sx, tiles,
+                       scratch, min_db, psnr, use_plain);
     return hipGetLastError();
 }
+
+hipError_t launch_conv_infer(int mode, const uint16_t *in, int h, int w, const uint16_t *wgt,
+                             const float *bias, uint16_t *out, float *out12, const uint32_t *skip,
+                             uint32_t grid, hipStream_t s)
+{
+    if (h <= 0 || w <= 0)
+        return hipSuccess;
+    const uint32_t tiles = (uint32_t)(((w + CT_COLS - 1) / CT_COLS) * ((h + CT_ROWS - 1) / CT_ROWS));
+    const dim3 g(std::min(grid, tiles));
+    if (mode == IM_HEAD)
+        hipLaunchKernelGGL(k_conv_infer<IM_HEAD>, g, dim3(256), 0, s, in, h, w, wgt, bias, out,
+                           out12, skip);
+    else if (mode == IM_RES)
+        hipLaunchKernelGGL(k_conv_infer<IM_RES>, g, dim3(256), 0, s, in, h, w, wgt, bias, out,
+                           out12, skip);
+    else
+        hipLaunchKernelGGL(k_conv_infer<IM_TAIL>, g, dim3(256), 0, s, in, h, w, wgt, bias, out,
+                           out12, skip);
+    return hipGetLastError();
+}
+
+const void *conv_infer_kernel() { return (const void *)k_conv_infer<IM_RES>; }
 
 hipError_t launch_weights(const float *params, uint16_t *wf, uint16_t *wt, hipStream_t s)
 {
@@ -737,20 +954,20 @@ hipError_t launch_loss(const NetFrame *frames, const Crop *crops, const uint32_t
     return hipGetLastError();
 }
 
-hipError_t launch_pack_frame(const NetFrame &f, uint16_t *x0, hipStream_t s)
+hipError_t launch_pack_frame(const NetFrame &f, uint16_t *x0, const uint32_t *skip, hipStream_t s)
 {
     const size_t cells = (size_t)(f.width / 2) * (f.height / 2);
     hipLaunchKernelGGL(k_pack_frame, dim3((uint32_t)((cells + 255) / 256)), dim3(256), 0, s, f,
-                       x0);
+                       x0, skip);
     return hipGetLastError();
 }
 
 hipError_t launch_compose(const NetFrame &f, const Finish &fin, const float *out, float *r,
-                          float *g, float *b, hipStream_t s)
+                          float *g, float *b, const uint32_t *skip, hipStream_t s)
 {
     const size_t cells = (size_t)(f.width / 2) * (f.height / 2);
     hipLaunchKernelGGL(k_compose, dim3((uint32_t)((cells + 255) / 256)), dim3(256), 0, s, f, fin,
-                       out, r, g, b);
+                       out, r, g, b, skip);
     return hipGetLastError();
 }
 

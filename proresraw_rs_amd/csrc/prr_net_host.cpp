@@ -60,10 +60,12 @@ struct prr_net {
     uint32_t decay = 0;
     /* inference */
     Buf ix0, ia, ib, iout, scratch, psnr, flag;
+    Buf wx0, wa, wb, wout; /* the sampled-tile windows */
     /* Sampled-tile lists per (width, height, count), uploaded on first use
      * so that later calls (and graph capture) copy nothing. */
     std::map<std::tuple<uint32_t, uint32_t, uint32_t>, Buf> tiles;
     uint32_t last_k = 0;
+    uint32_t conv_grid = 0; /* persistent inference convolution: workgroups */
 };
 
 static int grow(prr_net *net, Buf &b, size_t bytes, prr_error *err)
@@ -138,7 +140,8 @@ extern "C" void prr_net_destroy(prr_net *net)
                               &net->step, &net->frames, &net->crops, &net->x0, &net->out,
                               &net->dout, &net->dh[0], &net->dh[1], &net->dz, &net->part,
                               &net->loss_part, &net->loss, &net->ix0, &net->ia, &net->ib,
-                              &net->iout, &net->scratch, &net->psnr, &net->flag};
+                              &net->iout, &net->scratch, &net->psnr, &net->flag,
+                              &net->wx0, &net->wa, &net->wb, &net->wout};
     for (auto &kv : net->tiles)
         all.push_back(&kv.second);
     for (int l = 0; l < 7; l++) {
@@ -439,7 +442,15 @@ extern "C" int prr_net_develop_device(prr_net *net, const prr_header *hdr,
     GROW(net->ix0, 2 * cells * C);
     GROW(net->ia, 2 * cells * C);
     GROW(net->ib, 2 * cells * C);
-    GROW(net->iout, 4 * cells * C);
+    GROW(net->iout, 4 * cells * OUT_CH);
+    if (!net->conv_grid) {
+        hipDeviceProp_t prop;
+        NET_TRY(hipGetDeviceProperties(&prop, net->device), "hipGetDeviceProperties");
+        int per_cu = 0;
+        NET_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, conv_infer_kernel(), 256, 0),
+                "hipOccupancy");
+        net->conv_grid = (uint32_t)(prop.multiProcessorCount * std::max(per_cu, 1));
+    }
     const uint32_t k = std::min<uint32_t>(check_tiles, 64);
     GROW(net->scratch, sizeof(float) * 3 * 4096 * 64);
     GROW(net->psnr, sizeof(float) * 64);
@@ -464,25 +475,54 @@ extern "C" int prr_net_develop_device(prr_net *net, const prr_header *hdr,
 
     NET_TRY(launch_gate_clear(flag, 0u, s), "gate kernel");
     NET_TRY(launch_noise_gate(f, net->lo, net->hi, flag, s), "gate kernel");
-    NET_TRY(launch_pack_frame(f, net->ix0.as<uint16_t>(), s), "pack kernel");
+    /* The check first, on the network run over the sampled tiles' windows
+     * (the training convolution, the same arithmetic as the whole-frame
+     * one), so that a frame that falls back never runs the whole network. */
+    if (k) {
+        const int sy = std::min(hh, (int)WIN), sx = std::min(ww, (int)WIN);
+        const size_t wcells = (size_t)k * sy * sx;
+        GROW(net->wx0, 2 * wcells * C);
+        GROW(net->wa, 2 * wcells * C);
+        GROW(net->wb, 2 * wcells * C);
+        GROW(net->wout, 4 * wcells * C);
+        NET_TRY(launch_pack_windows(f, d_tiles, (int)k, sy, sx, net->wx0.as<uint16_t>(), s),
+                "pack kernel");
+        uint16_t *a = net->wa.as<uint16_t>(), *b = net->wb.as<uint16_t>();
+        NET_TRY(launch_conv(EPI_RELU, net->wx0.as<uint16_t>(), (int)k, sy, sx, W(0), B(0), nullptr,
+                            a, nullptr, nullptr, s),
+                "conv kernel");
+        for (int l = 1; l < 7; l++) {
+            NET_TRY(launch_conv(EPI_RESID_RELU, a, (int)k, sy, sx, W(l), B(l), a, b, nullptr,
+                                nullptr, s),
+                    "conv kernel");
+            std::swap(a, b);
+        }
+        NET_TRY(launch_conv(EPI_LINEAR_F32, a, (int)k, sy, sx, W(7), B(7), nullptr, nullptr,
+                            nullptr, net->wout.as<float>(), s),
+                "conv kernel");
+        DevelopDesc dc = d;
+        dc.has_matrix = 0;
+        NET_TRY(launch_check(dc, f, fin, net->wout.as<float>(), sy, sx, d_tiles, k,
+                             net->scratch.as<float>(), min_db, net->psnr.as<float>(), flag, s),
+                "check kernels");
+    }
+    /* The whole frame through the network when it did not fall back. */
+    NET_TRY(launch_pack_frame(f, net->ix0.as<uint16_t>(), flag, s), "pack kernel");
     uint16_t *a = net->ia.as<uint16_t>(), *b = net->ib.as<uint16_t>();
-    NET_TRY(launch_conv(EPI_RELU, net->ix0.as<uint16_t>(), 1, hh, ww, W(0), B(0), nullptr, a,
-                        nullptr, nullptr, s),
+    const uint32_t g = net->conv_grid;
+    NET_TRY(launch_conv_infer(IM_HEAD, net->ix0.as<uint16_t>(), hh, ww, W(0), B(0), a, nullptr,
+                              flag, g, s),
             "conv kernel");
     for (int l = 1; l < 7; l++) {
-        NET_TRY(launch_conv(EPI_RESID_RELU, a, 1, hh, ww, W(l), B(l), a, b, nullptr, nullptr, s),
+        NET_TRY(launch_conv_infer(IM_RES, a, hh, ww, W(l), B(l), b, nullptr, flag, g, s),
                 "conv kernel");
         std::swap(a, b);
     }
-    NET_TRY(launch_conv(EPI_LINEAR_F32, a, 1, hh, ww, W(7), B(7), nullptr, nullptr, nullptr,
-                        net->iout.as<float>(), s),
+    NET_TRY(launch_conv_infer(IM_TAIL, a, hh, ww, W(7), B(7), nullptr, net->iout.as<float>(), flag,
+                              g, s),
             "conv kernel");
-    NET_TRY(launch_compose(f, fin, net->iout.as<float>(), d_r, d_g, d_b, s), "compose kernel");
-    DevelopDesc dc = d;
-    dc.has_matrix = 0;
-    NET_TRY(launch_check(dc, f, fin, net->iout.as<float>(), d_tiles, k, net->scratch.as<float>(),
-                         min_db, net->psnr.as<float>(), flag, s),
-            "check kernels");
+    NET_TRY(launch_compose(f, fin, net->iout.as<float>(), d_r, d_g, d_b, flag, s),
+            "compose kernel");
     d.gate = flag;
     NET_TRY(launch_develop(d, d_r, d_g, d_b, s), "develop kernel");
     net->last_k = k;

@@ -167,6 +167,25 @@ def develop_pair(torch, L, frame, stage, net, **kw):
     return got, plain
 
 
+def sample_tiles(w, h, k):
+    """The sampled 64x64 tiles of the learned develop's check: k tiles
+    spread over the frame's tile grid (prr_net_host.cpp sample_tiles)."""
+    nbx, nby = (w + 63) // 64, (h + 63) // 64
+    out = []
+    for i in range(k):
+        j = (i * 5 + 2) % k
+        out.append((min(nbx - 1, (2 * i + 1) * nbx // (2 * k)),
+                    min(nby - 1, (2 * j + 1) * nby // (2 * k))))
+    return out
+
+
+def tile_psnr(got, plain, w, h, bx, by):
+    g = got.reshape(3, h, w)[:, 64 * by:64 * by + 64, 64 * bx:64 * bx + 64].astype(np.float64)
+    p = plain.reshape(3, h, w)[:, 64 * by:64 * by + 64, 64 * bx:64 * bx + 64].astype(np.float64)
+    mse = float(np.mean((g - p) ** 2))
+    return 200.0 if mse == 0 else -10.0 * np.log10(mse)
+
+
 def test_fallbacks_give_plain_develop_exactly(env):
     torch, L, frames = env
     from proresraw_rs_amd.api import Stage
@@ -234,3 +253,39 @@ def test_trained_network_passes_the_floor_and_beats_bilinear(env):
         assert p_net >= p_base + 4.0
         assert used == (worst < 45.0)
         assert np.array_equal(checked, plain if used else got)
+        # The check runs the network on windows around the sampled tiles
+        # only; its PSNRs are those of the whole-frame output on the tiles.
+        want = min(tile_psnr(got, plain, 5760, 3240, bx, by)
+                   for bx, by in sample_tiles(5760, 3240, 8))
+        assert abs(worst - want) < 1e-3, (worst, want)
+
+
+def test_inference_convolution_equals_the_training_forward(env):
+    """The inference convolution (persistent, weights held in registers,
+    12 output channels) does the training forward's arithmetic: on a frame
+    that is exactly one training crop, the learned develop at
+    Stage::Demosaiced equals, bit for bit, min(out + bilinear, 1) built
+    from the training forward's output of the same crop."""
+    torch, L, _ = env
+    from proresraw_rs_amd import synth as S
+    from proresraw_rs_amd.api import Stage
+    cells = 72
+    sc = S.scene(2 * cells, 2 * cells, 0, seed=4242, version=0, alignment=2, qscale=(30, 30),
+                 white_level=4095)
+    f = L.frame_pair(S.encode_scene(sc))
+    net = L.Net(L.init_params(11))
+    net.set_range(0.0, 100.0)
+    net.train_setup([f], np.zeros((1, 1, 3), np.uint32), cells, 0)
+    net.train_step(0.0)
+    torch.cuda.synchronize()
+    out = net.read(L.OUT)[0]
+    got, _ = develop_pair(torch, L, f, Stage.Demosaiced, net, check_tiles=0)
+    assert not net.last_check()[0]
+    win, _, base = crop_host(f, 0, 0, cells)
+    full = np.zeros_like(base)
+    for dy in range(2):
+        for dx in range(2):
+            for c in range(3):
+                full[c, dy::2, dx::2] = out[:, :, 3 * (2 * dy + dx) + c]
+    want = np.minimum((full + base) * np.float32(1.0), np.float32(1.0)).astype(np.float32)
+    np.testing.assert_array_equal(got, want.reshape(3, -1))
