@@ -127,12 +127,13 @@ def main():
             raise P.Error(err)
 
     def develop():
-        # develop(&frame, Rec2020): the noise model from the frame's
-        # max_qscale, read on the device.
-        rc = lib.prr_develop_frame_device(ctx.handle, ctypes.byref(raw_hdr), d_samples.data_ptr(),
-                                          d_status.data_ptr(), 0, 0.0, 0.0, 2,
-                                          planes[0].data_ptr(), planes[1].data_ptr(),
-                                          planes[2].data_ptr(), hs, ctypes.byref(err))
+        # develop(&frame, Rec2020) from the linearized picture the decode
+        # wrote (the same planes as from the code values): the noise model
+        # from the frame's max_qscale, read on the device.
+        rc = lib.prr_develop_linear_device(ctx.handle, ctypes.byref(raw_hdr), d_linear.data_ptr(),
+                                           d_status.data_ptr(), 0, 0.0, 0.0, 2,
+                                           planes[0].data_ptr(), planes[1].data_ptr(),
+                                           planes[2].data_ptr(), hs, ctypes.byref(err))
         if rc:
             raise P.Error(err)
 
@@ -186,7 +187,7 @@ def main():
     ms_per_step = 1e3 * elapsed / args.steps
     value = npix * args.steps / elapsed / 1e9
 
-    # Develop roofline: algorithmic bytes = code values read (2 B per visible
+    # Develop roofline: algorithmic bytes = linear values read (2 B per visible
     # sensel) + three f32 planes written (12 B per visible sensel).
     dev_bytes = 14.0 * npix
     achieved = dev_bytes / (t_develop_ms * 1e-3) / 1e9

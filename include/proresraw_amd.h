@@ -210,6 +210,16 @@ int prr_develop_frame_device(prr_context *ctx, const prr_header *hdr, const uint
                              const prr_device_status *d_status, int explicit_wb, float wb_red,
                              float wb_blue, int stage, float *d_r, float *d_g, float *d_b,
                              void *stream, prr_error *err);
+/* prr_develop_frame_device reading the linearized picture instead of the
+ * code values: d_linear is Frame::linearize's output (width*height values,
+ * e.g. the d_linear of prr_decode_frame_device), so the curve is not
+ * applied a second time. Same planes, bit for bit, as develop(&frame) on
+ * the code values those came from (demosaic.rs:190-215; Conversion::new
+ * linearizes each code value first, demosaic.rs:293-298). */
+int prr_develop_linear_device(prr_context *ctx, const prr_header *hdr, const uint16_t *d_linear,
+                              const prr_device_status *d_status, int explicit_wb, float wb_red,
+                              float wb_blue, int stage, float *d_r, float *d_g, float *d_b,
+                              void *stream, prr_error *err);
 
 /* ---- batches (no counterpart in the crate, which decodes one frame per
  * call): many frames of any size, version, alignment and qscale decoded

@@ -521,6 +521,30 @@ extern "C" int prr_develop_frame_device(prr_context *ctx, const prr_header *hdr,
     return PRR_OK;
 }
 
+extern "C" int prr_develop_linear_device(prr_context *ctx, const prr_header *hdr,
+                                         const uint16_t *d_linear,
+                                         const prr_device_status *d_status, int explicit_wb,
+                                         float wb_red, float wb_blue, int stage, float *d_r,
+                                         float *d_g, float *d_b, void *stream, prr_error *err)
+{
+    if (!ctx || !hdr || !d_linear || !d_status || !d_r)
+        return fail(err, PRR_E_INVALID_ARGUMENT, "null argument");
+    if ((d_g == nullptr) != (d_b == nullptr))
+        return fail(err, PRR_E_INVALID_ARGUMENT, "planes");
+    if (stage < PRR_STAGE_DEMOSAICED || stage > PRR_STAGE_REC2020)
+        return fail(err, PRR_E_INVALID_ARGUMENT, "stage", (uint64_t)stage);
+    if (((uintptr_t)d_r | (uintptr_t)d_g | (uintptr_t)d_b) & 7)
+        return fail(err, PRR_E_INVALID_ARGUMENT, "plane alignment");
+    hipStream_t s = (hipStream_t)stream;
+    HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+    DevelopDesc d = develop_desc_device(hdr, d_linear, d_status, explicit_wb, wb_red, wb_blue,
+                                        stage);
+    d.stride = hdr->width;
+    d.linear_input = 1;
+    HIP_TRY(launch_develop(d, d_r, d_g, d_b, s), "develop kernel");
+    return PRR_OK;
+}
+
 extern "C" int prr_develop_device(prr_context *ctx, const prr_header *hdr,
                                   const uint16_t *d_samples, const prr_demosaic_params *params,
                                   int stage, float *d_r, float *d_g, float *d_b, void *stream,

@@ -53,6 +53,9 @@ struct DevelopDesc {
     /* When set, the develop kernel runs only if *gate is non-zero (the
      * learned path's fallback decided on the device). */
     const uint32_t *gate;
+    /* When set, `samples` holds linearized values (Frame::linearize) and the
+     * curve is not applied again. */
+    uint32_t linear_input;
 };
 
 enum BatchMode : uint32_t {

@@ -1275,10 +1275,11 @@ __device__ __forceinline__ DevK dev_consts(const DevelopDesc &d)
 }
 
 /* Normalized value of a code value (Conversion::new folded table,
- * demosaic.rs:293-298). */
-__device__ __forceinline__ float norm_of(const DevelopDesc &d, const uint32_t *curve, uint32_t code)
+ * demosaic.rs:293-298), or of a linearized value when the input is the
+ * linearized picture. */
+__device__ __forceinline__ float norm_of(const DevelopDesc &d, const uint32_t *curve, uint32_t v)
 {
-    const int32_t lin = (int32_t)linearize(curve, code);
+    const int32_t lin = d.linear_input ? (int32_t)v : (int32_t)linearize(curve, v);
     return d.one_over_range * (float)(lin - d.black);
 }
 

@@ -136,3 +136,63 @@ def test_frames_beyond_the_register_scan(gpu):
     st.synchronize()
     assert slot.skipped_tiles() == sk and slot.status_host()[1] == mq
     np.testing.assert_array_equal(slot.samples.cpu().numpy().view(np.uint16), s)
+
+
+def device_decode_develop(pk, stage, wb=None, linear_input=False):
+    """decode (with the fused linearize) then develop on the device through
+    the C ABI; develop reads the code values or, with linear_input, the
+    linearized picture."""
+    import ctypes
+    import torch
+    from proresraw_rs_amd import _native as N
+    lib, ctx = N.lib(), P.context()
+    hdr, info = P.parse_frame_info(pk)
+    dev = torch.device("cuda")
+    padded = ((len(pk) + 3) & ~3) + 16
+    packet = torch.zeros(padded, dtype=torch.uint8, device=dev)
+    packet[:len(pk)] = torch.frombuffer(bytearray(pk), dtype=torch.uint8).to(dev)
+    samples = torch.zeros(info.coded_width * info.coded_height, dtype=torch.int16, device=dev)
+    n = hdr.width * hdr.height
+    linear = torch.zeros(n, dtype=torch.int16, device=dev)
+    flags = torch.zeros(max(info.nb_tiles, 1), dtype=torch.uint8, device=dev)
+    status = torch.zeros(4, dtype=torch.int32, device=dev)
+    planes = [torch.full((n,), -1.0, dtype=torch.float32, device=dev) for _ in range(3)]
+    s = torch.cuda.Stream()
+    hs = ctypes.c_void_p(s.cuda_stream)
+    err = N.PrrError()
+    assert lib.prr_decode_frame_device(ctx.handle, ctypes.byref(hdr._raw), ctypes.byref(info),
+                                       packet.data_ptr(), len(pk), 0, samples.data_ptr(),
+                                       linear.data_ptr(), flags.data_ptr(), status.data_ptr(), hs,
+                                       ctypes.byref(err)) == 0
+    fn = lib.prr_develop_linear_device if linear_input else lib.prr_develop_frame_device
+    src = linear if linear_input else samples
+    assert fn(ctx.handle, ctypes.byref(hdr._raw), src.data_ptr(), status.data_ptr(),
+              1 if wb else 0, ctypes.c_float(wb[0] if wb else 0.0),
+              ctypes.c_float(wb[1] if wb else 0.0), int(stage), planes[0].data_ptr(),
+              planes[1].data_ptr(), planes[2].data_ptr(), hs, ctypes.byref(err)) == 0
+    s.synchronize()
+    return [p.cpu().numpy() for p in planes]
+
+
+@pytest.mark.parametrize("size", [(4096, 2160), (4104, 2168)])
+def test_develop_from_the_linearized_picture(gpu, size):
+    """prr_develop_linear_device on the decode's linear output equals
+    prr_develop_frame_device on the code values, bit for bit."""
+    w, hh = size
+    pk = S.encode_scene(config1(w, hh))
+    for stage, wb in ((P.Stage.Rec2020, None), (P.Stage.WhiteBalanced, (2.0, 1.6))):
+        say(f"{w}x{hh} {stage.name} wb={wb}")
+        a = device_decode_develop(pk, stage, wb)
+        b = device_decode_develop(pk, stage, wb, linear_input=True)
+        for p, q in zip(a, b):
+            np.testing.assert_array_equal(p.view(np.uint32), q.view(np.uint32))
+    rng = S.Rng(9)
+    for i, spec in enumerate(S.specs(rng)):
+        spec.valid_tiles = 1
+        rpk = S.random_frame(spec, rng)
+        if i % 4:
+            continue
+        a = device_decode_develop(rpk, P.Stage.Demosaiced)
+        b = device_decode_develop(rpk, P.Stage.Demosaiced, linear_input=True)
+        for p, q in zip(a, b):
+            np.testing.assert_array_equal(p.view(np.uint32), q.view(np.uint32))
