@@ -316,8 +316,8 @@ extern "C" void prr_context_destroy(prr_context *ctx)
         return;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
-    for (DevBuf *b : {&ctx->packet, &ctx->offsets, &ctx->status, &ctx->flags, &ctx->samples,
-                      &ctx->out0, &ctx->out1, &ctx->out2})
+    for (DevBuf *b : {&ctx->packet, &ctx->offsets, &ctx->queue, &ctx->status, &ctx->flags,
+                      &ctx->samples, &ctx->out0, &ctx->out1, &ctx->out2})
         if (b->ptr)
             (void)hipFree(b->ptr);
     (void)hipStreamDestroy(ctx->stream);
