@@ -530,9 +530,21 @@ struct Group {
     uint8_t *qmat;       /* 64 entries */
     TileInfo *tiles;     /* up to 16 */
     int16_t *coefs;
-    uint16_t *stage;     /* 4096 entries */
     uint32_t n, T;       /* tiles in the group, capacity */
 };
+
+/* The transform stage of the chunk whose first slot is c0: an LDS copy of
+ * the chunk's tiles, placed over the chunk's own coefficient blocks. Each
+ * lane has read its block's coefficients into registers before the wave
+ * writes any stage value (a wave's LDS operations run in order), and the
+ * stage a chunk needs (16 rows of its tiles) never exceeds the blocks of
+ * those tiles: 4 streams x stride >= 4 x (per_block << alignment) values a
+ * tile, which is the tile's stage size. */
+__device__ inline uint16_t *chunk_stage(const Group &g, uint32_t c0, uint32_t amax,
+                                        uint32_t stride)
+{
+    return (uint16_t *)(g.coefs + (c0 >> amax) * stride);
+}
 
 /* The tiles of one group: list[0 .. count) when list is set (tiles in the
  * order a queue chose), else t0 .. t0 + count - 1. */
@@ -547,7 +559,7 @@ __device__ inline uint32_t tile_at(const TileSet &ts, uint32_t i)
 }
 
 /* Carve the dynamic LDS. Layout: [table 544 B][curve 36 B][qmat 64 B at
- * 592][tiles 16 x 20 B at 656][coefs][stage 8 KB]. */
+ * 592][tiles 16 x 20 B at 656][coefs]. */
 __device__ inline Group carve(uint32_t *smem, uint32_t spw, uint32_t stride)
 {
     Group g;
@@ -557,15 +569,13 @@ __device__ inline Group carve(uint32_t *smem, uint32_t spw, uint32_t stride)
     g.qmat = base + LDS_QMAT;
     g.tiles = (TileInfo *)(base + LDS_HEAD);
     g.coefs = (int16_t *)(base + LDS_HEAD + 16 * sizeof(TileInfo));
-    size_t coef_bytes = ((size_t)spw * stride * 2 + 15) & ~(size_t)15;
-    g.stage = (uint16_t *)((uint8_t *)g.coefs + coef_bytes);
     return g;
 }
 
 __host__ inline size_t group_lds_bytes(uint32_t spw, uint32_t stride)
 {
     size_t coef_bytes = ((size_t)spw * stride * 2 + 15) & ~(size_t)15;
-    return LDS_HEAD + 16 * sizeof(TileInfo) + coef_bytes + 8192;
+    return LDS_HEAD + 16 * sizeof(TileInfo) + coef_bytes;
 }
 
 /* Everything up to and including the entropy decode. Each lane `l` below
@@ -899,6 +909,7 @@ __device__ inline void decode_group_full(const FrameDesc &f, const TileSet &ts, 
         const uint32_t ls = slot >> amax, b = slot & (blocks_max - 1);
         const uint32_t tl = ls >> 2, s = ls & 3;
         const uint32_t tc = tl - (c0 >> amax) / 4;
+        uint16_t *const stage = chunk_stage(g, c0, amax, stride);
         if (tl < g.n) {
             TileInfo ti = g.tiles[tl];
             if (ti.valid == 1 && b < (1u << ti.log2)) {
@@ -914,7 +925,7 @@ __device__ inline void decode_group_full(const FrameDesc &f, const TileSet &ts, 
                     apple_8x8(d, v);
                 const uint32_t pr = (s == 0 || s == 2) ? 1 - rr : rr;
                 const uint32_t pc = (s == 1 || s == 2) ? 1 - rc : rc;
-                uint16_t *dst = g.stage + tc * 16 * W + pr * W + 16 * b + pc;
+                uint16_t *dst = stage + tc * 16 * W + pr * W + 16 * b + pc;
 #pragma unroll
                 for (int i = 0; i < 8; i++)
 #pragma unroll
@@ -941,7 +952,7 @@ __device__ inline void decode_group_full(const FrameDesc &f, const TileSet &ts, 
                 continue;
             uint32_t pair = 0;
             if (ti.valid == 1)
-                pair = *(const uint32_t *)(g.stage + tcc * 16 * W + row * W + 2 * dcol);
+                pair = *(const uint32_t *)(stage + tcc * 16 * W + row * W + 2 * dcol);
             const uint32_t y = ti.y + row, x = ti.x + 2 * dcol;
             *(uint32_t *)(samples + (size_t)y * f.coded_width + x) = pair;
             if (linear && y < f.height && x < f.width) {
@@ -982,6 +993,7 @@ __device__ inline void decode_group_reduced(const FrameDesc &f, const TileSet &t
         const uint32_t ls = slot >> amax, b = slot & (blocks_max - 1);
         const uint32_t tl = ls >> 2, s = ls & 3;
         const uint32_t tc = tl - (c0 >> amax) / 4;
+        uint16_t *const stage = chunk_stage(g, c0, amax, stride);
         if (tl < g.n && ((want >> s) & 1u)) {
             TileInfo ti = g.tiles[tl];
             if (ti.valid == 1 && b < (1u << ti.log2)) {
@@ -1008,7 +1020,7 @@ __device__ inline void decode_group_reduced(const FrameDesc &f, const TileSet &t
                         d[k] = dequantize(src[kInv2.v[k] << amax], (int32_t)g.qmat[qi[k]] * (int32_t)ti.qscale);
                     apple_2x2(d, v);
                 }
-                uint16_t *dst = g.stage + ((tc * 4 + s) * side) * W + b * side;
+                uint16_t *dst = stage + ((tc * 4 + s) * side) * W + b * side;
 #pragma unroll
                 for (uint32_t i = 0; i < side; i++)
 #pragma unroll
@@ -1035,7 +1047,7 @@ __device__ inline void decode_group_reduced(const FrameDesc &f, const TileSet &t
                 uint32_t vals[2];
                 for (int h = 0; h < 2; h++) {
                     uint32_t e = 2 * dcol + h, px = e / 3, ch = e - px * 3;
-                    const uint16_t *base = g.stage + (tcc * 4 * side + row) * W + px;
+                    const uint16_t *base = stage + (tcc * 4 * side + row) * W + px;
                     uint32_t v;
                     if (ch == 0) {
                         v = base[3 * side * W];
