@@ -127,8 +127,8 @@ void *prr_context_stream(prr_context *ctx);
 /* Version string of this library. */
 const char *prr_version(void);
 /* Decode launch shape (no counterpart in the crate, which has no launch to
- * shape): lanes decoding streams per workgroup (16, 32 or 64); 0 keeps the
- * current value. */
+ * shape): lanes decoding streams per workgroup (8, 16, 32 or 64; 8 by
+ * default for single frames); 0 keeps the current value. */
 int prr_set_decode_config(prr_context *ctx, uint32_t streams_per_wg, prr_error *err);
 
 /* ---- host-side frame structure (no device work) ----------------------- */
@@ -253,9 +253,11 @@ int prr_batch_create(prr_context *ctx, prr_batch **out, prr_error *err);
 /* Add a frame (at most 256 per batch); `hdr`/`info` from prr_parse_frame_info. */
 int prr_batch_add(prr_batch *batch, const prr_header *hdr, const prr_frame_info *info,
                   const prr_batch_frame *frame, prr_error *err);
-/* Launch shape: streams per workgroup (16/32/64; frames already added are
- * laid out again), queue entries taken per atomic grab, waves per CU of the
- * persistent launch (0 = as many as fit); 0 keeps the current spw / chunk.
+/* Launch shape: streams per workgroup (16/32/64, 64 by default; a frame's
+ * is halved until its group fits the batch's 32 KiB LDS budget; frames
+ * already added are laid out again), queue entries taken per atomic grab,
+ * waves per CU of the persistent launch (0 = as many as fit); 0 keeps the
+ * current spw / chunk.
  * Takes effect at the next prr_batch_run, which then uploads the frame
  * table again (capture a graph only after that run). */
 int prr_batch_set_config(prr_batch *batch, uint32_t streams_per_wg, uint32_t chunk,
