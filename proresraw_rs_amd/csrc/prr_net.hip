@@ -188,11 +188,9 @@ __global__ __launch_bounds__(256) void k_conv(const uint16_t *__restrict__ in, i
 /* Inference convolution: the arithmetic of k_conv (the same MFMAs in the
  * same order, so the same bits), in a persistent workgroup that keeps the
  * layer's weights in registers (A operands of all 9 taps, loaded once),
- * walks the frame's 4 x 64 tiles in a grid stride, and loads the next
- * tile's input into registers while the current one is on the matrix
- * cores. Vertically adjacent tiles are a multiple of 8 tiles apart, so
- * with workgroups spread round robin over the 8 L2s a tile's halo rows are
- * usually its neighbours' rows in the same L2.
+ * walks a contiguous run of the frame's 4 x 64 tiles down its columns, and
+ * loads the next tile's input into registers while the current one is on
+ * the matrix cores.
  *   IM_HEAD  h = relu(conv(x0) + b)
  *   IM_RES   h' = h + relu(conv(h) + b) with h read from the staged tile
  *   IM_TAIL  out[px][0..11] = conv(h) + b in fp32, the 12 channels the
@@ -215,8 +213,13 @@ __global__ __launch_bounds__(256) void k_conv_infer(const uint16_t *__restrict__
     constexpr int MT = MODE == IM_TAIL ? 1 : 2;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int r16 = lane & 15, kq = (lane >> 4) * 8;
-    const int tiles_x = (W + CT_COLS - 1) / CT_COLS;
-    const int n_tiles = tiles_x * ((H + CT_ROWS - 1) / CT_ROWS);
+    const int tiles_x = (W + CT_COLS - 1) / CT_COLS, tiles_y = (H + CT_ROWS - 1) / CT_ROWS;
+    const int n_tiles = tiles_x * tiles_y;
+    /* Tiles in column-major order, a contiguous run per workgroup: the next
+     * tile is usually the one below, whose top halo rows this tile just
+     * loaded (cache hits rather than HBM reads). */
+    const int per = (n_tiles + (int)gridDim.x - 1) / (int)gridDim.x;
+    const int first = (int)blockIdx.x * per, last = min(first + per, n_tiles);
 
     bf16x8 wa[TAPS][MT];
 #pragma unroll
@@ -233,7 +236,7 @@ __global__ __launch_bounds__(256) void k_conv_infer(const uint16_t *__restrict__
 
     uint4 pre[IT_PER];
     auto fetch = [&](int tile) {
-        const int ty = tile / tiles_x, tx = tile - ty * tiles_x;
+        const int tx = tile / tiles_y, ty = tile - tx * tiles_y;
         const int x0 = tx * CT_COLS, y0 = ty * CT_ROWS;
 #pragma unroll
         for (int k = 0; k < IT_PER; k++) {
@@ -247,10 +250,9 @@ __global__ __launch_bounds__(256) void k_conv_infer(const uint16_t *__restrict__
             pre[k] = v;
         }
     };
-    int tile = blockIdx.x;
-    if (tile < n_tiles)
-        fetch(tile);
-    for (; tile < n_tiles; tile += gridDim.x) {
+    if (first < last)
+        fetch(first);
+    for (int tile = first; tile < last; tile++) {
         __syncthreads();
 #pragma unroll
         for (int k = 0; k < IT_PER; k++) {
@@ -262,10 +264,10 @@ __global__ __launch_bounds__(256) void k_conv_infer(const uint16_t *__restrict__
             }
         }
         __syncthreads();
-        const int ty = tile / tiles_x, tx = tile - ty * tiles_x;
+        const int tx = tile / tiles_y, ty = tile - tx * tiles_y;
         const int x0 = tx * CT_COLS, y0 = ty * CT_ROWS;
-        if (tile + (int)gridDim.x < n_tiles)
-            fetch(tile + gridDim.x);
+        if (tile + 1 < last)
+            fetch(tile + 1);
 
         f32x4 acc[MT][4];
 #pragma unroll

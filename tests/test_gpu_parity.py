@@ -196,3 +196,30 @@ def test_develop_from_the_linearized_picture(gpu, size):
         b = device_decode_develop(rpk, P.Stage.Demosaiced, linear_input=True)
         for p, q in zip(a, b):
             np.testing.assert_array_equal(p.view(np.uint32), q.view(np.uint32))
+
+
+@pytest.mark.parametrize("spw", [8, 16, 32, 64])
+def test_every_decode_launch_shape_is_bit_exact(gpu, spw):
+    """The single-frame decode gives the oracle's code values, linear values
+    and skipped tiles at every streams-per-workgroup setting, on the
+    4104x2168 partial-tile frame (full decode, both transforms) and on
+    reduced decodes."""
+    import ctypes
+    from proresraw_rs_amd import _native as N
+    lib, ctx = N.lib(), P.context()
+    err = N.PrrError()
+    assert lib.prr_set_decode_config(ctx.handle, spw, ctypes.byref(err)) == 0
+    try:
+        pk = S.encode_scene(config1(4104, 2168))
+        for idct in (P.Idct.Apple, P.Idct.Ffmpeg):
+            f = P.decode_frame_with(pk, idct)
+            h, s, sk, mq = O.decode_frame(pk, ffmpeg=idct == P.Idct.Ffmpeg)
+            assert f.skipped_tiles == sk and f.max_qscale == mq
+            np.testing.assert_array_equal(f.samples, s)
+        for shift in (1, 2, 3):
+            r = P.decode_frame_reduced_with(pk, P.Scale(shift), P.Greens.Average)
+            h, rgb, sk, mq = O.decode_frame_reduced(pk, shift, True)
+            assert r.skipped_tiles == sk
+            np.testing.assert_array_equal(r.samples, rgb)
+    finally:
+        assert lib.prr_set_decode_config(ctx.handle, 8, ctypes.byref(err)) == 0
