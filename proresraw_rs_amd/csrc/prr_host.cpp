@@ -783,7 +783,8 @@ struct prr_batch {
     std::vector<BatchDevelop> jobs;
     std::vector<size_t> offset_base; /* per frame, in u64 entries */
     size_t offsets_total = 0;
-    DevBuf d_frames, d_jobs, d_queue, d_head, d_offsets;
+    std::vector<uint32_t> chunks; /* (frame << 16) | chunk of the size-table scan */
+    DevBuf d_frames, d_jobs, d_queue, d_head, d_hist, d_offsets, d_chunks, d_chunk_bytes;
     uint32_t n_items = 0;
     uint32_t max_w = 0, max_h = 0;
     uint32_t spw = 64, chunk = 1, waves_per_cu = 0;
@@ -814,7 +815,8 @@ extern "C" void prr_batch_destroy(prr_batch *b)
         return;
     (void)hipSetDevice(b->ctx->device);
     (void)hipStreamSynchronize(b->ctx->stream);
-    for (DevBuf *d : {&b->d_frames, &b->d_jobs, &b->d_queue, &b->d_head, &b->d_offsets})
+    for (DevBuf *d : {&b->d_frames, &b->d_jobs, &b->d_queue, &b->d_head, &b->d_hist, &b->d_offsets,
+                      &b->d_chunks, &b->d_chunk_bytes})
         if (d->ptr)
             (void)hipFree(d->ptr);
     delete b;
@@ -846,7 +848,6 @@ static void relayout(prr_batch *b)
         bf.spw = frame_spw(b->spw, sh.per_block, sh.alignment,
                            std::min(b->ctx->launch.lds_limit, b->lds_budget), lds);
         const uint32_t T = bf.spw / 4;
-        bf.first_item = b->n_items;
         b->n_items += (sh.nb_tiles + T - 1) / T;
         b->lds = std::max(b->lds, lds);
     }
@@ -887,16 +888,23 @@ extern "C" int prr_batch_add(prr_batch *b, const prr_header *hdr, const prr_fram
     bf.status = fr->d_status;
     if ((info->nb_tiles + 3) / 4 >= (1u << 20))
         return fail(err, PRR_E_INVALID_ARGUMENT, "too many tile groups in one frame");
+    /* Every check comes before the batch changes: a failed add leaves it as
+     * it was. */
+    if (fr->kind == PRR_BATCH_DEVELOP) {
+        if (!fr->d_r || (fr->d_g == nullptr) != (fr->d_b == nullptr))
+            return fail(err, PRR_E_INVALID_ARGUMENT, "planes");
+        if (fr->stage < PRR_STAGE_DEMOSAICED || fr->stage > PRR_STAGE_REC2020)
+            return fail(err, PRR_E_INVALID_ARGUMENT, "stage");
+    }
+    const uint32_t n_chunks = std::max<uint32_t>((info->nb_tiles + SCAN_CHUNK - 1) / SCAN_CHUNK, 1);
+    for (uint32_t c = 0; c < n_chunks; c++)
+        b->chunks.push_back(((uint32_t)b->frames.size() << 16) | c);
     b->offset_base.push_back(b->offsets_total);
     b->offsets_total += (size_t)info->nb_tiles + 1;
     b->frames.push_back(bf);
     b->shapes.push_back({per_block, hdr->alignment, info->nb_tiles});
     relayout(b);
     if (fr->kind == PRR_BATCH_DEVELOP) {
-        if (!fr->d_r || (fr->d_g == nullptr) != (fr->d_b == nullptr))
-            return fail(err, PRR_E_INVALID_ARGUMENT, "planes");
-        if (fr->stage < PRR_STAGE_DEMOSAICED || fr->stage > PRR_STAGE_REC2020)
-            return fail(err, PRR_E_INVALID_ARGUMENT, "stage");
         BatchDevelop j;
         j.d = develop_desc_device(hdr, fr->d_samples, fr->d_status, fr->explicit_wb, fr->wb_red,
                                   fr->wb_blue, fr->stage);
@@ -946,12 +954,17 @@ extern "C" int prr_batch_run(prr_batch *b, void *stream, prr_error *err)
             (rc = ensure(ctx, b->d_frames, sizeof(BatchFrame) * b->frames.size(), err)) ||
             (rc = ensure(ctx, b->d_jobs, sizeof(BatchDevelop) * std::max<size_t>(b->jobs.size(), 1), err)) ||
             (rc = ensure(ctx, b->d_queue, sizeof(uint32_t) * std::max<uint32_t>(b->n_items, 1), err)) ||
-            (rc = ensure(ctx, b->d_head, 64, err)))
+            (rc = ensure(ctx, b->d_head, 64, err)) ||
+            (rc = ensure(ctx, b->d_hist, sizeof(uint32_t) * 64 * b->frames.size(), err)) ||
+            (rc = ensure(ctx, b->d_chunks, sizeof(uint32_t) * b->chunks.size(), err)) ||
+            (rc = ensure(ctx, b->d_chunk_bytes, sizeof(uint32_t) * b->chunks.size(), err)))
             return rc;
         for (size_t i = 0; i < b->frames.size(); i++)
             b->frames[i].f.offsets = (uint64_t *)b->d_offsets.ptr + b->offset_base[i];
         HIP_TRY(hipMemcpyAsync(b->d_frames.ptr, b->frames.data(), sizeof(BatchFrame) * b->frames.size(),
                                hipMemcpyHostToDevice, s), "hipMemcpy frames");
+        HIP_TRY(hipMemcpyAsync(b->d_chunks.ptr, b->chunks.data(), sizeof(uint32_t) * b->chunks.size(),
+                               hipMemcpyHostToDevice, s), "hipMemcpy chunks");
         if (!b->jobs.empty())
             HIP_TRY(hipMemcpyAsync(b->d_jobs.ptr, b->jobs.data(), sizeof(BatchDevelop) * b->jobs.size(),
                                    hipMemcpyHostToDevice, s), "hipMemcpy jobs");
@@ -972,8 +985,10 @@ extern "C" int prr_batch_run(prr_batch *b, void *stream, prr_error *err)
     }
     HIP_TRY(launch_batch((const BatchFrame *)b->d_frames.ptr, (uint32_t)b->frames.size(), b->n_items,
                          (uint32_t *)b->d_queue.ptr, (uint32_t *)b->d_head.ptr,
-                         (const BatchDevelop *)b->d_jobs.ptr, (uint32_t)b->jobs.size(), b->max_w,
-                         b->max_h, b->grid, b->chunk, b->lds, s),
+                         (uint32_t *)b->d_hist.ptr, (const uint32_t *)b->d_chunks.ptr,
+                         (uint32_t)b->chunks.size(), (uint32_t *)b->d_chunk_bytes.ptr,
+                         (const BatchDevelop *)b->d_jobs.ptr,
+                         (uint32_t)b->jobs.size(), b->max_w, b->max_h, b->grid, b->chunk, b->lds, s),
             "batch kernels");
     return PRR_OK;
 }

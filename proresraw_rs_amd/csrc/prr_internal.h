@@ -66,13 +66,16 @@ enum BatchMode : uint32_t {
     BATCH_EIGHTH = 4,
 };
 
+/* Tiles of one workgroup of the batch's size-table scan (a multiple of
+ * every group size, 1 to 16 tiles). */
+enum { SCAN_CHUNK = 2048 };
+
 /* One frame of a batch as the persistent kernel sees it. */
 struct BatchFrame {
     FrameDesc f;
     uint32_t mode;
     uint32_t average;
     uint32_t spw;
-    uint32_t first_item;     /* index of its first tile group in the batch */
     uint16_t *samples;       /* full decode */
     uint16_t *linear;        /* full decode, optional */
     uint16_t *rgb;           /* reduced decode */
@@ -105,8 +108,9 @@ hipError_t launch_linearize(const uint16_t *samples, uint32_t stride, uint32_t w
                             hipStream_t s);
 hipError_t launch_develop(const DevelopDesc &d, float *r, float *g, float *b, hipStream_t s);
 hipError_t launch_batch(const BatchFrame *d_frames, uint32_t n_frames, uint32_t n_items,
-                        uint32_t *d_queue, uint32_t *d_head, const BatchDevelop *d_jobs,
-                        uint32_t n_jobs, uint32_t max_w, uint32_t max_h, uint32_t grid,
+                        uint32_t *d_queue, uint32_t *d_head, uint32_t *d_hist,
+                        const uint32_t *d_chunks, uint32_t n_chunks, uint32_t *d_chunk_bytes,
+                        const BatchDevelop *d_jobs, uint32_t n_jobs, uint32_t max_w, uint32_t max_h, uint32_t grid,
                         uint32_t chunk, size_t lds, hipStream_t s);
 size_t decode_lds_bytes(uint32_t spw, uint32_t per_block, uint32_t amax);
 

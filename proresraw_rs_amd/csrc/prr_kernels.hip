@@ -684,13 +684,13 @@ __device__ inline bool front_end(const FrameDesc &f, Group &g, const TileSet &ts
 
 /* ------------------------------------------------------------------ */
 
-/* Exclusive prefix sum of one value per lane over a 1024-lane workgroup:
- * within each wave with shuffles, then across the 16 waves through `sums`
- * (32 entries of LDS). Returns the lane's exclusive prefix; `total` gets
- * the sum of all. */
+/* Exclusive prefix sum of one value per lane over a workgroup of up to
+ * 1024 lanes: within each wave with shuffles, then across the waves
+ * through `sums` (32 entries of LDS). Returns the lane's exclusive prefix;
+ * `total` gets the sum of all. */
 __device__ inline uint64_t block_exclusive(uint64_t v, uint64_t *sums, uint64_t &total)
 {
-    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6, waves = blockDim.x >> 6;
     uint64_t incl = v;
 #pragma unroll
     for (uint32_t d = 1; d < 64; d <<= 1) {
@@ -702,7 +702,7 @@ __device__ inline uint64_t block_exclusive(uint64_t v, uint64_t *sums, uint64_t 
         sums[wave] = incl;
     __syncthreads();
     if (wave == 0) {
-        uint64_t w = lane < 16 ? sums[lane] : 0;
+        uint64_t w = lane < waves ? sums[lane] : 0;
 #pragma unroll
         for (uint32_t d = 1; d < 16; d <<= 1) {
             const uint64_t u = __shfl_up(w, d);
@@ -718,90 +718,69 @@ __device__ inline uint64_t block_exclusive(uint64_t v, uint64_t *sums, uint64_t 
 }
 
 /* Size-table walk of parse_frame (decoder.rs:104-116) by one 1024-lane
- * workgroup, each lane over a run of `per` consecutive tiles: the tile
- * offsets (a prefix sum of the size table), the first tile that runs past
- * the packet, and the largest qscale byte. */
-__device__ inline void scan_frame(const FrameDesc &f, prr_device_status *status, uint64_t *sums)
-{
-    const uint32_t n = f.nb_tiles, t = threadIdx.x;
-    const uint32_t per = (n + 1023) / 1024;
-    const uint32_t lo = min(t * per, n), hi = min(lo + per, n);
-    const uint8_t *tab = f.packet + f.table_off;
-    uint64_t sum = 0;
-    for (uint32_t i = lo; i < hi; i++)
-        sum += be16_at(tab + 2 * (size_t)i);
-    uint64_t total;
-    uint64_t off = (uint64_t)f.table_off + 2ull * n + block_exclusive(sum, sums, total);
-    uint32_t oob = 0xFFFFFFFFu, mq = 0;
-    for (uint32_t i = lo; i < hi; i++) {
-        const uint32_t size = be16_at(tab + 2 * (size_t)i);
-        const uint32_t q = off + 1 < f.len ? f.packet[off + 1] : 0u;
-        mq = q > mq ? q : mq;
-        if (off + size > f.len && oob == 0xFFFFFFFFu)
-            oob = i;
-        f.offsets[i] = off;
-        off += size;
-    }
-    if (t == 1023)
-        f.offsets[n] = (uint64_t)f.table_off + 2ull * n + total;
-    if (oob != 0xFFFFFFFFu)
-        atomicMin(&status->oob_tile, oob);
-    if (mq)
-        atomicMax(&status->max_qscale, mq);
-}
-
-/* scan_frame for frames of at most 1024 * PREP_PER tiles, with a lane's
- * sizes kept in registers so the size table is read once and every
- * qscale byte is requested before any is waited for; also adds each
- * tile's size to gbytes[tile / T], its group's coded bytes, when gbytes is
- * not NULL. */
+ * workgroup: the tile offsets (a prefix sum of the size table), the first
+ * tile that runs past the packet, and the largest qscale byte. The frame is
+ * walked in passes of 1024 * PER tiles, each lane over PER consecutive tiles
+ * of a pass with their sizes in registers, so the size table is read once
+ * and every qscale byte of a pass is requested before any is waited for.
+ * When gbytes is not NULL (frames of one pass), each tile's size is also
+ * added to gbytes[tile / T], its group's coded bytes. `f` is a copy in
+ * registers: the offset stores cannot alias it. */
 enum { PREP_PER = 8, PREP_SMALL = 1024 * PREP_PER };
-__device__ inline void scan_frame_small(const FrameDesc &f, prr_device_status *status,
-                                        uint64_t *sums, uint32_t *gbytes, uint32_t T)
+template <uint32_t PER>
+__device__ inline void scan_frame(const FrameDesc f, prr_device_status *status, uint64_t *sums,
+                                  uint32_t *gbytes, uint32_t T)
 {
+    constexpr uint32_t PASS = 1024 * PER;
     const uint32_t n = f.nb_tiles, t = threadIdx.x;
-    const uint32_t per = (n + 1023) / 1024;
-    const uint32_t lo = min(t * per, n), hi = min(lo + per, n);
     const uint8_t *tab = f.packet + f.table_off;
-    uint32_t sz[PREP_PER];
-    uint64_t sum = 0;
-#pragma unroll
-    for (uint32_t k = 0; k < PREP_PER; k++) {
-        sz[k] = lo + k < hi ? be16_at(tab + 2 * (size_t)(lo + k)) : 0u;
-        sum += sz[k];
-    }
-    uint64_t total;
-    const uint64_t first = (uint64_t)f.table_off + 2ull * n + block_exclusive(sum, sums, total);
-    /* Every qscale load is issued (at a clamped, in-packet address) before
-     * any is used. */
-    uint64_t offs[PREP_PER];
-    uint32_t q[PREP_PER];
-    {
-        uint64_t off = first;
-#pragma unroll
-        for (uint32_t k = 0; k < PREP_PER; k++) {
-            offs[k] = off;
-            off += sz[k];
-            q[k] = f.packet[min(offs[k] + 1, f.len - 1)];
-        }
-    }
+    uint64_t base = (uint64_t)f.table_off + 2ull * n;
     uint32_t oob = 0xFFFFFFFFu, mq = 0;
+    for (uint32_t p0 = 0; p0 < n; p0 += PASS) {
+        const uint32_t m = min(n - p0, PASS);
+        const uint32_t per = (m + 1023) / 1024;
+        const uint32_t lo = p0 + min(t * per, m), hi = p0 + min(t * per + per, m);
+        uint32_t sz[PER];
+        uint64_t sum = 0;
 #pragma unroll
-    for (uint32_t k = 0; k < PREP_PER; k++) {
-        const uint32_t i = lo + k;
-        if (i < hi) {
-            const uint64_t off = offs[k];
-            const uint32_t qk = off + 1 < f.len ? q[k] : 0u;
-            mq = qk > mq ? qk : mq;
-            if (off + sz[k] > f.len && oob == 0xFFFFFFFFu)
-                oob = i;
-            f.offsets[i] = off;
-            if (gbytes)
-                atomicAdd(&gbytes[i / T], sz[k]);
+        for (uint32_t k = 0; k < PER; k++) {
+            sz[k] = lo + k < hi ? be16_at(tab + 2 * (size_t)(lo + k)) : 0u;
+            sum += sz[k];
+        }
+        /* block_exclusive's reads of sums end before any lane reaches its
+         * next call's first barrier, so passes need no barrier of their own. */
+        uint64_t total;
+        const uint64_t first = base + block_exclusive(sum, sums, total);
+        base += total;
+        /* A lane's tiles start at first + rel, rel < PER * 65536. */
+        uint32_t q[PER];
+        {
+            uint32_t rel = 0;
+#pragma unroll
+            for (uint32_t k = 0; k < PER; k++) {
+                q[k] = f.packet[min(first + rel + 1, f.len - 1)];
+                rel += sz[k];
+            }
+        }
+        uint32_t rel = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < PER; k++) {
+            const uint32_t i = lo + k;
+            const uint64_t off = first + rel;
+            rel += sz[k];
+            if (i < hi) {
+                const uint32_t qk = off + 1 < f.len ? q[k] : 0u;
+                mq = qk > mq ? qk : mq;
+                if (off + sz[k] > f.len && oob == 0xFFFFFFFFu)
+                    oob = i;
+                f.offsets[i] = off;
+                if (gbytes)
+                    atomicAdd(&gbytes[i / T], sz[k]);
+            }
         }
     }
-    if (t == 1023)
-        f.offsets[n] = (uint64_t)f.table_off + 2ull * n + total;
+    if (t == 0)
+        f.offsets[n] = base;
     /* One atomic per wave. */
 #pragma unroll
     for (uint32_t d = 32; d > 0; d >>= 1) {
@@ -816,6 +795,31 @@ __device__ inline void scan_frame_small(const FrameDesc &f, prr_device_status *s
     }
 }
 
+/* The longest-first order's bucket of a tile group of `bytes` coded bytes:
+ * 64 buckets on a half-octave log scale, the longest in bucket 0. */
+enum { QUEUE_BUCKETS = 64 };
+__device__ inline uint32_t bytes_bucket(uint64_t bytes)
+{
+    const float l2 = bytes ? __log2f((float)bytes) : 0.0f;
+    int b = (int)(2.0f * l2);
+    b = b < 0 ? 0 : b > QUEUE_BUCKETS - 1 ? QUEUE_BUCKETS - 1 : b;
+    return (uint32_t)(QUEUE_BUCKETS - 1 - b);
+}
+
+/* Exclusive prefix sum over the 64 lanes of a wave. */
+__device__ inline uint32_t wave_exclusive(uint32_t v)
+{
+    const uint32_t lane = threadIdx.x & 63;
+    uint32_t incl = v;
+#pragma unroll
+    for (uint32_t d = 1; d < 64; d <<= 1) {
+        const uint32_t u = __shfl_up(incl, d);
+        if (lane >= d)
+            incl += u;
+    }
+    return incl - v;
+}
+
 /* Tile groups ordered by coded length, longest first, into queue[0 ..
  * groups): a bucket sort on a half-octave log scale of each group's bytes
  * (`bytes_of(g)`), by one 1024-lane workgroup. Waves that start first get
@@ -825,25 +829,14 @@ template <class Bytes>
 __device__ inline void order_groups(uint32_t groups, uint32_t *queue, uint32_t *hist,
                                     uint32_t *base, Bytes bytes_of)
 {
-    constexpr uint32_t NB = 64;
+    constexpr uint32_t NB = QUEUE_BUCKETS;
     const uint32_t tid = threadIdx.x;
-    auto bucket_of = [&](uint32_t g) {
-        const uint64_t bytes = bytes_of(g);
-        const float l2 = bytes ? __log2f((float)bytes) : 0.0f;
-        int b = (int)(2.0f * l2);
-        b = b < 0 ? 0 : b > (int)NB - 1 ? (int)NB - 1 : b;
-        return (uint32_t)(NB - 1 - b);
-    };
+    auto bucket_of = [&](uint32_t g) { return bytes_bucket(bytes_of(g)); };
     for (uint32_t g = tid; g < groups; g += 1024)
         atomicAdd(&hist[bucket_of(g)], 1u);
     __syncthreads();
-    if (tid == 0) {
-        uint32_t run = 0;
-        for (uint32_t b = 0; b < NB; b++) {
-            base[b] = run;
-            run += hist[b];
-        }
-    }
+    if (tid < NB)
+        base[tid] = wave_exclusive(hist[tid]);
     __syncthreads();
     for (uint32_t g = tid; g < groups; g += 1024)
         queue[atomicAdd(&base[bucket_of(g)], 1u)] = g;
@@ -870,12 +863,12 @@ __global__ __launch_bounds__(1024) void k_prepare(FrameDesc f, prr_device_status
         for (uint32_t g = tid; g < groups; g += 1024)
             gbytes[g] = 0;
         __syncthreads();
-        scan_frame_small(f, status, sums, gbytes, T);
+        scan_frame<PREP_PER>(f, status, sums, gbytes, T);
         __syncthreads();
         order_groups(groups, queue, hist, base, [&](uint32_t g) { return (uint64_t)gbytes[g]; });
     } else {
         __syncthreads();
-        scan_frame(f, status, sums);
+        scan_frame<PREP_PER>(f, status, sums, nullptr, T);
         __syncthreads();
         order_groups(groups, queue, hist, base, [&](uint32_t g) {
             const uint32_t t0 = g * T, t1 = min(t0 + T, n);
@@ -1572,81 +1565,189 @@ __device__ inline T load_uniform(const T *p)
 /* ------------------------------------------------------------------ */
 /* Batched decode: one persistent launch over tile groups of many frames. */
 
-/* Scan every frame of the batch (one workgroup per frame) and reset its
- * status; the last workgroup also resets the queue head. */
-__global__ __launch_bounds__(1024) void k_batch_scan(const BatchFrame *frames, uint32_t *head)
+/* Bucket of tile group g (tiles g*T .. g*T+T-1) of a frame, from its
+ * scanned offsets. */
+__device__ inline uint32_t group_bucket(const FrameDesc &f, uint32_t T, uint32_t g)
 {
-    __shared__ uint64_t sums[32];
-    const BatchFrame &bf = frames[blockIdx.x];
-    if (threadIdx.x == 0) {
-        bf.status->oob_tile = 0xFFFFFFFFu;
-        bf.status->max_qscale = 0;
-        bf.status->n_skipped = 0;
-        bf.status->overrun = 0;
-        if (blockIdx.x == 0)
-            *head = 0;
-    }
-    __syncthreads();
-    if (bf.f.nb_tiles <= PREP_SMALL)
-        scan_frame_small(bf.f, bf.status, sums, nullptr, 1);
-    else
-        scan_frame(bf.f, bf.status, sums);
+    const uint32_t t0 = g * T, t1 = min(t0 + T, f.nb_tiles);
+    return bytes_bucket(f.offsets[t1] - f.offsets[t0]);
 }
 
-/* Order the tile groups of the batch by coded length, longest first: a
- * bucket sort on a half-octave log scale of each group's byte count. The
- * group list is item = (frame << 20) | group. */
-__global__ __launch_bounds__(1024) void k_build_queue(const BatchFrame *frames, uint32_t n_frames,
-                                                      uint32_t n_items, uint32_t *queue)
+/* The batch's size-table scans run in chunks of SCAN_CHUNK tiles, so that a
+ * large frame's scan is spread over many workgroups (one CU alone cannot
+ * keep enough of a frame's scattered qscale reads in flight: a 73,000-tile
+ * frame took 0.27 ms in one workgroup). chunks[c] = (frame << 16) | chunk
+ * index in the frame; a frame's chunks are consecutive entries. */
+enum { SCAN_LANES = 256, SCAN_PER = SCAN_CHUNK / SCAN_LANES };
+
+/* Pass 1: the coded bytes of each chunk's tiles. Chunk 0 of a frame also
+ * resets the frame's status and group histogram, entry 0 the queue head. */
+__global__ __launch_bounds__(SCAN_LANES) void k_batch_chunk_bytes(const BatchFrame *frames,
+                                                                  const uint32_t *chunks,
+                                                                  uint32_t *chunk_bytes,
+                                                                  uint32_t *hist, uint32_t *head)
 {
-    constexpr uint32_t NB = 64;
-    __shared__ uint32_t hist[NB], base[NB];
-    __shared__ uint32_t first_item[256];
-    const uint32_t tid = threadIdx.x;
-    if (tid < NB)
-        hist[tid] = 0;
-    for (uint32_t i = tid; i < n_frames && i < 256; i += 1024)
-        first_item[i] = frames[i].first_item;
-    __syncthreads();
-    auto bucket_of = [&](uint32_t item, uint32_t &packed) {
-        uint32_t lo = 0, hi = n_frames - 1;
-        while (lo < hi) {
-            uint32_t mid = (lo + hi + 1) / 2;
-            if (first_item[mid] <= item)
-                lo = mid;
-            else
-                hi = mid - 1;
+    __shared__ uint32_t part[SCAN_LANES / 64];
+    const uint32_t e = chunks[blockIdx.x], fr = e >> 16, c = e & 0xFFFFu, tid = threadIdx.x;
+    const BatchFrame bf = load_uniform(&frames[fr]);
+    if (c == 0) {
+        if (tid == 0) {
+            bf.status->oob_tile = 0xFFFFFFFFu;
+            bf.status->max_qscale = 0;
+            bf.status->n_skipped = 0;
+            bf.status->overrun = 0;
+            if (blockIdx.x == 0)
+                *head = 0;
         }
-        const BatchFrame &bf = frames[lo];
-        uint32_t g = item - bf.first_item;
-        uint32_t T = bf.spw >> 2;
-        uint32_t t0 = g * T, t1 = min(t0 + T, bf.f.nb_tiles);
-        uint64_t bytes = bf.f.offsets[t1] - bf.f.offsets[t0];
-        packed = (lo << 20) | g;
-        float l2 = bytes ? __log2f((float)bytes) : 0.0f;
-        int b = (int)(2.0f * l2);
-        b = b < 0 ? 0 : b > (int)NB - 1 ? (int)NB - 1 : b;
-        return (uint32_t)(NB - 1 - b);
-    };
-    for (uint32_t i = tid; i < n_items; i += 1024) {
-        uint32_t packed;
-        atomicAdd(&hist[bucket_of(i, packed)], 1u);
+        if (tid < QUEUE_BUCKETS)
+            hist[fr * QUEUE_BUCKETS + tid] = 0;
     }
+    const uint32_t n = bf.f.nb_tiles;
+    const uint8_t *tab = bf.f.packet + bf.f.table_off;
+    uint32_t sum = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < SCAN_PER; k++) {
+        const uint32_t i = c * SCAN_CHUNK + k * SCAN_LANES + tid;
+        sum += i < n ? be16_at(tab + 2 * (size_t)i) : 0u;
+    }
+#pragma unroll
+    for (uint32_t d = 32; d > 0; d >>= 1)
+        sum += (uint32_t)__shfl_xor((int)sum, (int)d);
+    if ((tid & 63) == 0)
+        part[tid >> 6] = sum;
     __syncthreads();
     if (tid == 0) {
-        uint32_t run = 0;
-        for (uint32_t b = 0; b < NB; b++) {
-            base[b] = run;
-            run += hist[b];
+        uint32_t all = 0;
+#pragma unroll
+        for (uint32_t w = 0; w < SCAN_LANES / 64; w++)
+            all += part[w];
+        chunk_bytes[blockIdx.x] = all;
+    }
+}
+
+/* Pass 2: the walk of parse_frame (decoder.rs:104-116) over one chunk,
+ * starting after the bytes of the frame's earlier chunks: tile offsets,
+ * first tile past the packet, largest qscale byte; then the chunk's tile
+ * groups (T tiles each; T divides SCAN_CHUNK, so no group spans two
+ * chunks) counted per length bucket into the frame's histogram. */
+__global__ __launch_bounds__(SCAN_LANES) void k_batch_scan(const BatchFrame *frames,
+                                                           const uint32_t *chunks,
+                                                           const uint32_t *chunk_bytes,
+                                                           uint32_t *hist)
+{
+    __shared__ uint64_t sums[32];
+    __shared__ uint32_t gbytes[SCAN_CHUNK];
+    __shared__ uint32_t count[QUEUE_BUCKETS];
+    const uint32_t e = chunks[blockIdx.x], fr = e >> 16, c = e & 0xFFFFu, tid = threadIdx.x;
+    const BatchFrame bf = load_uniform(&frames[fr]);
+    const FrameDesc &f = bf.f;
+    const uint32_t n = f.nb_tiles, T = bf.spw >> 2;
+    const uint32_t c0 = c * SCAN_CHUNK, m = c0 < n ? min(n - c0, (uint32_t)SCAN_CHUNK) : 0u;
+    for (uint32_t g = tid; g < SCAN_CHUNK; g += SCAN_LANES)
+        gbytes[g] = 0;
+    if (tid < QUEUE_BUCKETS)
+        count[tid] = 0;
+    uint64_t before = 0;
+    for (uint32_t j = tid; j < c; j += SCAN_LANES)
+        before += chunk_bytes[blockIdx.x - c + j];
+    uint64_t earlier;
+    (void)block_exclusive(before, sums, earlier);
+    const uint64_t base = (uint64_t)f.table_off + 2ull * n + earlier;
+
+    const uint32_t per = (m + SCAN_LANES - 1) / SCAN_LANES;
+    const uint32_t lo = c0 + min(tid * per, m), hi = c0 + min(tid * per + per, m);
+    const uint8_t *tab = f.packet + f.table_off;
+    uint32_t sz[SCAN_PER];
+    uint32_t sum = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < SCAN_PER; k++) {
+        sz[k] = lo + k < hi ? be16_at(tab + 2 * (size_t)(lo + k)) : 0u;
+        sum += sz[k];
+    }
+    /* The first call's reads of sums end before any lane reaches this
+     * call's first barrier. */
+    uint64_t total;
+    const uint64_t first = base + block_exclusive(sum, sums, total);
+    uint32_t q[SCAN_PER];
+    {
+        uint32_t rel = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < SCAN_PER; k++) {
+            q[k] = f.packet[min(first + rel + 1, f.len - 1)];
+            rel += sz[k];
         }
     }
-    __syncthreads();
-    for (uint32_t i = tid; i < n_items; i += 1024) {
-        uint32_t packed;
-        uint32_t b = bucket_of(i, packed);
-        uint32_t at = atomicAdd(&base[b], 1u);
-        queue[at] = packed;
+    uint32_t oob = 0xFFFFFFFFu, mq = 0, rel = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < SCAN_PER; k++) {
+        const uint32_t i = lo + k;
+        const uint64_t off = first + rel;
+        rel += sz[k];
+        if (i < hi) {
+            const uint32_t qk = off + 1 < f.len ? q[k] : 0u;
+            mq = qk > mq ? qk : mq;
+            if (off + sz[k] > f.len && oob == 0xFFFFFFFFu)
+                oob = i;
+            f.offsets[i] = off;
+            atomicAdd(&gbytes[(i - c0) / T], sz[k]);
+        }
     }
+    if (tid == 0 && c0 + SCAN_CHUNK >= n)
+        f.offsets[n] = base + total;
+#pragma unroll
+    for (uint32_t d = 32; d > 0; d >>= 1) {
+        oob = min(oob, (uint32_t)__shfl_xor((int)oob, (int)d));
+        mq = max(mq, (uint32_t)__shfl_xor((int)mq, (int)d));
+    }
+    if ((tid & 63) == 0) {
+        if (oob != 0xFFFFFFFFu)
+            atomicMin(&bf.status->oob_tile, oob);
+        if (mq)
+            atomicMax(&bf.status->max_qscale, mq);
+    }
+    __syncthreads();
+    const uint32_t groups = (m + T - 1) / T;
+    for (uint32_t g = tid; g < groups; g += SCAN_LANES)
+        atomicAdd(&count[bytes_bucket(gbytes[g])], 1u);
+    __syncthreads();
+    if (tid < QUEUE_BUCKETS && count[tid])
+        atomicAdd(&hist[fr * QUEUE_BUCKETS + tid], count[tid]);
+}
+
+/* Place the batch's tile groups in the queue, longest first: the groups of
+ * bucket b of frame f start after every group of the buckets before b and
+ * the bucket-b groups of the frames before f. One workgroup per frame;
+ * entries are (frame << 20) | group. */
+__global__ __launch_bounds__(1024) void k_build_queue(const BatchFrame *frames, uint32_t n_frames,
+                                                      const uint32_t *hist, uint32_t *queue)
+{
+    __shared__ uint32_t part_all[16][QUEUE_BUCKETS], part_before[16][QUEUE_BUCKETS];
+    __shared__ uint32_t base[QUEUE_BUCKETS];
+    const uint32_t tid = threadIdx.x, bucket = tid & (QUEUE_BUCKETS - 1), part = tid >> 6;
+    const uint32_t fr = blockIdx.x;
+    uint32_t all = 0, before = 0;
+    for (uint32_t i = part; i < n_frames; i += 16) {
+        const uint32_t v = hist[i * QUEUE_BUCKETS + bucket];
+        all += v;
+        before += i < fr ? v : 0u;
+    }
+    part_all[part][bucket] = all;
+    part_before[part][bucket] = before;
+    __syncthreads();
+    if (tid < QUEUE_BUCKETS) {
+        uint32_t in_bucket = 0, ahead = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < 16; k++) {
+            in_bucket += part_all[k][tid];
+            ahead += part_before[k][tid];
+        }
+        base[tid] = wave_exclusive(in_bucket) + ahead;
+    }
+    __syncthreads();
+    const BatchFrame bf = load_uniform(&frames[fr]);
+    const uint32_t T = bf.spw >> 2, groups = (bf.f.nb_tiles + T - 1) / T;
+    for (uint32_t g = tid; g < groups; g += 1024)
+        queue[atomicAdd(&base[group_bucket(bf.f, T, g)], 1u)] = (fr << 20) | g;
 }
 
 /* The persistent decode: every wave pulls `chunk` queue entries at a time
@@ -1807,16 +1908,21 @@ hipError_t launch_develop(const DevelopDesc &d, float *r, float *g, float *b, hi
 }
 
 hipError_t launch_batch(const BatchFrame *d_frames, uint32_t n_frames, uint32_t n_items,
-                        uint32_t *d_queue, uint32_t *d_head, const BatchDevelop *d_jobs,
+                        uint32_t *d_queue, uint32_t *d_head, uint32_t *d_hist,
+                        const uint32_t *d_chunks, uint32_t n_chunks, uint32_t *d_chunk_bytes,
+                        const BatchDevelop *d_jobs,
                         uint32_t n_jobs, uint32_t max_w, uint32_t max_h, uint32_t grid,
                         uint32_t chunk, size_t lds, hipStream_t s)
 {
     if (n_frames == 0)
         return hipSuccess;
-    hipLaunchKernelGGL(k_batch_scan, dim3(n_frames), dim3(1024), 0, s, d_frames, d_head);
+    hipLaunchKernelGGL(k_batch_chunk_bytes, dim3(n_chunks), dim3(SCAN_LANES), 0, s, d_frames,
+                       d_chunks, d_chunk_bytes, d_hist, d_head);
+    hipLaunchKernelGGL(k_batch_scan, dim3(n_chunks), dim3(SCAN_LANES), 0, s, d_frames, d_chunks,
+                       (const uint32_t *)d_chunk_bytes, d_hist);
     if (n_items) {
-        hipLaunchKernelGGL(k_build_queue, dim3(1), dim3(1024), 0, s, d_frames, n_frames, n_items,
-                           d_queue);
+        hipLaunchKernelGGL(k_build_queue, dim3(n_frames), dim3(1024), 0, s, d_frames, n_frames,
+                           (const uint32_t *)d_hist, d_queue);
         hipLaunchKernelGGL(k_batch_decode, dim3(grid), dim3(64), lds, s, d_frames, d_queue,
                            n_items, d_head, chunk);
     }

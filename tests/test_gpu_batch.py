@@ -120,3 +120,22 @@ def test_batch_graph_replays_match_the_oracle():
         torch.cuda.synchronize()
         say(f"replay {rep}")
         verify(plan)
+
+
+def test_a_refused_frame_leaves_the_batch_as_it_was():
+    """prr_batch_add checks every argument before it changes the batch: a
+    develop frame with a stage out of range is refused, and the batch still
+    holds and decodes exactly the frames it had."""
+    import torch
+    packets = mixed_packets()[:6]
+    b, plan = build(packets)
+    for name, pk, kind, a, c, slot in plan:
+        slot.wb = None
+    before = b.info()["tile_groups"]
+    with pytest.raises(P.Error):
+        b.add_full(packets[0][1], P.Idct.Apple, develop=7)
+    assert b.info()["tile_groups"] == before and len(b.slots) == len(plan)
+    s = torch.cuda.Stream()
+    b.run(s)
+    s.synchronize()
+    verify(plan)

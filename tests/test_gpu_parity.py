@@ -111,8 +111,9 @@ def test_develop_random_frames_all_stages(gpu):
 
 
 def test_frames_beyond_the_register_scan(gpu):
-    """More than 8192 tiles: the size-table scan takes its general path
-    (one frame decode, one reduced decode, one batched frame)."""
+    """More than 8192 tiles: the single-frame size-table scan takes more
+    than one pass and the batch scan several chunks (frame decode, reduced
+    decode, a batch of three frames, a tile past the packet)."""
     from proresraw_rs_amd.batch import Batch
     import torch
     sc = S.scene(2048, 1088, 1, seed=21, version=1, alignment=0, qscale=(15, 45),
@@ -129,13 +130,34 @@ def test_frames_beyond_the_register_scan(gpu):
     h, rgb, sk2, mq2 = O.decode_frame_reduced(pk, 1, True)
     assert r.skipped_tiles == sk2 and r.max_qscale == mq2
     np.testing.assert_array_equal(r.samples, rgb)
+    # A tile that runs past the packet in the scan's second pass of 8192
+    # tiles (and the batch scan's fifth chunk of 2048): TileOutOfBounds
+    # names it, on both paths.
+    k = 8500
+    bad = bytearray(pk)
+    bad[info.header_end + 2 * k:info.header_end + 2 * k + 2] = b"\xff\xff"
+    bad = bytes(bad)
+    with pytest.raises(O.OracleError) as oe:
+        O.decode_frame(bad)
+    with pytest.raises(P.Error) as ge:
+        P.decode_frame(bad)
+    assert oe.value.code == 11 and oe.value.a == k
+    assert ge.value.kind == "TileOutOfBounds" and ge.value.tile == k
+    # One batch: the frame decoded whole, reduced at Half, and with the bad
+    # tile, which affects no other frame.
     b = Batch()
     slot = b.add_full(pk, P.Idct.Apple)
+    slot_r = b.add_reduced(pk, P.Scale.Half, P.Greens.Average)
+    slot_bad = b.add_full(bad, P.Idct.Apple)
     st = torch.cuda.Stream()
     b.run(st)
     st.synchronize()
     assert slot.skipped_tiles() == sk and slot.status_host()[1] == mq
     np.testing.assert_array_equal(slot.samples.cpu().numpy().view(np.uint16), s)
+    assert slot_r.skipped_tiles() == sk2 and slot_r.status_host()[1] == mq2
+    np.testing.assert_array_equal(slot_r.rgb.cpu().numpy().view(np.uint16), rgb)
+    assert slot.status_host()[0] == slot_r.status_host()[0] == 0xFFFFFFFF
+    assert slot_bad.status_host()[0] == k
 
 
 def device_decode_develop(pk, stage, wb=None, linear_input=False):
