@@ -1230,13 +1230,20 @@ __device__ inline float estimate_pre(const float *xx, const float *gg, const flo
 
 enum { DTW = 64, DTH = 32, NORM_W = DTW + 6, NORM_H = DTH + 6, GRN_W = DTW + 4, GRN_H = DTH + 2 };
 
-/* LDS of one develop tile. Each green-pass cell holds its red/blue value,
- * its green, and the two noise deviations every chroma estimate that reads
- * the cell needs: sqrt(variance(rb)) and sqrt(variance(g)) at the chroma
- * offset, computed once per cell instead of once per use. */
+/* LDS of one develop tile: normalized values, and beside each the noise
+ * deviation it has as a green-pass neighbour (sqrt(variance) at the green
+ * pass offset of the value clipped at 1 for a green sensel, at its row's
+ * clip for a red or blue one), computed once per sensel instead of once per
+ * use. The green-pass cells of a row alternate red/blue and green, so each
+ * kind has its own array indexed by column / 2: a red/blue cell holds its
+ * value, its green and the two deviations the chroma estimates reading it
+ * need (sqrt(variance(rb)) and sqrt(variance(g)) at the chroma offset); a
+ * green cell holds its value and sqrt(variance(g)). */
 struct DevelopLds {
     float norm[NORM_H][NORM_W];
-    float4 green[GRN_H][GRN_W];
+    float dev[NORM_H][NORM_W];
+    float4 rb[GRN_H][GRN_W / 2];
+    float2 gr[GRN_H][GRN_W / 2];
     uint32_t curve[9];
 };
 
@@ -1289,19 +1296,24 @@ __device__ __forceinline__ float norm_of(const DevelopDesc &d, const uint32_t *c
 }
 
 /* Green pass (green_row_impl, demosaic.rs:481-541) of one red/blue sensel
- * (gx, gy): `norm(y, x)` reads normalized values at picture coordinates
- * (rows and columns outside the picture hold clamped copies). Returns the
- * cell (rb, green, sqrt(variance(rb)), sqrt(variance(green))) at the
- * chroma offset. */
-template <class Norm>
+ * (gx, gy): `norm(y, x)` reads normalized values and `dev(y, x)` their
+ * deviations as neighbours at picture coordinates (rows and columns outside
+ * the picture hold clamped copies, which the mirroring replaces; it can
+ * only act when edge_x / edge_y say the tile reaches within two sensels of
+ * the picture's side). Returns the cell (rb, green, sqrt(variance(rb)),
+ * sqrt(variance(green))) at the chroma offset. */
+template <class Norm, class Dev>
 __device__ __forceinline__ float4 green_rb(const DevK &k, int32_t gx, int32_t gy, float clip,
-                                           Norm norm)
+                                           Norm norm, Dev dev, bool edge_x, bool edge_y)
 {
     const float n0 = k.n0, n1 = k.n1, n2 = k.n2, bs = k.bs;
     const float centre = norm(gy, gx);
     const float c = fminf(centre, clip);
+    const float sc = dev(gy, gx);
     float hh[4] = {norm(gy, gx - 1), norm(gy, gx + 1), norm(gy, gx - 2), norm(gy, gx + 2)};
     float vv[4] = {norm(gy - 1, gx), norm(gy + 1, gx), norm(gy - 2, gx), norm(gy + 2, gx)};
+    float sh[4] = {dev(gy, gx - 1), dev(gy, gx + 1), dev(gy, gx - 2), dev(gy, gx + 2)};
+    float sv[4] = {dev(gy - 1, gx), dev(gy + 1, gx), dev(gy - 2, gx), dev(gy + 2, gx)};
     hh[0] = fminf(hh[0], 1.0f);
     hh[1] = fminf(hh[1], 1.0f);
     hh[2] = fminf(hh[2], clip);
@@ -1310,28 +1322,32 @@ __device__ __forceinline__ float4 green_rb(const DevK &k, int32_t gx, int32_t gy
     vv[1] = fminf(vv[1], 1.0f);
     vv[2] = fminf(vv[2], clip);
     vv[3] = fminf(vv[3], clip);
-    mirror4(hh, (uint32_t)gx, k.w, c);
-    mirror4(vv, (uint32_t)gy, k.h, c);
-    float var_h[4], var_v[4];
+    if (edge_x) {
+        mirror4(hh, (uint32_t)gx, k.w, c);
+        mirror4(sh, (uint32_t)gx, k.w, sc);
+    }
+    if (edge_y) {
+        mirror4(vv, (uint32_t)gy, k.h, c);
+        mirror4(sv, (uint32_t)gy, k.h, sc);
+    }
+    float var_h[2], var_v[2];
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
+    for (int q = 0; q < 2; q++) {
         var_h[q] = variance(hh[q], bs, n0, n1, n2);
         var_v[q] = variance(vv[q], bs, n0, n1, n2);
     }
     const float sums0 = hh[0] + hh[1], sums1 = vv[0] + vv[1];
     const float sums2 = hh[2] + hh[3], sums3 = vv[2] + vv[3];
     float ratio0 = 0.0f, ratio1 = 0.0f;
-    if (c > sqrtf(variance(c, bs, n0, n1, n2))) {
+    if (c > sc) {
         const float t0 = c * 2.0f + sums2, t1 = c * 2.0f + sums3;
         const float t2 = c * -2.0f + sums2, t3 = c * -2.0f + sums3;
 #pragma unroll
         for (int dir = 0; dir < 2; dir++) {
             const float d0 = dir ? vv[0] : hh[0], d1 = dir ? vv[1] : hh[1];
             const float d2 = dir ? vv[2] : hh[2], d3 = dir ? vv[3] : hh[3];
-            const float sd0 = sqrtf(dir ? var_v[0] : var_h[0]);
-            const float sd1 = sqrtf(dir ? var_v[1] : var_h[1]);
-            const float sd2 = sqrtf(dir ? var_v[2] : var_h[2]);
-            const float sd3 = sqrtf(dir ? var_v[3] : var_h[3]);
+            const float sd0 = dir ? sv[0] : sh[0], sd1 = dir ? sv[1] : sh[1];
+            const float sd2 = dir ? sv[2] : sh[2], sd3 = dir ? sv[3] : sh[3];
             float a = d0 - d1, b = d2 - d3;
             bool any = a * b > 0.0f || (d2 - c) * (d3 - c) > 0.0f || sd0 + sd1 > fabsf(a) ||
                        sd2 + sd3 > fabsf(b);
@@ -1355,19 +1371,22 @@ __device__ __forceinline__ float4 green_rb(const DevK &k, int32_t gx, int32_t gy
                        sqrtf(variance(gout, k.coff, n0, n1, n2)));
 }
 
-/* The green-pass cell of a green sensel: its own value as green. */
-__device__ __forceinline__ float4 green_g(const DevK &k, float centre)
+/* The green-pass cell of a green sensel: its own value as green, and
+ * sqrt(variance(green)) at the chroma offset. */
+__device__ __forceinline__ float2 green_g(const DevK &k, float centre)
 {
-    return make_float4(0.0f, centre, 0.0f, sqrtf(variance(centre, k.coff, k.n0, k.n1, k.n2)));
+    return make_float2(centre, sqrtf(variance(centre, k.coff, k.n0, k.n1, k.n2)));
 }
 
 /* Chroma pass (chroma_row_with, demosaic.rs:648-688), white balance and
  * the output matrix (matrix_planes_impl, demosaic.rs:711-720) of the pair
- * of pixels (xb, y), (xb + 1, y), xb even: `green(y, x)` reads green-pass
- * cells at picture coordinates. res[i] is pixel xb + i. */
-template <class Green>
+ * of pixels (xb, y), (xb + 1, y), xb even: `green(y, x)` and `grc(y, x)`
+ * read the red/blue and the green green-pass cells at picture coordinates.
+ * res[i] is pixel xb + i. */
+template <class RbCell, class GCell>
 __device__ __forceinline__ void chroma_pair(const DevelopDesc &d, const DevK &k, uint32_t y,
-                                            uint32_t xb, Green green, float (&res)[2][3])
+                                            uint32_t xb, RbCell green, GCell grc,
+                                            float (&res)[2][3])
 {
     const uint32_t w = k.w, h = k.h;
     const float n0 = k.n0, n1 = k.n1, n2 = k.n2, off = k.coff;
@@ -1405,10 +1424,12 @@ __device__ __forceinline__ void chroma_pair(const DevelopDesc &d, const DevK &k,
         finish(on_red_row ? mid.x : est, mid.y, on_red_row ? est : mid.x, res_rb);
     }
     {
-        /* The green pixel: horizontal and vertical neighbour pairs. */
+        /* The green pixel: horizontal and vertical neighbour pairs (red or
+         * blue cells). */
         const uint32_t x = xb + (1 - rb_col);
         const uint32_t xm = x > 0 ? x - 1 : x + 1, xp = x < w - 1 ? x + 1 : x - 1;
-        const float4 mid = green(y, x);
+        const float2 gm = grc(y, x);
+        const float4 mid = make_float4(0.0f, gm.x, 0.0f, gm.y);
         const float4 l = green(y, xm), r = green(y, xp);
         const float4 u = green(ym, x), dn = green(yp, x);
         const float hx[4] = {l.x, r.x, l.x, r.x}, hg[4] = {l.y, r.y, l.y, r.y};
@@ -1469,13 +1490,21 @@ __device__ inline void develop_tile(const DevelopDesc &d, float *out_r, float *o
         lds.curve[tid] = tid < 8 ? d.curve[tid] : 65536u;
     __syncthreads();
 
-    /* Normalized values, rows and columns clamped to the picture. */
+    /* Normalized values, rows and columns clamped to the picture, and each
+     * one's deviation as a green-pass neighbour (outside the picture, by the
+     * Bayer phase of the position: those entries are never used unmirrored). */
+    const float clip_red = d.clip[0], clip_blue = d.clip[1];
     for (uint32_t i = tid; i < NORM_H * NORM_W; i += 256) {
-        int32_t ly = (int32_t)(i / NORM_W), lx = (int32_t)(i - (uint32_t)ly * NORM_W);
-        int32_t y = y0 - 3 + ly, x = x0 - 3 + lx;
-        y = y < 0 ? 0 : y > (int32_t)h - 1 ? (int32_t)h - 1 : y;
-        x = x < 0 ? 0 : x > (int32_t)w - 1 ? (int32_t)w - 1 : x;
-        lds.norm[ly][lx] = norm_of(d, lds.curve, d.samples[(size_t)y * d.stride + (uint32_t)x]);
+        const int32_t ly = (int32_t)(i / NORM_W), lx = (int32_t)(i - (uint32_t)ly * NORM_W);
+        const int32_t py = y0 - 3 + ly, px = x0 - 3 + lx;
+        const int32_t y = py < 0 ? 0 : py > (int32_t)h - 1 ? (int32_t)h - 1 : py;
+        const int32_t x = px < 0 ? 0 : px > (int32_t)w - 1 ? (int32_t)w - 1 : px;
+        const float v = norm_of(d, lds.curve, d.samples[(size_t)y * d.stride + (uint32_t)x]);
+        const bool red_row = ((uint32_t)py & 1u) == k.red_row;
+        const bool rb = ((uint32_t)px & 1u) == (red_row ? k.red_col : 1u - k.red_col);
+        const float cap = rb ? (red_row ? clip_red : clip_blue) : 1.0f;
+        lds.norm[ly][lx] = v;
+        lds.dev[ly][lx] = sqrtf(variance(fminf(v, cap), k.bs, k.n0, k.n1, k.n2));
     }
     __syncthreads();
 
@@ -1483,6 +1512,11 @@ __device__ inline void develop_tile(const DevelopDesc &d, float *out_r, float *o
      * does the green sensel, then the red/blue one, so every lane of a wave
      * runs the same code whatever the Bayer phase of its row. */
     auto norm = [&](int32_t y, int32_t x) { return lds.norm[y - (y0 - 3)][x - (x0 - 3)]; };
+    auto dev = [&](int32_t y, int32_t x) { return lds.dev[y - (y0 - 3)][x - (x0 - 3)]; };
+    /* Green-pass cells lie in [x0 - 1, x0 + DTW] x [y0 - 1, y0 + DTH]; the
+     * mirroring acts only within two sensels of the picture's sides. */
+    const bool edge_x = x0 - 1 <= 1 || x0 + DTW >= (int32_t)w - 2;
+    const bool edge_y = y0 - 1 <= 1 || y0 + DTH >= (int32_t)h - 2;
     for (uint32_t i = tid; i < GRN_H * (GRN_W / 2); i += 256) {
         const int32_t ly = (int32_t)(i / (GRN_W / 2));
         const int32_t lp = (int32_t)(i - (uint32_t)ly * (GRN_W / 2));
@@ -1498,19 +1532,22 @@ __device__ inline void develop_tile(const DevelopDesc &d, float *out_r, float *o
         {
             const int32_t gx = x0 - 2 + 2 * lp + (1 - first);
             if (in_range(gx))
-                lds.green[ly][gx - (x0 - 2)] = green_g(k, norm(gy, gx));
+                lds.gr[ly][lp] = green_g(k, norm(gy, gx));
         }
         const int32_t gx = x0 - 2 + 2 * lp + first;
         if (!in_range(gx))
             continue;
-        lds.green[ly][gx - (x0 - 2)] = green_rb(k, gx, gy, clip, norm);
+        lds.rb[ly][lp] = green_rb(k, gx, gy, clip, norm, dev, edge_x, edge_y);
     }
     __syncthreads();
 
     /* Chroma pass, white balance and output matrix: each lane takes the
      * red/blue pixel and the green pixel of one pair. */
-    auto green = [&](uint32_t y, uint32_t x) {
-        return lds.green[(int32_t)y - (y0 - 1)][(int32_t)x - (x0 - 2)];
+    auto rbc = [&](uint32_t y, uint32_t x) {
+        return lds.rb[(int32_t)y - (y0 - 1)][((int32_t)x - (x0 - 2)) >> 1];
+    };
+    auto grc = [&](uint32_t y, uint32_t x) {
+        return lds.gr[(int32_t)y - (y0 - 1)][((int32_t)x - (x0 - 2)) >> 1];
     };
     for (uint32_t i = tid; i < DTH * (DTW / 2); i += 256) {
         const uint32_t ly = i / (DTW / 2), lp = i - ly * (DTW / 2);
@@ -1519,7 +1556,7 @@ __device__ inline void develop_tile(const DevelopDesc &d, float *out_r, float *o
         if (y >= h || xb >= w)
             continue;
         float res[2][3];
-        chroma_pair(d, k, y, xb, green, res);
+        chroma_pair(d, k, y, xb, rbc, grc, res);
         store_pair(out_r, out_g, out_b, (size_t)(y - oy) * ostride + (xb - ox), xb, w, res);
     }
 }
