@@ -20,9 +20,11 @@
  *
  * Floating point follows the crate operation for operation: this file is
  * compiled with -ffp-contract=off, fmaf appears exactly where the crate
- * calls mul_add, and division and square root are the correctly rounded
- * defaults.
+ * calls mul_add, and division and square root are correctly rounded (the
+ * defaults, or div_in_range where its bounds are shown to hold).
  */
+#include <type_traits>
+
 #include "prr_internal.h"
 #include "prr_net.h"
 
@@ -1153,51 +1155,32 @@ __device__ inline void mirror4(float *d, uint32_t pos, uint32_t len, float c)
     d[3] = p0 ? d3 : pl ? d2 : pl2 ? c : d3;
 }
 
-/* estimate (demosaic.rs:572-604). */
-__device__ inline float estimate(const float *xx, const float *gg, float cg, float cs, float off,
-                                 float n0, float n1, float n2)
+/* n / d correctly rounded, as the general division sequence computes it
+ * (v_div_scale of both operands, v_rcp, five FMAs, v_div_fmas,
+ * v_div_fixup) without the three steps that only act outside these bounds:
+ * for 2^-103 <= n, 2^-43 <= d <= 2^12, n / d >= 2^-126 and n < 2^96 d,
+ * v_div_scale returns its operand unchanged with no scale flag, v_div_fmas
+ * is a plain FMA and v_div_fixup passes the quotient through, so the
+ * remaining operations, in the same order, give the same bits. */
+__device__ __forceinline__ float div_in_range(float n, float d)
 {
-    float mean_x = dot4(xx[0], xx[1], xx[2], xx[3], 0.25f, 0.25f, 0.25f, 0.25f);
-    if (!(cs < cg))
-        return mean_x;
-    float var_x[4], sx[4], sg[4];
-    bool above = true;
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-        var_x[i] = variance(xx[i], off, n0, n1, n2);
-        sx[i] = sqrtf(var_x[i]);
-        sg[i] = sqrtf(variance(gg[i], off, n0, n1, n2));
-        above = above & (sx[i] < xx[i]) & (sg[i] < gg[i]);
-    }
-    if (!above)
-        return mean_x;
-    float lo[4], mid[4], hi[4];
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-        lo[i] = ((cg - cs) * (xx[i] - sx[i])) / (sg[i] + gg[i]);
-        mid[i] = (cg * xx[i]) / gg[i];
-        hi[i] = ((cs + cg) * (sx[i] + xx[i])) / (gg[i] - sg[i]);
-    }
-    float spread = dot4(mid[0] - lo[0], mid[1] - lo[1], mid[2] - lo[2], mid[3] - lo[3],
-                        hi[0] - mid[0], hi[1] - mid[1], hi[2] - mid[2], hi[3] - mid[3]);
-    float mean_mid = dot4(mid[0], mid[1], mid[2], mid[3], 0.25f, 0.25f, 0.25f, 0.25f);
-    float dm0 = mid[0] - mean_mid, dm1 = mid[1] - mean_mid, dm2 = mid[2] - mean_mid,
-          dm3 = mid[3] - mean_mid;
-    float b = fmaf(spread, 0.75f, dot4(dm0, dm1, dm2, dm3, dm0, dm1, dm2, dm3));
-    float dx0 = xx[0] - mean_x, dx1 = xx[1] - mean_x, dx2 = xx[2] - mean_x, dx3 = xx[3] - mean_x;
-    float a = fmaf(dot4(var_x[0], var_x[1], var_x[2], var_x[3], 1.0f, 1.0f, 1.0f, 1.0f), 0.75f,
-                   dot4(dx0, dx1, dx2, dx3, dx0, dx1, dx2, dx3));
-    float weight = a / (b + a);
-    return fmaf(mean_mid - mean_x, weight, mean_x);
+    const float r = __builtin_amdgcn_rcpf(d);
+    const float r1 = fmaf(fmaf(-d, r, 1.0f), r, r);
+    const float q = n * r1;
+    const float q1 = fmaf(fmaf(-d, q, n), r1, q);
+    return fmaf(fmaf(-d, q1, n), r1, q1);
 }
 
 /* estimate (demosaic.rs:572-604) with sx[i] = sqrt(variance(xx[i])) and
  * sg[i] = sqrt(variance(gg[i])) already known (the same operations on the
- * same values, so the same bits). */
+ * same values, so the same bits). IN_RANGE: the frame meets
+ * chroma_in_range, and the lo / mid / hi quotients use div_in_range. */
+template <bool IN_RANGE>
 __device__ inline float estimate_pre(const float *xx, const float *gg, const float *sx,
                                      const float *sg, float cg, float cs, float off, float n0,
                                      float n1, float n2)
 {
+    auto div = [](float n, float d) { return IN_RANGE ? div_in_range(n, d) : n / d; };
     float mean_x = dot4(xx[0], xx[1], xx[2], xx[3], 0.25f, 0.25f, 0.25f, 0.25f);
     if (!(cs < cg))
         return mean_x;
@@ -1211,9 +1194,9 @@ __device__ inline float estimate_pre(const float *xx, const float *gg, const flo
 #pragma unroll
     for (int i = 0; i < 4; i++) {
         var_x[i] = variance(xx[i], off, n0, n1, n2);
-        lo[i] = ((cg - cs) * (xx[i] - sx[i])) / (sg[i] + gg[i]);
-        mid[i] = (cg * xx[i]) / gg[i];
-        hi[i] = ((cs + cg) * (sx[i] + xx[i])) / (gg[i] - sg[i]);
+        lo[i] = div((cg - cs) * (xx[i] - sx[i]), sg[i] + gg[i]);
+        mid[i] = div(cg * xx[i], gg[i]);
+        hi[i] = div((cs + cg) * (sx[i] + xx[i]), gg[i] - sg[i]);
     }
     float spread = dot4(mid[0] - lo[0], mid[1] - lo[1], mid[2] - lo[2], mid[3] - lo[3],
                         hi[0] - mid[0], hi[1] - mid[1], hi[2] - mid[2], hi[3] - mid[3]);
@@ -1266,6 +1249,7 @@ struct DevK {
     float coff;       /* black_over_range: the chroma pass offset */
     uint32_t red_row, red_col;
     uint32_t w, h;
+    bool in_range; /* chroma_in_range */
 };
 
 __device__ __forceinline__ DevK dev_consts(const DevelopDesc &d)
@@ -1283,6 +1267,19 @@ __device__ __forceinline__ DevK dev_consts(const DevelopDesc &d)
     k.red_col = (uint32_t)(d.cfa_phase & 1);
     k.w = d.width;
     k.h = d.height;
+    /* Whether every lo / mid / hi quotient of the chroma estimates meets
+     * div_in_range's bounds. They are taken only when cs < cg, sx < xx and
+     * sg < gg, with cs, sx, sg = sqrt(variance) >= sqrt(n2 + n1) >= 2^-20;
+     * so every difference cg - cs, xx - sx, gg - sg is >= 2^-43 (an exact
+     * difference of floats >= 2^-20, or more than half the larger one). A
+     * normalized value is one_over_range * (linear - black), below 2^9 in
+     * magnitude when one_over_range <= 2^-8 and |black| <= 2^16; a red/blue
+     * value xx <= 2^9 and a green estimate cg, gg (between two halves of
+     * neighbour sums of values clipped at 1) < 2^11. Then numerators lie in
+     * [2^-86, 2^22], denominators in [2^-43, 2^12], quotients in [2^-98,
+     * 2^65]. */
+    k.in_range = k.n1 + k.n2 >= 0x1p-40f && d.one_over_range > 0.0f &&
+                 d.one_over_range <= 0x1p-8f && d.black >= -65536 && d.black <= 65536;
     return k;
 }
 
@@ -1383,7 +1380,7 @@ __device__ __forceinline__ float2 green_g(const DevK &k, float centre)
  * of pixels (xb, y), (xb + 1, y), xb even: `green(y, x)` and `grc(y, x)`
  * read the red/blue and the green green-pass cells at picture coordinates.
  * res[i] is pixel xb + i. */
-template <class RbCell, class GCell>
+template <bool IN_RANGE, class RbCell, class GCell>
 __device__ __forceinline__ void chroma_pair(const DevelopDesc &d, const DevK &k, uint32_t y,
                                             uint32_t xb, RbCell green, GCell grc,
                                             float (&res)[2][3])
@@ -1420,7 +1417,7 @@ __device__ __forceinline__ void chroma_pair(const DevelopDesc &d, const DevK &k,
         const float gg[4] = {a.y, b.y, c2.y, e.y};
         const float sx[4] = {a.z, b.z, c2.z, e.z};
         const float sg[4] = {a.w, b.w, c2.w, e.w};
-        const float est = estimate_pre(xx, gg, sx, sg, mid.y, mid.w, off, n0, n1, n2);
+        const float est = estimate_pre<IN_RANGE>(xx, gg, sx, sg, mid.y, mid.w, off, n0, n1, n2);
         finish(on_red_row ? mid.x : est, mid.y, on_red_row ? est : mid.x, res_rb);
     }
     {
@@ -1436,8 +1433,8 @@ __device__ __forceinline__ void chroma_pair(const DevelopDesc &d, const DevK &k,
         const float hsx[4] = {l.z, r.z, l.z, r.z}, hsg[4] = {l.w, r.w, l.w, r.w};
         const float vx[4] = {u.x, dn.x, u.x, dn.x}, vg[4] = {u.y, dn.y, u.y, dn.y};
         const float vsx[4] = {u.z, dn.z, u.z, dn.z}, vsg[4] = {u.w, dn.w, u.w, dn.w};
-        const float eh = estimate_pre(hx, hg, hsx, hsg, mid.y, mid.w, off, n0, n1, n2);
-        const float ev = estimate_pre(vx, vg, vsx, vsg, mid.y, mid.w, off, n0, n1, n2);
+        const float eh = estimate_pre<IN_RANGE>(hx, hg, hsx, hsg, mid.y, mid.w, off, n0, n1, n2);
+        const float ev = estimate_pre<IN_RANGE>(vx, vg, vsx, vsg, mid.y, mid.w, off, n0, n1, n2);
         finish(on_red_row ? eh : ev, mid.y, on_red_row ? ev : eh, res_g);
     }
 #pragma unroll
@@ -1549,16 +1546,22 @@ __device__ inline void develop_tile(const DevelopDesc &d, float *out_r, float *o
     auto grc = [&](uint32_t y, uint32_t x) {
         return lds.gr[(int32_t)y - (y0 - 1)][((int32_t)x - (x0 - 2)) >> 1];
     };
-    for (uint32_t i = tid; i < DTH * (DTW / 2); i += 256) {
-        const uint32_t ly = i / (DTW / 2), lp = i - ly * (DTW / 2);
-        const uint32_t y = (uint32_t)y0 + ly;
-        const uint32_t xb = (uint32_t)x0 + 2 * lp;
-        if (y >= h || xb >= w)
-            continue;
-        float res[2][3];
-        chroma_pair(d, k, y, xb, rbc, grc, res);
-        store_pair(out_r, out_g, out_b, (size_t)(y - oy) * ostride + (xb - ox), xb, w, res);
-    }
+    auto chroma = [&](auto in_range) {
+        for (uint32_t i = tid; i < DTH * (DTW / 2); i += 256) {
+            const uint32_t ly = i / (DTW / 2), lp = i - ly * (DTW / 2);
+            const uint32_t y = (uint32_t)y0 + ly;
+            const uint32_t xb = (uint32_t)x0 + 2 * lp;
+            if (y >= h || xb >= w)
+                continue;
+            float res[2][3];
+            chroma_pair<decltype(in_range)::value>(d, k, y, xb, rbc, grc, res);
+            store_pair(out_r, out_g, out_b, (size_t)(y - oy) * ostride + (xb - ox), xb, w, res);
+        }
+    };
+    if (k.in_range)
+        chroma(std::true_type{});
+    else
+        chroma(std::false_type{});
 }
 
 __global__ __launch_bounds__(256) void k_develop(DevelopDesc d, float *out_r, float *out_g,

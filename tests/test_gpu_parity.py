@@ -2,7 +2,9 @@
 
 Code values, linear values, reduced RGB and skipped tiles must be
 bit-exact; develop output within 1e-5 relative max-abs (max |gpu - oracle|
-over max |oracle|), with bit-exactness reported."""
+over max |oracle|), the bar for floating point, and also bit-exact, which
+the develop kernel's operation-for-operation arithmetic gives (asserted, so
+that a change to it that moves any bit is seen)."""
 import numpy as np
 import pytest
 
@@ -63,6 +65,7 @@ def check_develop(frame, h, samples, prm, oprm, stage):
         exact &= bool(np.array_equal(got, want))
     say(f"  stage {stage}: relative max-abs {worst:.3g}, bit-exact {exact}")
     assert worst <= DEVELOP_RTOL
+    assert exact
     return exact
 
 
@@ -108,6 +111,36 @@ def test_develop_random_frames_all_stages(gpu):
         oprm = O.params(h, q, 1.8, 1.3)
         for stage in (0, 1, 2):
             check_develop(f, h, s, prm, oprm, stage)
+
+
+def test_develop_outside_the_in_range_division_bounds(gpu):
+    """Frames whose chroma quotients may fall outside div_in_range's bounds
+    take the general division: a white level of 400 (one_over_range 1/144)
+    and a noise model with n1 = n2 = 0; both bit-exact against the oracle,
+    as the frames inside the bounds are."""
+    import dataclasses
+    sc = S.scene(640, 360, 0, seed=5, version=0, alignment=1, qscale=(15, 45), white_level=400)
+    pk = S.encode_scene(sc)
+    f = P.decode_frame(pk)
+    h, s, sk, mq = O.decode_frame(pk)
+    np.testing.assert_array_equal(f.samples, s)
+    assert f.header.white_level == 400
+    q = h.quant_matrix[0] * mq
+    prm = P.DemosaicParams.with_white_balance(f.header, q, 2.0, 1.6)
+    oprm = O.params(h, q, 2.0, 1.6)
+    assert prm.one_over_range > 2.0 ** -8
+    for stage in (0, 1, 2):
+        check_develop(f, h, s, prm, oprm, stage)
+    pk0 = S.encode_scene(config0())
+    f0 = P.decode_frame(pk0)
+    h0, s0, sk0, mq0 = O.decode_frame(pk0)
+    q0 = h0.quant_matrix[0] * mq0
+    prm0 = P.DemosaicParams.new(f0.header, q0)
+    prm0 = dataclasses.replace(prm0, noise=(prm0.noise[0], 0.0, 0.0))
+    oprm0 = O.params(h0, q0)
+    oprm0.noise[1] = 0.0
+    oprm0.noise[2] = 0.0
+    check_develop(f0, h0, s0, prm0, oprm0, 2)
 
 
 def test_frames_beyond_the_register_scan(gpu):
