@@ -23,8 +23,6 @@
  * calls mul_add, and division and square root are correctly rounded (the
  * defaults, or div_in_range where its bounds are shown to hold).
  */
-#include <type_traits>
-
 #include "prr_internal.h"
 #include "prr_net.h"
 
@@ -1171,6 +1169,20 @@ __device__ __forceinline__ float div_in_range(float n, float d)
     return fmaf(fmaf(-d, q1, n), r1, q1);
 }
 
+/* sqrt(x) correctly rounded, as the general square root sequence computes
+ * it (scale below 2^-96, v_sqrt, the two one-ulp corrections by FMA
+ * residuals, unscale, pass zeros and +inf through) without the steps that
+ * only act on x < 2^-96, zeros and +inf: for x >= 2^-96 the remaining
+ * operations, in the same order, give the same bits. */
+__device__ __forceinline__ float sqrt_in_range(float x)
+{
+    const float s = __builtin_amdgcn_sqrtf(x);
+    const float down = __uint_as_float(__float_as_uint(s) - 1u);
+    const float up = __uint_as_float(__float_as_uint(s) + 1u);
+    const float r = fmaf(-down, s, x) <= 0.0f ? down : s;
+    return fmaf(-up, s, x) > 0.0f ? up : r;
+}
+
 /* estimate (demosaic.rs:572-604) with sx[i] = sqrt(variance(xx[i])) and
  * sg[i] = sqrt(variance(gg[i])) already known (the same operations on the
  * same values, so the same bits). IN_RANGE: the frame meets
@@ -1268,7 +1280,7 @@ __device__ __forceinline__ DevK dev_consts(const DevelopDesc &d)
     k.w = d.width;
     k.h = d.height;
     /* Whether every lo / mid / hi quotient of the chroma estimates meets
-     * div_in_range's bounds. They are taken only when cs < cg, sx < xx and
+     * div_in_range's bounds (and every variance sqrt_in_range's). They are taken only when cs < cg, sx < xx and
      * sg < gg, with cs, sx, sg = sqrt(variance) >= sqrt(n2 + n1) >= 2^-20;
      * so every difference cg - cs, xx - sx, gg - sg is >= 2^-43 (an exact
      * difference of floats >= 2^-20, or more than half the larger one). A
@@ -1281,6 +1293,16 @@ __device__ __forceinline__ DevK dev_consts(const DevelopDesc &d)
     k.in_range = k.n1 + k.n2 >= 0x1p-40f && d.one_over_range > 0.0f &&
                  d.one_over_range <= 0x1p-8f && d.black >= -65536 && d.black <= 65536;
     return k;
+}
+
+/* sqrt(variance(v)) at offset `off`. Every variance is at least n2 + n1
+ * (variance's max is at least n1), so on an in-range frame at least 2^-40
+ * and sqrt_in_range applies. */
+template <bool IN_RANGE>
+__device__ __forceinline__ float deviation(float v, float off, const DevK &k)
+{
+    const float var = variance(v, off, k.n0, k.n1, k.n2);
+    return IN_RANGE ? sqrt_in_range(var) : sqrtf(var);
 }
 
 /* Normalized value of a code value (Conversion::new folded table,
@@ -1299,7 +1321,7 @@ __device__ __forceinline__ float norm_of(const DevelopDesc &d, const uint32_t *c
  * only act when edge_x / edge_y say the tile reaches within two sensels of
  * the picture's side). Returns the cell (rb, green, sqrt(variance(rb)),
  * sqrt(variance(green))) at the chroma offset. */
-template <class Norm, class Dev>
+template <bool IN_RANGE, class Norm, class Dev>
 __device__ __forceinline__ float4 green_rb(const DevK &k, int32_t gx, int32_t gy, float clip,
                                            Norm norm, Dev dev, bool edge_x, bool edge_y)
 {
@@ -1364,15 +1386,16 @@ __device__ __forceinline__ float4 green_rb(const DevK &k, int32_t gx, int32_t gy
     const float eh = dot4(dh0, dh1, dh2, dh3, dh0, dh1, dh2, dh3) + (var_h[0] + var_h[1]);
     const float ev = dot4(dv0, dv1, dv2, dv3, dv0, dv1, dv2, dv3) + (var_v[0] + var_v[1]);
     const float gout = gv + (gh - gv) * ev / (eh + ev);
-    return make_float4(c, gout, sqrtf(variance(c, k.coff, n0, n1, n2)),
-                       sqrtf(variance(gout, k.coff, n0, n1, n2)));
+    return make_float4(c, gout, deviation<IN_RANGE>(c, k.coff, k),
+                       deviation<IN_RANGE>(gout, k.coff, k));
 }
 
 /* The green-pass cell of a green sensel: its own value as green, and
  * sqrt(variance(green)) at the chroma offset. */
+template <bool IN_RANGE>
 __device__ __forceinline__ float2 green_g(const DevK &k, float centre)
 {
-    return make_float2(centre, sqrtf(variance(centre, k.coff, k.n0, k.n1, k.n2)));
+    return make_float2(centre, deviation<IN_RANGE>(centre, k.coff, k));
 }
 
 /* Chroma pass (chroma_row_with, demosaic.rs:648-688), white balance and
@@ -1475,11 +1498,11 @@ __device__ __forceinline__ void store_pair(float *out_r, float *out_g, float *ou
  * LDS; each lane handles a red/blue + green pair of sensels so every lane
  * does the same work. Output sensel (x, y) goes to (y - oy) * ostride +
  * (x - ox) of the planes (ox = oy = 0, ostride = width for a whole frame). */
-__device__ inline void develop_tile(const DevelopDesc &d, float *out_r, float *out_g,
-                                    float *out_b, uint32_t bx, uint32_t by, DevelopLds &lds,
-                                    uint32_t ox, uint32_t oy, uint32_t ostride)
+template <bool IN_RANGE>
+__device__ inline void develop_tile_as(const DevelopDesc &d, const DevK &k, float *out_r,
+                                       float *out_g, float *out_b, uint32_t bx, uint32_t by,
+                                       DevelopLds &lds, uint32_t ox, uint32_t oy, uint32_t ostride)
 {
-    const DevK k = dev_consts(d);
     const uint32_t tid = threadIdx.x;
     const uint32_t w = k.w, h = k.h;
     const int32_t x0 = (int32_t)(bx * DTW), y0 = (int32_t)(by * DTH);
@@ -1501,7 +1524,7 @@ __device__ inline void develop_tile(const DevelopDesc &d, float *out_r, float *o
         const bool rb = ((uint32_t)px & 1u) == (red_row ? k.red_col : 1u - k.red_col);
         const float cap = rb ? (red_row ? clip_red : clip_blue) : 1.0f;
         lds.norm[ly][lx] = v;
-        lds.dev[ly][lx] = sqrtf(variance(fminf(v, cap), k.bs, k.n0, k.n1, k.n2));
+        lds.dev[ly][lx] = deviation<IN_RANGE>(fminf(v, cap), k.bs, k);
     }
     __syncthreads();
 
@@ -1523,18 +1546,18 @@ __device__ inline void develop_tile(const DevelopDesc &d, float *out_r, float *o
         const bool phase = ((uint32_t)gy & 1u) == k.red_row;
         const int32_t first = (int32_t)(phase ? k.red_col : 1 - k.red_col);
         const float clip = phase ? d.clip[0] : d.clip[1];
-        auto in_range = [&](int32_t gx) {
+        auto cell_in_tile = [&](int32_t gx) {
             return gx >= x0 - 1 && gx <= x0 + DTW && gx >= 0 && gx < (int32_t)w;
         };
         {
             const int32_t gx = x0 - 2 + 2 * lp + (1 - first);
-            if (in_range(gx))
-                lds.gr[ly][lp] = green_g(k, norm(gy, gx));
+            if (cell_in_tile(gx))
+                lds.gr[ly][lp] = green_g<IN_RANGE>(k, norm(gy, gx));
         }
         const int32_t gx = x0 - 2 + 2 * lp + first;
-        if (!in_range(gx))
+        if (!cell_in_tile(gx))
             continue;
-        lds.rb[ly][lp] = green_rb(k, gx, gy, clip, norm, dev, edge_x, edge_y);
+        lds.rb[ly][lp] = green_rb<IN_RANGE>(k, gx, gy, clip, norm, dev, edge_x, edge_y);
     }
     __syncthreads();
 
@@ -1546,22 +1569,29 @@ __device__ inline void develop_tile(const DevelopDesc &d, float *out_r, float *o
     auto grc = [&](uint32_t y, uint32_t x) {
         return lds.gr[(int32_t)y - (y0 - 1)][((int32_t)x - (x0 - 2)) >> 1];
     };
-    auto chroma = [&](auto in_range) {
-        for (uint32_t i = tid; i < DTH * (DTW / 2); i += 256) {
-            const uint32_t ly = i / (DTW / 2), lp = i - ly * (DTW / 2);
-            const uint32_t y = (uint32_t)y0 + ly;
-            const uint32_t xb = (uint32_t)x0 + 2 * lp;
-            if (y >= h || xb >= w)
-                continue;
-            float res[2][3];
-            chroma_pair<decltype(in_range)::value>(d, k, y, xb, rbc, grc, res);
-            store_pair(out_r, out_g, out_b, (size_t)(y - oy) * ostride + (xb - ox), xb, w, res);
-        }
-    };
+    for (uint32_t i = tid; i < DTH * (DTW / 2); i += 256) {
+        const uint32_t ly = i / (DTW / 2), lp = i - ly * (DTW / 2);
+        const uint32_t y = (uint32_t)y0 + ly;
+        const uint32_t xb = (uint32_t)x0 + 2 * lp;
+        if (y >= h || xb >= w)
+            continue;
+        float res[2][3];
+        chroma_pair<IN_RANGE>(d, k, y, xb, rbc, grc, res);
+        store_pair(out_r, out_g, out_b, (size_t)(y - oy) * ostride + (xb - ox), xb, w, res);
+    }
+}
+
+/* One 64x32 tile of develop: the in-range form when the frame's constants
+ * allow it (the same bits either way). */
+__device__ inline void develop_tile(const DevelopDesc &d, float *out_r, float *out_g,
+                                    float *out_b, uint32_t bx, uint32_t by, DevelopLds &lds,
+                                    uint32_t ox, uint32_t oy, uint32_t ostride)
+{
+    const DevK k = dev_consts(d);
     if (k.in_range)
-        chroma(std::true_type{});
+        develop_tile_as<true>(d, k, out_r, out_g, out_b, bx, by, lds, ox, oy, ostride);
     else
-        chroma(std::false_type{});
+        develop_tile_as<false>(d, k, out_r, out_g, out_b, bx, by, lds, ox, oy, ostride);
 }
 
 __global__ __launch_bounds__(256) void k_develop(DevelopDesc d, float *out_r, float *out_g,
