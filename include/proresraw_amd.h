@@ -253,6 +253,9 @@ int prr_batch_create(prr_context *ctx, prr_batch **out, prr_error *err);
 /* Add a frame (at most 256 per batch); `hdr`/`info` from prr_parse_frame_info. */
 int prr_batch_add(prr_batch *batch, const prr_header *hdr, const prr_frame_info *info,
                   const prr_batch_frame *frame, prr_error *err);
+/* Remove every frame, keeping the batch's device buffers for the next
+ * frames (a clip decoded window by window); the launch shape stays. */
+int prr_batch_clear(prr_batch *batch, prr_error *err);
 /* Launch shape: streams per workgroup (16/32/64, 64 by default; a frame's
  * is halved until its group fits the batch's 32 KiB LDS budget; frames
  * already added are laid out again), queue entries taken per atomic grab,

@@ -1,5 +1,5 @@
-"""ctypes declarations of the C ABI in include/proresraw_amd.h and
-include/proresraw_learned.h.
+"""ctypes declarations of the C ABI in include/proresraw_amd.h,
+include/proresraw_learned.h and include/proresraw_mov.h.
 
 Loading fails loudly when the native library has not been built: there is
 no fallback implementation behind this module.
@@ -113,6 +113,16 @@ class PrrNetFrame(ctypes.Structure):
     ]
 
 
+class PrrMovSample(ctypes.Structure):
+    _fields_ = [
+        ("offset", ctypes.c_uint64),
+        ("size", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
+    ]
+
+
+MOV_ACCEPT = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.POINTER(ctypes.c_uint8), ctypes.c_void_p)
+
 NET_PARAMS = 73984
 NET_LAYER_PARAMS = 9248
 
@@ -157,6 +167,7 @@ SIGNATURES = {
     "prr_batch_create": (i32, [c_vp, ctypes.POINTER(c_vp), E]),
     "prr_batch_add": (i32, [c_vp, H, FI, ctypes.POINTER(PrrBatchFrame), E]),
     "prr_batch_set_config": (i32, [c_vp, u32, u32, u32, E]),
+    "prr_batch_clear": (i32, [c_vp, E]),
     "prr_batch_run": (i32, [c_vp, c_vp, E]),
     "prr_batch_info": (i32, [c_vp, ctypes.POINTER(u32), ctypes.POINTER(u32), ctypes.POINTER(u32)]),
     "prr_batch_destroy": (None, [c_vp]),
@@ -173,6 +184,26 @@ SIGNATURES = {
     "prr_net_develop_device": (i32, [c_vp, H, c_vp, DP, c_vp, i32, u32, ctypes.c_float, c_vp,
                                      c_vp, c_vp, c_vp, E]),
     "prr_net_last_check": (i32, [c_vp, ctypes.POINTER(u32), ctypes.POINTER(ctypes.c_float), E]),
+    "prr_is_prores_raw_fourcc": (i32, [c_vp]),
+    "prr_mov_read_video_track": (i32, [i32, MOV_ACCEPT, c_vp, ctypes.POINTER(c_vp),
+                                       ctypes.c_char_p, sz]),
+    "prr_mov_track_free": (None, [c_vp]),
+    "prr_mov_track_fourcc": (None, [c_vp, c_vp]),
+    "prr_mov_track_fps": (None, [c_vp, ctypes.POINTER(u32), ctypes.POINTER(u32)]),
+    "prr_mov_track_variable_durations": (i32, [c_vp]),
+    "prr_mov_track_samples": (sz, [c_vp, ctypes.POINTER(ctypes.POINTER(PrrMovSample))]),
+    "prr_mov_track_creation_time": (i32, [c_vp, ctypes.POINTER(ctypes.c_uint64)]),
+    "prr_mov_track_orientation": (i32, [c_vp]),
+    "prr_mov_track_n_warnings": (sz, [c_vp]),
+    "prr_mov_track_warning": (c_vp, [c_vp, sz, ctypes.POINTER(sz)]),
+    "prr_mov_track_n_metadata": (sz, [c_vp]),
+    "prr_mov_track_metadata_key": (c_vp, [c_vp, sz, ctypes.POINTER(sz)]),
+    "prr_mov_track_metadata_value": (c_vp, [c_vp, sz, ctypes.POINTER(sz)]),
+    "prr_mov_track_text": (c_vp, [c_vp, ctypes.POINTER(ctypes.c_char_p), sz, ctypes.POINTER(sz)]),
+    "prr_mov_read_at": (i32, [i32, ctypes.c_uint64, c_vp, sz, ctypes.c_char_p, sz]),
+    "prr_mov_read_samples": (i32, [i32, ctypes.POINTER(PrrMovSample), sz, c_vp,
+                                   ctypes.POINTER(ctypes.c_uint64), ctypes.c_uint, ctypes.c_char_p,
+                                   sz]),
 }
 
 _lib = None

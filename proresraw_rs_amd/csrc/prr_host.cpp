@@ -919,6 +919,27 @@ extern "C" int prr_batch_add(prr_batch *b, const prr_header *hdr, const prr_fram
     return PRR_OK;
 }
 
+/* Remove every frame and keep the device buffers, so a caller decoding a
+ * stream of frame sets (windows of a clip) reuses one batch. Frames added
+ * after this are uploaded by the next prr_batch_run, in stream order after
+ * the work already enqueued on that stream. */
+extern "C" int prr_batch_clear(prr_batch *b, prr_error *err)
+{
+    if (!b)
+        return fail(err, PRR_E_INVALID_ARGUMENT, "batch");
+    b->frames.clear();
+    b->shapes.clear();
+    b->jobs.clear();
+    b->offset_base.clear();
+    b->offsets_total = 0;
+    b->chunks.clear();
+    b->n_items = 0;
+    b->max_w = b->max_h = 0;
+    b->lds = 0;
+    b->uploaded = false;
+    return PRR_OK;
+}
+
 extern "C" int prr_batch_set_config(prr_batch *b, uint32_t streams_per_wg, uint32_t chunk,
                                     uint32_t waves_per_cu, prr_error *err)
 {
