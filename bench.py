@@ -40,6 +40,7 @@ def parse_args():
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--learned-frames", type=int, default=32)
     ap.add_argument("--no-clip", action="store_true")
+    ap.add_argument("--no-dng", action="store_true")
     ap.add_argument("--clip-frames", type=int, default=48)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device to device 10 times (the reference reading "
@@ -224,6 +225,7 @@ def main():
     batched = None if args.no_batch else batched_bench(args)
     learned = None if args.no_learned else learned_bench(args, value)
     clip = None if args.no_clip else clip_bench(args)
+    dng = None if args.no_dng else dng_bench(args)
 
     cpu = None
     if not args.no_cpu:
@@ -250,6 +252,7 @@ def main():
         "batched": batched,
         "batched_learned": learned,
         "clip": clip,
+        "dng_tiles": dng,
     }
     if cpu:
         line["gpu_over_cpu"] = round(value / cpu["value"], 1)
@@ -498,6 +501,65 @@ def clip_bench(args):
             "workload": f"{frames}-frame .mov of 5760x3240 RGGB v0 a3 q45 (8 scenes cycled): "
                         "sample table + windowed pread into pinned memory + copy + batched "
                         "decode+develop(WhiteBalanced), 16 frames a window"}
+
+
+def dng_bench(args):
+    """DNG raw tiles of a decoded 5760x3240 RGGB frame (v0, alignment 3,
+    qscale 45): its visible code values, read in place at the coded stride,
+    coded as 299 lossless JPEG tiles on the device; checked once against
+    the oracle's bytes and timed beside the oracle on the host cores."""
+    import torch
+
+    import proresraw_rs_amd as P
+    from oracle import oracle as O
+    from proresraw_rs_amd import dng as D
+    from proresraw_rs_amd import synth as S
+
+    packet = S.encode_scene(S.scene(5760, 3240, 0, seed=77, version=0, alignment=3,
+                                    qscale=(45, 45)))
+    frame = P.decode_frame(packet)
+    h = frame.header
+    w, ht = h.width, h.height
+    stride = (w + 15) & ~15
+    samples = np.ascontiguousarray(frame.samples, dtype=np.uint16).reshape(-1, stride)
+    d_samples = torch.from_numpy(samples.view(np.int16).reshape(-1)).cuda()
+    n = D.tile_count(w, ht)
+    out = torch.empty(D.capacity(w, ht), dtype=torch.uint8, device="cuda")
+    off = torch.empty(n, dtype=torch.int32, device="cuda")
+    ln = torch.empty(n, dtype=torch.int32, device="cuda")
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        D.lj92_tiles_device(d_samples, w, ht, stride, out, off, ln, stream=s)
+        s.synchronize()
+        reps = 20
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(s)
+        for _ in range(reps):
+            D.lj92_tiles_device(d_samples, w, ht, stride, out, off, ln, stream=s)
+        e1.record(s)
+        e1.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    offsets = off.cpu().numpy().view(np.uint32)
+    lengths = ln.cpu().numpy().view(np.uint32)
+    total = int(offsets[-1]) + int(lengths[-1])
+    got = out[:total].cpu().numpy().tobytes()
+    visible = samples[:ht, :w]
+    t0 = time.perf_counter()
+    want, w_off, w_len = O.dng_lj92_tiles(visible, threads=O.THREADS)
+    cpu_s = time.perf_counter() - t0
+    exact = got == want and np.array_equal(offsets, w_off) and np.array_equal(lengths, w_len)
+    sensels = w * ht
+    return {"value": round(sensels / (ms * 1e-3) / 1e9, 3), "unit": "Gsensels/s",
+            "ms_per_frame": round(ms, 4), "tiles": n, "bytes": total,
+            "bits_per_sensel": round(8.0 * total / sensels, 3),
+            "algorithmic_gb_per_s": round((2 * sensels + total) / (ms * 1e-3) / 1e9, 1),
+            "bit_exact_vs_oracle": bool(exact),
+            "cpu_baseline": {"value": round(sensels / cpu_s / 1e9, 4), "unit": "Gsensels/s",
+                             "cores": O.THREADS, "kind": "port",
+                             "sample": "one frame's tiles, oracle/lj92_oracle.c"},
+            "workload": "DNG LosslessJpeg tiles (256x256, 2 components, predictor 1, optimal "
+                        "Huffman table a tile) of a decoded 5760x3240 RGGB v0 a3 q45 frame's "
+                        "code values at the coded stride"}
 
 
 def cpu_baseline(packet, seconds):

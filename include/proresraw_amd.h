@@ -221,6 +221,35 @@ int prr_develop_linear_device(prr_context *ctx, const prr_header *hdr, const uin
                               float wb_blue, int stage, float *d_r, float *d_g, float *d_b,
                               void *stream, prr_error *err);
 
+/* ---- DNG raw tiles: the LosslessJpeg branch of the DNG writer
+ * (examples/prr2dng/dng.rs:92-113): every 256x256 tile of a width x height
+ * picture, padded as dng.rs `tile` pads it (sensels past the right or
+ * bottom edge repeat the last column or row of the same CFA phase,
+ * dng.rs:115-128), coded as lj92::encode(tile, 256, 256, 2)
+ * (examples/prr2dng/lj92.rs:14-93: two interleaved components, predictor
+ * 1, one optimal Huffman table a tile). The streams are written back to
+ * back in tile order (row-major over tiles) with each one's byte offset and
+ * length (TileOffsets / TileByteCounts relative to the first). The picture
+ * is the visible code values (Frame::rows, stride coded_width) or the
+ * linearized values (stride width). width and height are at least 2. ---- */
+/* Worst-case bytes of the streams (every tile at 31 bits a sample, every
+ * byte stuffed). */
+size_t prr_dng_lj92_capacity(uint32_t width, uint32_t height);
+/* Device buffers, enqueued on `stream`: d_out holds at least
+ * prr_dng_lj92_capacity bytes, d_offsets and d_lengths one uint32 per tile.
+ * Uses the context's scratch: calls on one context are serialized by the
+ * caller. */
+int prr_dng_lj92_tiles_device(prr_context *ctx, const uint16_t *d_values, uint32_t width,
+                              uint32_t height, uint32_t stride, uint8_t *d_out, size_t out_cap,
+                              uint32_t *d_offsets, uint32_t *d_lengths, void *stream,
+                              prr_error *err);
+/* Host buffers, synchronous: `n_tiles` must be the picture's tile count;
+ * *total receives the streams' size, and PRR_E_OUTPUT_SIZE is returned
+ * (with *total set) when out_cap is smaller. */
+int prr_dng_lj92_tiles(prr_context *ctx, const uint16_t *values, uint32_t width, uint32_t height,
+                       uint32_t stride, uint8_t *out, size_t out_cap, uint32_t *offsets,
+                       uint32_t *lengths, size_t n_tiles, size_t *total, prr_error *err);
+
 /* ---- batches (no counterpart in the crate, which decodes one frame per
  * call): many frames of any size, version, alignment and qscale decoded
  * in one persistent launch whose waves pull tile groups from a device

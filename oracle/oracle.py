@@ -78,6 +78,11 @@ _SIG = {
     "orc_green_pass": (None, [vp, sz, sz, PP, vp]),
     "orc_chroma_pass": (None, [vp, sz, sz, PP, vp]),
     "orc_apply_matrix": (None, [vp, sz, vp]),
+    "orc_dng_tile": (None, [vp, sz, sz, sz, sz, vp]),
+    "orc_lj92_table": (i32, [vp, vp, vp, vp, vp]),
+    "orc_lj92_encode": (sz, [vp, sz, sz, sz, vp, sz]),
+    "orc_lj92_tile_capacity": (sz, []),
+    "orc_dng_lj92_tiles": (sz, [vp, sz, sz, vp, sz, vp, vp, i32]),
 }
 
 _lib = None
@@ -204,3 +209,53 @@ def develop_interleaved(h, samples, prm, stage, threads=None):
     lib().orc_develop(ctypes.byref(h), samples.ctypes.data, cw, ctypes.byref(prm), int(stage),
                       threads or THREADS, out.ctypes.data, None, None)
     return out
+
+
+# ---- lossless JPEG tiles of a DNG (oracle/lj92_oracle.c) -------------------
+
+def dng_tile(values, left, top):
+    """dng.rs:115-128: the 256x256 tile at (left, top) of a height x width
+    picture (uint16, row-major)."""
+    v = np.ascontiguousarray(values, dtype=np.uint16)
+    out = np.zeros((256, 256), dtype=np.uint16)
+    lib().orc_dng_tile(v.ctypes.data, v.shape[1], v.shape[0], left, top, out.ctypes.data)
+    return out
+
+
+def lj92_table(histogram):
+    """HuffmanTable::optimal (lj92.rs:105-206): (counts[16], values,
+    codes[17], lengths[17])."""
+    h = np.ascontiguousarray(histogram, dtype=np.uint64)
+    counts = np.zeros(16, dtype=np.uint8)
+    values = np.zeros(17, dtype=np.uint8)
+    code = np.zeros(17, dtype=np.uint16)
+    length = np.zeros(17, dtype=np.uint8)
+    n = lib().orc_lj92_table(h.ctypes.data, counts.ctypes.data, values.ctypes.data,
+                             code.ctypes.data, length.ctypes.data)
+    return counts, values[:n].copy(), code, length
+
+
+def lj92_encode(samples, components=2):
+    """encode (lj92.rs:14-88) of a height x width uint16 array."""
+    s = np.ascontiguousarray(samples, dtype=np.uint16)
+    h, w = s.shape
+    cap = 128 + 4 * s.size + 16
+    out = np.zeros(cap, dtype=np.uint8)
+    n = lib().orc_lj92_encode(s.ctypes.data, w, h, components, out.ctypes.data, cap)
+    assert n <= cap
+    return out[:n].tobytes()
+
+
+def dng_lj92_tiles(values, threads=None):
+    """dng.rs:92-113: every 256x256 tile's stream in tile order:
+    (bytes, offsets, lengths)."""
+    v = np.ascontiguousarray(values, dtype=np.uint16)
+    h, w = v.shape
+    n = ((w + 255) // 256) * ((h + 255) // 256)
+    cap = n * lib().orc_lj92_tile_capacity()
+    out = np.zeros(cap, dtype=np.uint8)
+    offsets = np.zeros(n, dtype=np.uint32)
+    lengths = np.zeros(n, dtype=np.uint32)
+    total = lib().orc_dng_lj92_tiles(v.ctypes.data, w, h, out.ctypes.data, cap,
+                                     offsets.ctypes.data, lengths.ctypes.data, threads or THREADS)
+    return out[:total].tobytes(), offsets, lengths

@@ -164,6 +164,10 @@ SIGNATURES = {
                                        i32, c_vp, c_vp, c_vp, c_vp, E]),
     "prr_develop_linear_device": (i32, [c_vp, H, c_vp, c_vp, i32, ctypes.c_float, ctypes.c_float,
                                         i32, c_vp, c_vp, c_vp, c_vp, E]),
+    "prr_dng_lj92_capacity": (sz, [u32, u32]),
+    "prr_dng_lj92_tiles_device": (i32, [c_vp, c_vp, u32, u32, u32, c_vp, sz, c_vp, c_vp, c_vp, E]),
+    "prr_dng_lj92_tiles": (i32, [c_vp, c_vp, u32, u32, u32, c_vp, sz, c_vp, c_vp, sz,
+                                 ctypes.POINTER(sz), E]),
     "prr_batch_create": (i32, [c_vp, ctypes.POINTER(c_vp), E]),
     "prr_batch_add": (i32, [c_vp, H, FI, ctypes.POINTER(PrrBatchFrame), E]),
     "prr_batch_set_config": (i32, [c_vp, u32, u32, u32, E]),

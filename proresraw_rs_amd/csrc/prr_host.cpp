@@ -265,6 +265,7 @@ struct prr_context {
     hipStream_t stream = nullptr;
     DecodeLaunch launch{8, 65536, 256};
     DevBuf packet, offsets, queue, status, flags, samples, out0, out1, out2;
+    DevBuf lj_raw, lj_records, lj_values, lj_out, lj_meta; /* DNG tile streams */
     std::vector<uint8_t> host_flags;
 };
 
@@ -317,7 +318,8 @@ extern "C" void prr_context_destroy(prr_context *ctx)
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     for (DevBuf *b : {&ctx->packet, &ctx->offsets, &ctx->queue, &ctx->status, &ctx->flags,
-                      &ctx->samples, &ctx->out0, &ctx->out1, &ctx->out2})
+                      &ctx->samples, &ctx->out0, &ctx->out1, &ctx->out2, &ctx->lj_raw,
+                      &ctx->lj_records, &ctx->lj_values, &ctx->lj_out, &ctx->lj_meta})
         if (b->ptr)
             (void)hipFree(b->ptr);
     (void)hipStreamDestroy(ctx->stream);
@@ -797,6 +799,94 @@ struct prr_batch {
     uint32_t grid = 0;
     bool uploaded = false;
 };
+
+/* ------------------------------------------------------------------ */
+/* Lossless JPEG tiles of a DNG (dng.rs:92-128, lj92.rs:14-93). */
+
+static uint64_t lj92_tiles(uint32_t width, uint32_t height)
+{
+    return (uint64_t)((width + LJ92_TILE - 1) / LJ92_TILE) *
+           ((height + LJ92_TILE - 1) / LJ92_TILE);
+}
+
+extern "C" size_t prr_dng_lj92_capacity(uint32_t width, uint32_t height)
+{
+    return (size_t)(lj92_tiles(width, height) * LJ92_TILE_BYTES);
+}
+
+static int lj92_check(uint32_t width, uint32_t height, uint32_t stride, prr_error *err)
+{
+    /* dng.rs's padding repeats the last two rows and columns. */
+    if (width < 2 || height < 2 || stride < width)
+        return fail(err, PRR_E_INVALID_ARGUMENT, "dng picture size", width, height);
+    /* Tile offsets are 32-bit, as TIFF's: the worst case must fit. */
+    if (lj92_tiles(width, height) * LJ92_TILE_BYTES > UINT32_MAX)
+        return fail(err, PRR_E_INVALID_ARGUMENT, "dng picture too large", width, height);
+    return PRR_OK;
+}
+
+extern "C" int prr_dng_lj92_tiles_device(prr_context *ctx, const uint16_t *d_values,
+                                         uint32_t width, uint32_t height, uint32_t stride,
+                                         uint8_t *d_out, size_t out_cap, uint32_t *d_offsets,
+                                         uint32_t *d_lengths, void *stream, prr_error *err)
+{
+    if (!ctx || !d_values || !d_out || !d_offsets || !d_lengths)
+        return fail(err, PRR_E_INVALID_ARGUMENT, "null argument");
+    int rc;
+    if ((rc = lj92_check(width, height, stride, err)))
+        return rc;
+    if (out_cap < prr_dng_lj92_capacity(width, height))
+        return fail(err, PRR_E_OUTPUT_SIZE, "dng tile streams", prr_dng_lj92_capacity(width, height),
+                    out_cap);
+    HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+    const uint64_t n = lj92_tiles(width, height);
+    if ((rc = ensure(ctx, ctx->lj_raw, (size_t)n * LJ92_RAW_WORDS * 4, err)) ||
+        (rc = ensure(ctx, ctx->lj_records, (size_t)n * sizeof(Lj92Record), err)))
+        return rc;
+    HIP_TRY(launch_lj92(d_values, stride, width, height, (uint32_t *)ctx->lj_raw.ptr,
+                        (Lj92Record *)ctx->lj_records.ptr, d_out, d_offsets, d_lengths,
+                        (hipStream_t)stream),
+            "lj92 kernels");
+    return PRR_OK;
+}
+
+extern "C" int prr_dng_lj92_tiles(prr_context *ctx, const uint16_t *values, uint32_t width,
+                                  uint32_t height, uint32_t stride, uint8_t *out, size_t out_cap,
+                                  uint32_t *offsets, uint32_t *lengths, size_t n_tiles,
+                                  size_t *total, prr_error *err)
+{
+    if (!ctx || !values || !offsets || !lengths || !total || (!out && out_cap))
+        return fail(err, PRR_E_INVALID_ARGUMENT, "null argument");
+    int rc;
+    if ((rc = lj92_check(width, height, stride, err)))
+        return rc;
+    const uint64_t n = lj92_tiles(width, height);
+    if (n_tiles != n)
+        return fail(err, PRR_E_OUTPUT_SIZE, "dng tile count", n, n_tiles);
+    HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+    const size_t cap = prr_dng_lj92_capacity(width, height);
+    if ((rc = ensure(ctx, ctx->lj_values, (size_t)width * height * 2, err)) ||
+        (rc = ensure(ctx, ctx->lj_out, cap, err)) ||
+        (rc = ensure(ctx, ctx->lj_meta, (size_t)n * 8, err)))
+        return rc;
+    hipStream_t s = ctx->stream;
+    HIP_TRY(hipMemcpy2DAsync(ctx->lj_values.ptr, (size_t)width * 2, values, (size_t)stride * 2,
+                             (size_t)width * 2, height, hipMemcpyHostToDevice, s),
+            "hipMemcpy2D values");
+    uint32_t *d_off = (uint32_t *)ctx->lj_meta.ptr, *d_len = d_off + n;
+    if ((rc = prr_dng_lj92_tiles_device(ctx, (const uint16_t *)ctx->lj_values.ptr, width, height,
+                                        width, (uint8_t *)ctx->lj_out.ptr, cap, d_off, d_len, s,
+                                        err)))
+        return rc;
+    HIP_TRY(hipMemcpyAsync(offsets, d_off, (size_t)n * 4, hipMemcpyDeviceToHost, s), "hipMemcpy");
+    HIP_TRY(hipMemcpyAsync(lengths, d_len, (size_t)n * 4, hipMemcpyDeviceToHost, s), "hipMemcpy");
+    HIP_TRY(hipStreamSynchronize(s), "hipStreamSynchronize");
+    *total = (size_t)offsets[n - 1] + lengths[n - 1];
+    if (*total > out_cap)
+        return fail(err, PRR_E_OUTPUT_SIZE, "dng tile streams", *total, out_cap);
+    HIP_TRY(hipMemcpy(out, ctx->lj_out.ptr, *total, hipMemcpyDeviceToHost), "hipMemcpy out");
+    return PRR_OK;
+}
 
 extern "C" int prr_batch_create(prr_context *ctx, prr_batch **out, prr_error *err)
 {

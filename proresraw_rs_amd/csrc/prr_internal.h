@@ -97,6 +97,25 @@ struct DecodeLaunch {
 
 /* Status reset + size-table scan + tile order (k_prepare), then the
  * decode; `queue` is device scratch of nb_tiles uint32. */
+/* Lossless JPEG tiles of a DNG (prr_dng.hip). */
+enum { LJ92_TILE = 256 };
+/* Raw (unstuffed) entropy-coded words a tile may need: 31 bits a sample at
+ * most (a 16-bit code and 15 extra bits), rounded up. */
+enum { LJ92_RAW_WORDS = 63504 };
+/* Bytes a tile's complete stream may need: headers, every raw byte
+ * stuffed, EOI. */
+enum { LJ92_TILE_BYTES = 128 + 2 * 4 * LJ92_RAW_WORDS };
+struct Lj92Record {
+    uint32_t raw_bits;   /* entropy-coded bits before the padding */
+    uint32_t ff;         /* 0xFF bytes among the raw bytes */
+    uint32_t header_len; /* SOI, SOF3, DHT, SOS */
+    uint32_t total;      /* the tile's complete stream in bytes */
+    uint8_t header[96];
+};
+hipError_t launch_lj92(const uint16_t *values, uint32_t stride, uint32_t width, uint32_t height,
+                       uint32_t *raw, Lj92Record *records, uint8_t *out, uint32_t *offsets,
+                       uint32_t *lengths, hipStream_t s);
+
 hipError_t launch_decode(const FrameDesc &f, int ffmpeg, uint16_t *samples, uint16_t *linear,
                          uint8_t *flags, prr_device_status *status, uint32_t *queue,
                          const DecodeLaunch &cfg, hipStream_t s);
