@@ -41,6 +41,7 @@ def parse_args():
     ap.add_argument("--learned-frames", type=int, default=32)
     ap.add_argument("--no-clip", action="store_true")
     ap.add_argument("--no-dng", action="store_true")
+    ap.add_argument("--no-orient", action="store_true")
     ap.add_argument("--clip-frames", type=int, default=48)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device to device 10 times (the reference reading "
@@ -226,6 +227,7 @@ def main():
     learned = None if args.no_learned else learned_bench(args, value)
     clip = None if args.no_clip else clip_bench(args)
     dng = None if args.no_dng else dng_bench(args)
+    orient = None if args.no_orient else orient_bench(args)
 
     cpu = None
     if not args.no_cpu:
@@ -253,6 +255,7 @@ def main():
         "batched_learned": learned,
         "clip": clip,
         "dng_tiles": dng,
+        "orient": orient,
     }
     if cpu:
         line["gpu_over_cpu"] = round(value / cpu["value"], 1)
@@ -560,6 +563,38 @@ def dng_bench(args):
             "workload": "DNG LosslessJpeg tiles (256x256, 2 components, predictor 1, optimal "
                         "Huffman table a tile) of a decoded 5760x3240 RGGB v0 a3 q45 frame's "
                         "code values at the coded stride"}
+
+
+def orient_bench(args):
+    """Crop + display orientation of three 5760x3240 f32 develop planes
+    (crop 8/8/4/4, EXIF 6 = a quarter turn clockwise, and 3 = a half turn):
+    HBM bytes moved = 12 B/sensel read + 12 B/sensel written."""
+    import torch
+
+    from proresraw_rs_amd import orient as R
+
+    w, h = 5760, 3240
+    margins = (8, 8, 4, 4)
+    planes = [torch.rand(w * h, device="cuda") for _ in range(3)]
+    res = {}
+    for o in (6, 3):
+        ow, oh = R.oriented_size(w, h, margins, o)
+        outs = [torch.empty(ow * oh, device="cuda") for _ in range(3)]
+        s = torch.cuda.Stream()
+        with torch.cuda.stream(s):
+            R.orient_planes_device(planes, w, h, w, margins, o, outs, ow, stream=s)
+            reps = 20
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(s)
+            for _ in range(reps):
+                R.orient_planes_device(planes, w, h, w, margins, o, outs, ow, stream=s)
+            e1.record(s)
+            e1.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        moved = 24.0 * ow * oh
+        res[f"exif{o}"] = {"ms": round(ms, 4), "gb_per_s": round(moved / (ms * 1e-3) / 1e9, 1),
+                           "frac_of_8tbs": round(moved / (ms * 1e-3) / 8e12, 3)}
+    return {"workload": "crop (8, 8, 4, 4) + orientation of three 5760x3240 f32 planes", **res}
 
 
 def cpu_baseline(packet, seconds):

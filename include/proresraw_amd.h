@@ -250,6 +250,37 @@ int prr_dng_lj92_tiles(prr_context *ctx, const uint16_t *values, uint32_t width,
                        uint32_t stride, uint8_t *out, size_t out_cap, uint32_t *offsets,
                        uint32_t *lengths, size_t n_tiles, size_t *total, prr_error *err);
 
+/* ---- crop and orientation for display. The crate records them and
+ * leaves them to readers: mov::Orientation (src/mov.rs:121-157, the eight
+ * EXIF symmetries of a track's display matrix) and header::Crop
+ * (src/header.rs:98-109), written as the DNG Orientation and DefaultCrop
+ * tags (examples/prr2dng/dng.rs:155-158, 190-196). These entry points apply
+ * them on the device: the picture is cropped by crop = {left, right, top,
+ * bottom} (not at all when the margins leave no sensel, as dng.rs writes no
+ * DefaultCrop then), then turned as EXIF orientation 1..8 says to show it
+ * upright (6 = turned 90 degrees clockwise). ---- */
+/* Size after crop and orientation (PRR_E_INVALID_ARGUMENT for an
+ * orientation outside 1..8 or an empty picture). */
+int prr_oriented_size(uint32_t width, uint32_t height, const uint8_t crop[4], int orientation,
+                      uint32_t *out_width, uint32_t *out_height);
+/* BayerPattern code (0 RGGB, 1 GRBG, 2 GBRG, 3 BGGR) of a CFA picture with
+ * pattern `bayer` after prr_orient_cfa_device; -1 when invalid. */
+int prr_oriented_bayer(int bayer, const uint8_t crop[4], uint32_t width, uint32_t height,
+                       int orientation);
+/* The three develop planes (width x height at in_stride floats a row) to
+ * three output planes of prr_oriented_size at out_stride; enqueued on
+ * `stream`. Inputs and outputs must not overlap. */
+int prr_orient_planes_device(prr_context *ctx, const float *d_r, const float *d_g,
+                             const float *d_b, uint32_t width, uint32_t height,
+                             uint32_t in_stride, const uint8_t crop[4], int orientation,
+                             float *d_out_r, float *d_out_g, float *d_out_b,
+                             uint32_t out_stride, void *stream, prr_error *err);
+/* A CFA picture (code or linear values) likewise. */
+int prr_orient_cfa_device(prr_context *ctx, const uint16_t *d_in, uint32_t width,
+                          uint32_t height, uint32_t in_stride, const uint8_t crop[4],
+                          int orientation, uint16_t *d_out, uint32_t out_stride, void *stream,
+                          prr_error *err);
+
 /* ---- batches (no counterpart in the crate, which decodes one frame per
  * call): many frames of any size, version, alignment and qscale decoded
  * in one persistent launch whose waves pull tile groups from a device

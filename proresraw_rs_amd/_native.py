@@ -168,6 +168,11 @@ SIGNATURES = {
     "prr_dng_lj92_tiles_device": (i32, [c_vp, c_vp, u32, u32, u32, c_vp, sz, c_vp, c_vp, c_vp, E]),
     "prr_dng_lj92_tiles": (i32, [c_vp, c_vp, u32, u32, u32, c_vp, sz, c_vp, c_vp, sz,
                                  ctypes.POINTER(sz), E]),
+    "prr_oriented_size": (i32, [u32, u32, c_vp, i32, ctypes.POINTER(u32), ctypes.POINTER(u32)]),
+    "prr_oriented_bayer": (i32, [i32, c_vp, u32, u32, i32]),
+    "prr_orient_planes_device": (i32, [c_vp, c_vp, c_vp, c_vp, u32, u32, u32, c_vp, i32, c_vp,
+                                       c_vp, c_vp, u32, c_vp, E]),
+    "prr_orient_cfa_device": (i32, [c_vp, c_vp, u32, u32, u32, c_vp, i32, c_vp, u32, c_vp, E]),
     "prr_batch_create": (i32, [c_vp, ctypes.POINTER(c_vp), E]),
     "prr_batch_add": (i32, [c_vp, H, FI, ctypes.POINTER(PrrBatchFrame), E]),
     "prr_batch_set_config": (i32, [c_vp, u32, u32, u32, E]),

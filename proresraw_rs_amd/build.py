@@ -1,7 +1,7 @@
 """Builds the native libraries of proresraw_rs_amd in-tree.
 
 - ``_lib/libproresraw_amd.so``: the HIP kernels (decode, develop, learned
-  demosaic, DNG tile coding), the C-ABI host layer and the movie
+  demosaic, DNG tile coding, crop and orientation), the C-ABI host layer and the movie
   sample-table reader,
   compiled for gfx950 with hipcc;
 - ``_lib/libprr_synth.so``: the synthetic frame generator (plain C).
@@ -42,8 +42,8 @@ def build_hip(verbose=True):
     os.makedirs(LIB, exist_ok=True)
     out = os.path.join(LIB, "libproresraw_amd.so")
     srcs = [os.path.join(CSRC, n) for n in
-            ("prr_kernels.hip", "prr_net.hip", "prr_dng.hip", "prr_host.cpp", "prr_net_host.cpp",
-             "prr_mov.cpp")]
+            ("prr_kernels.hip", "prr_net.hip", "prr_dng.hip", "prr_orient.hip", "prr_host.cpp",
+             "prr_net_host.cpp", "prr_mov.cpp")]
     tmp = out + ".tmp"
     _run(["/opt/rocm/bin/hipcc", *HIPCC_FLAGS, *srcs, "-o", tmp], verbose)
     os.replace(tmp, out)

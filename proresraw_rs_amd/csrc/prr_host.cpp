@@ -888,6 +888,146 @@ extern "C" int prr_dng_lj92_tiles(prr_context *ctx, const uint16_t *values, uint
     return PRR_OK;
 }
 
+/* ------------------------------------------------------------------ */
+/* Crop and orientation (mov.rs:121-157, header.rs:98-109, dng.rs:190-196). */
+
+namespace {
+
+struct Cropped {
+    uint32_t left, top, cw, ch, ow, oh;
+};
+
+/* The crop is applied only when it leaves a picture, as dng.rs:190-196
+ * writes DefaultCrop only then. */
+bool crop_and_orient(uint32_t width, uint32_t height, const uint8_t *crop, int orientation,
+                     Cropped &c)
+{
+    if (orientation < 1 || orientation > 8 || width == 0 || height == 0)
+        return false;
+    c.left = c.top = 0;
+    c.cw = width;
+    c.ch = height;
+    if (crop) {
+        uint32_t h_margin = (uint32_t)crop[0] + crop[1], v_margin = (uint32_t)crop[2] + crop[3];
+        if (width > h_margin && height > v_margin) {
+            c.left = crop[0];
+            c.top = crop[2];
+            c.cw = width - h_margin;
+            c.ch = height - v_margin;
+        }
+    }
+    const bool swap = orientation >= 5;
+    c.ow = swap ? c.ch : c.cw;
+    c.oh = swap ? c.cw : c.ch;
+    return true;
+}
+
+void source_pixel(int orient, uint32_t cw, uint32_t ch, uint32_t x, uint32_t y, uint32_t &sx,
+                  uint32_t &sy)
+{
+    switch (orient) {
+    case 2: sx = cw - 1 - x; sy = y; break;
+    case 3: sx = cw - 1 - x; sy = ch - 1 - y; break;
+    case 4: sx = x; sy = ch - 1 - y; break;
+    case 5: sx = y; sy = x; break;
+    case 6: sx = y; sy = ch - 1 - x; break;
+    case 7: sx = cw - 1 - y; sy = ch - 1 - x; break;
+    case 8: sx = cw - 1 - y; sy = x; break;
+    default: sx = x; sy = y;
+    }
+}
+
+} // namespace
+
+extern "C" int prr_oriented_size(uint32_t width, uint32_t height, const uint8_t crop[4],
+                                 int orientation, uint32_t *out_width, uint32_t *out_height)
+{
+    Cropped c;
+    if (!out_width || !out_height || !crop_and_orient(width, height, crop, orientation, c))
+        return PRR_E_INVALID_ARGUMENT;
+    *out_width = c.ow;
+    *out_height = c.oh;
+    return PRR_OK;
+}
+
+extern "C" int prr_oriented_bayer(int bayer, const uint8_t crop[4], uint32_t width,
+                                  uint32_t height, int orientation)
+{
+    Cropped c;
+    if (bayer < 0 || bayer > 3 || !crop_and_orient(width, height, crop, orientation, c) ||
+        c.cw < 2 || c.ch < 2)
+        return -1;
+    /* BayerPattern::red_position (header.rs:66-73) is (code >> 1, code & 1). */
+    const uint32_t red_row = (uint32_t)bayer >> 1, red_col = (uint32_t)bayer & 1;
+    for (uint32_t y = 0; y < 2; y++)
+        for (uint32_t x = 0; x < 2; x++) {
+            uint32_t sx, sy;
+            source_pixel(orientation, c.cw, c.ch, x, y, sx, sy);
+            if (((c.top + sy) & 1) == red_row && ((c.left + sx) & 1) == red_col)
+                return (int)(2 * y + x);
+        }
+    return -1;
+}
+
+static int orient_launch(prr_context *ctx, const void *const in[3], void *const out[3],
+                         uint32_t planes, int element_bytes, uint32_t width, uint32_t height,
+                         uint32_t in_stride, const uint8_t crop[4], int orientation,
+                         uint32_t out_stride, void *stream, prr_error *err)
+{
+    if (!ctx)
+        return fail(err, PRR_E_INVALID_ARGUMENT, "null argument");
+    for (uint32_t p = 0; p < planes; p++)
+        if (!in[p] || !out[p])
+            return fail(err, PRR_E_INVALID_ARGUMENT, "null plane");
+    Cropped c;
+    if (!crop_and_orient(width, height, crop, orientation, c))
+        return fail(err, PRR_E_INVALID_ARGUMENT, "size or orientation", width, (uint64_t)orientation);
+    if (in_stride < width || out_stride < c.ow)
+        return fail(err, PRR_E_INVALID_ARGUMENT, "stride", in_stride, out_stride);
+    HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+    OrientDesc d;
+    std::memset(&d, 0, sizeof d);
+    for (uint32_t p = 0; p < planes; p++) {
+        d.in[p] = in[p];
+        d.out[p] = out[p];
+    }
+    d.planes = planes;
+    d.orient = (uint32_t)orientation;
+    d.cw = c.cw;
+    d.ch = c.ch;
+    d.ow = c.ow;
+    d.oh = c.oh;
+    d.left = c.left;
+    d.top = c.top;
+    d.in_stride = in_stride;
+    d.out_stride = out_stride;
+    HIP_TRY(launch_orient(d, element_bytes, (hipStream_t)stream), "orient kernel");
+    return PRR_OK;
+}
+
+extern "C" int prr_orient_planes_device(prr_context *ctx, const float *d_r, const float *d_g,
+                                        const float *d_b, uint32_t width, uint32_t height,
+                                        uint32_t in_stride, const uint8_t crop[4], int orientation,
+                                        float *d_out_r, float *d_out_g, float *d_out_b,
+                                        uint32_t out_stride, void *stream, prr_error *err)
+{
+    const void *in[3] = {d_r, d_g, d_b};
+    void *out[3] = {d_out_r, d_out_g, d_out_b};
+    return orient_launch(ctx, in, out, 3, 4, width, height, in_stride, crop, orientation,
+                         out_stride, stream, err);
+}
+
+extern "C" int prr_orient_cfa_device(prr_context *ctx, const uint16_t *d_in, uint32_t width,
+                                     uint32_t height, uint32_t in_stride, const uint8_t crop[4],
+                                     int orientation, uint16_t *d_out, uint32_t out_stride,
+                                     void *stream, prr_error *err)
+{
+    const void *in[3] = {d_in, nullptr, nullptr};
+    void *out[3] = {d_out, nullptr, nullptr};
+    return orient_launch(ctx, in, out, 1, 2, width, height, in_stride, crop, orientation,
+                         out_stride, stream, err);
+}
+
 extern "C" int prr_batch_create(prr_context *ctx, prr_batch **out, prr_error *err)
 {
     if (!ctx || !out)

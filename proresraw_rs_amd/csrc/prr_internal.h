@@ -97,6 +97,19 @@ struct DecodeLaunch {
 
 /* Status reset + size-table scan + tile order (k_prepare), then the
  * decode; `queue` is device scratch of nb_tiles uint32. */
+/* Crop + orientation of planes or a CFA picture (prr_orient.hip). */
+struct OrientDesc {
+    const void *in[3];
+    void *out[3];
+    uint32_t planes;      /* 3 planes of float, or 1 plane of uint16 */
+    uint32_t orient;      /* EXIF 1..8 */
+    uint32_t cw, ch;      /* cropped size */
+    uint32_t ow, oh;      /* output size (cw x ch, swapped for 5..8) */
+    uint32_t left, top;   /* crop origin in the input */
+    uint32_t in_stride, out_stride;
+};
+hipError_t launch_orient(const OrientDesc &d, int element_bytes, hipStream_t s);
+
 /* Lossless JPEG tiles of a DNG (prr_dng.hip). */
 enum { LJ92_TILE = 256 };
 /* Raw (unstuffed) entropy-coded words a tile may need: 31 bits a sample at
