@@ -221,6 +221,39 @@ int prr_develop_linear_device(prr_context *ctx, const prr_header *hdr, const uin
                               float wb_blue, int stage, float *d_r, float *d_g, float *d_b,
                               void *stream, prr_error *err);
 
+/* ---- frame pipeline (no counterpart in the crate, whose callers decode
+ * and develop one frame per call): a stream of frames whose packets are in
+ * device memory, decoded (with the linearize fused) on one HIP stream and
+ * developed from the linear picture on a second, so frame i + 1's decode
+ * (a latency-bound entropy chain) runs beside frame i's develop (VALU
+ * bound). The pipeline owns `depth` sets of intermediate buffers (code
+ * values, linear values, tile flags, status) used in turn; a decode waits
+ * on the device for the develop that last used its set. Each frame's planes
+ * equal prr_develop_frame_device's after prr_decode_frame_device. ---- */
+typedef struct prr_pipeline prr_pipeline;
+/* depth: buffer sets, 2..8 (0: 3). */
+int prr_pipeline_create(prr_context *ctx, uint32_t depth, prr_pipeline **out, prr_error *err);
+/* Enqueue one frame (packet rules of prr_decode_frame_device; the packet
+ * must stay valid until the pipeline is joined or synchronized): decode with
+ * `idct`, then develop(&frame, stage) (explicit_wb as
+ * prr_develop_frame_device) into the planes. d_status_out, when not NULL,
+ * receives the frame's prr_device_status after its develop. Returns at once. */
+int prr_pipeline_submit(prr_pipeline *p, const prr_header *hdr, const prr_frame_info *info,
+                        const uint8_t *d_packet, size_t len, int idct, int explicit_wb,
+                        float wb_red, float wb_blue, int stage, float *d_r, float *d_g,
+                        float *d_b, prr_device_status *d_status_out, prr_error *err);
+/* Make `stream` wait, on the device, for every frame submitted so far. */
+int prr_pipeline_join(prr_pipeline *p, void *stream, prr_error *err);
+/* Wait on the host for every frame submitted so far. */
+int prr_pipeline_sync(prr_pipeline *p, prr_error *err);
+/* Per-kernel timing with HIP events on the pipeline's own streams: returns
+ * the mean decode (prepare + decode) and develop milliseconds of the frames
+ * submitted since the last call (waiting for them), resets, and times the
+ * frames submitted from now on when `enable` is set. */
+int prr_pipeline_timing(prr_pipeline *p, int enable, float *decode_ms, float *develop_ms,
+                        uint64_t *frames, prr_error *err);
+void prr_pipeline_destroy(prr_pipeline *p);
+
 /* ---- DNG raw tiles: the LosslessJpeg branch of the DNG writer
  * (examples/prr2dng/dng.rs:92-113): every 256x256 tile of a width x height
  * picture, padded as dng.rs `tile` pads it (sensels past the right or

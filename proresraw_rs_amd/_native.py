@@ -164,6 +164,15 @@ SIGNATURES = {
                                        i32, c_vp, c_vp, c_vp, c_vp, E]),
     "prr_develop_linear_device": (i32, [c_vp, H, c_vp, c_vp, i32, ctypes.c_float, ctypes.c_float,
                                         i32, c_vp, c_vp, c_vp, c_vp, E]),
+    "prr_pipeline_create": (i32, [c_vp, u32, ctypes.POINTER(c_vp), E]),
+    "prr_pipeline_submit": (i32, [c_vp, H, FI, c_vp, sz, i32, i32, ctypes.c_float, ctypes.c_float,
+                                  i32, c_vp, c_vp, c_vp, c_vp, E]),
+    "prr_pipeline_join": (i32, [c_vp, c_vp, E]),
+    "prr_pipeline_sync": (i32, [c_vp, E]),
+    "prr_pipeline_timing": (i32, [c_vp, i32, ctypes.POINTER(ctypes.c_float),
+                                  ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_uint64),
+                                  E]),
+    "prr_pipeline_destroy": (None, [c_vp]),
     "prr_dng_lj92_capacity": (sz, [u32, u32]),
     "prr_dng_lj92_tiles_device": (i32, [c_vp, c_vp, u32, u32, u32, c_vp, sz, c_vp, c_vp, c_vp, E]),
     "prr_dng_lj92_tiles": (i32, [c_vp, c_vp, u32, u32, u32, c_vp, sz, c_vp, c_vp, sz,

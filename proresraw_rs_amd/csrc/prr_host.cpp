@@ -801,6 +801,219 @@ struct prr_batch {
 };
 
 /* ------------------------------------------------------------------ */
+/* Frame pipeline: decode of frame i + 1 beside develop of frame i. */
+
+namespace {
+struct PipeSet {
+    DevBuf samples, linear, flags, status;
+    hipEvent_t decoded = nullptr, developed = nullptr;
+    hipEvent_t t0 = nullptr, t1 = nullptr, t2 = nullptr, t3 = nullptr; /* timing */
+    bool used = false, timed = false;
+};
+} // namespace
+
+struct prr_pipeline {
+    prr_context *ctx = nullptr;   /* the caller's: device and launch shape */
+    prr_context *dctx = nullptr;  /* decode stream and the decode's scratch */
+    hipStream_t develop = nullptr;
+    std::vector<PipeSet> sets;
+    uint64_t submitted = 0;
+    bool timing = false;
+    double decode_ms = 0, develop_ms = 0;
+    uint64_t timed_frames = 0;
+};
+
+static void pipeline_release(prr_pipeline *p)
+{
+    if (p->dctx)
+        (void)hipStreamSynchronize(p->dctx->stream);
+    if (p->develop)
+        (void)hipStreamSynchronize(p->develop);
+    for (PipeSet &st : p->sets) {
+        for (DevBuf *b : {&st.samples, &st.linear, &st.flags, &st.status})
+            if (b->ptr)
+                (void)hipFree(b->ptr);
+        for (hipEvent_t e : {st.decoded, st.developed, st.t0, st.t1, st.t2, st.t3})
+            if (e)
+                (void)hipEventDestroy(e);
+    }
+    if (p->develop)
+        (void)hipStreamDestroy(p->develop);
+    if (p->dctx)
+        prr_context_destroy(p->dctx);
+    delete p;
+}
+
+extern "C" int prr_pipeline_create(prr_context *ctx, uint32_t depth, prr_pipeline **out,
+                                   prr_error *err)
+{
+    if (!ctx || !out)
+        return fail(err, PRR_E_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    if (depth == 0)
+        depth = 3;
+    if (depth < 2 || depth > 8)
+        return fail(err, PRR_E_INVALID_ARGUMENT, "pipeline depth", depth);
+    HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+    prr_pipeline *p = new prr_pipeline();
+    p->ctx = ctx;
+    p->sets.resize(depth);
+    int rc = prr_context_create(ctx->device, &p->dctx, err);
+    if (rc) {
+        pipeline_release(p);
+        return rc;
+    }
+    p->dctx->launch = ctx->launch;
+    hipError_t e = hipStreamCreateWithFlags(&p->develop, hipStreamNonBlocking);
+    for (PipeSet &st : p->sets) {
+        for (hipEvent_t *ev : {&st.decoded, &st.developed})
+            if (e == hipSuccess)
+                e = hipEventCreateWithFlags(ev, hipEventDisableTiming);
+        for (hipEvent_t *ev : {&st.t0, &st.t1, &st.t2, &st.t3})
+            if (e == hipSuccess)
+                e = hipEventCreate(ev);
+    }
+    if (e != hipSuccess) {
+        pipeline_release(p);
+        return device_fail(err, e, "pipeline streams");
+    }
+    *out = p;
+    return PRR_OK;
+}
+
+extern "C" void prr_pipeline_destroy(prr_pipeline *p)
+{
+    if (p)
+        pipeline_release(p);
+}
+
+/* Averages of the frames timed since timing was switched on. */
+static int pipeline_collect(prr_pipeline *p, prr_error *err)
+{
+    for (PipeSet &st : p->sets) {
+        if (!st.timed)
+            continue;
+        HIP_TRY(hipEventSynchronize(st.t3), "hipEventSynchronize");
+        float a = 0, b = 0;
+        HIP_TRY(hipEventElapsedTime(&a, st.t0, st.t1), "hipEventElapsedTime");
+        HIP_TRY(hipEventElapsedTime(&b, st.t2, st.t3), "hipEventElapsedTime");
+        p->decode_ms += a;
+        p->develop_ms += b;
+        p->timed_frames++;
+        st.timed = false;
+    }
+    return PRR_OK;
+}
+
+extern "C" int prr_pipeline_submit(prr_pipeline *p, const prr_header *hdr,
+                                   const prr_frame_info *info, const uint8_t *d_packet, size_t len,
+                                   int idct, int explicit_wb, float wb_red, float wb_blue,
+                                   int stage, float *d_r, float *d_g, float *d_b,
+                                   prr_device_status *d_status_out, prr_error *err)
+{
+    if (!p || !hdr || !info || !d_packet || !d_r)
+        return fail(err, PRR_E_INVALID_ARGUMENT, "null argument");
+    if ((d_g == nullptr) != (d_b == nullptr))
+        return fail(err, PRR_E_INVALID_ARGUMENT, "planes");
+    if (stage < PRR_STAGE_DEMOSAICED || stage > PRR_STAGE_REC2020)
+        return fail(err, PRR_E_INVALID_ARGUMENT, "stage", (uint64_t)stage);
+    HIP_TRY(hipSetDevice(p->ctx->device), "hipSetDevice");
+    PipeSet &st = p->sets[p->submitted % p->sets.size()];
+    hipStream_t ds = p->dctx->stream, vs = p->develop;
+    int rc;
+    if (p->timing && st.timed && (rc = pipeline_collect(p, err)))
+        return rc;
+    const size_t coded = (size_t)info->coded_width * info->coded_height;
+    const size_t visible = (size_t)hdr->width * hdr->height;
+    if (st.samples.bytes < coded * 2 || st.linear.bytes < visible * 2 ||
+        st.flags.bytes < std::max<size_t>(info->nb_tiles, 1)) {
+        /* Growing a set: its last frame must be done with it. */
+        if (st.used)
+            HIP_TRY(hipEventSynchronize(st.developed), "hipEventSynchronize");
+        if ((rc = ensure(p->dctx, st.samples, coded * 2, err)) ||
+            (rc = ensure(p->dctx, st.linear, visible * 2, err)) ||
+            (rc = ensure(p->dctx, st.flags, std::max<size_t>(info->nb_tiles, 1), err)))
+            return rc;
+    }
+    if ((rc = ensure(p->dctx, st.status, sizeof(prr_device_status), err)))
+        return rc;
+    /* The decode waits (on the device) for the develop that last read this
+     * set; the develop waits for this decode. */
+    if (st.used)
+        HIP_TRY(hipStreamWaitEvent(ds, st.developed, 0), "hipStreamWaitEvent");
+    const bool timed = p->timing;
+    if (timed)
+        HIP_TRY(hipEventRecord(st.t0, ds), "hipEventRecord");
+    if ((rc = prr_decode_frame_device(p->dctx, hdr, info, d_packet, len, idct,
+                                      (uint16_t *)st.samples.ptr, (uint16_t *)st.linear.ptr,
+                                      (uint8_t *)st.flags.ptr, (prr_device_status *)st.status.ptr,
+                                      ds, err)))
+        return rc;
+    if (timed)
+        HIP_TRY(hipEventRecord(st.t1, ds), "hipEventRecord");
+    HIP_TRY(hipEventRecord(st.decoded, ds), "hipEventRecord");
+    HIP_TRY(hipStreamWaitEvent(vs, st.decoded, 0), "hipStreamWaitEvent");
+    if (timed)
+        HIP_TRY(hipEventRecord(st.t2, vs), "hipEventRecord");
+    if ((rc = prr_develop_linear_device(p->dctx, hdr, (const uint16_t *)st.linear.ptr,
+                                        (const prr_device_status *)st.status.ptr, explicit_wb,
+                                        wb_red, wb_blue, stage, d_r, d_g, d_b, vs, err)))
+        return rc;
+    if (timed)
+        HIP_TRY(hipEventRecord(st.t3, vs), "hipEventRecord");
+    if (d_status_out)
+        HIP_TRY(launch_copy_words((const uint32_t *)st.status.ptr, (uint32_t *)d_status_out,
+                                  sizeof(prr_device_status) / 4, vs),
+                "status copy");
+    HIP_TRY(hipEventRecord(st.developed, vs), "hipEventRecord");
+    st.used = true;
+    st.timed = timed;
+    p->submitted++;
+    return PRR_OK;
+}
+
+extern "C" int prr_pipeline_join(prr_pipeline *p, void *stream, prr_error *err)
+{
+    if (!p)
+        return fail(err, PRR_E_INVALID_ARGUMENT, "pipeline");
+    HIP_TRY(hipSetDevice(p->ctx->device), "hipSetDevice");
+    for (PipeSet &st : p->sets)
+        if (st.used)
+            HIP_TRY(hipStreamWaitEvent((hipStream_t)stream, st.developed, 0), "hipStreamWaitEvent");
+    return PRR_OK;
+}
+
+extern "C" int prr_pipeline_sync(prr_pipeline *p, prr_error *err)
+{
+    if (!p)
+        return fail(err, PRR_E_INVALID_ARGUMENT, "pipeline");
+    HIP_TRY(hipSetDevice(p->ctx->device), "hipSetDevice");
+    HIP_TRY(hipStreamSynchronize(p->develop), "hipStreamSynchronize");
+    HIP_TRY(hipStreamSynchronize(p->dctx->stream), "hipStreamSynchronize");
+    return PRR_OK;
+}
+
+extern "C" int prr_pipeline_timing(prr_pipeline *p, int enable, float *decode_ms,
+                                   float *develop_ms, uint64_t *frames, prr_error *err)
+{
+    if (!p)
+        return fail(err, PRR_E_INVALID_ARGUMENT, "pipeline");
+    int rc = pipeline_collect(p, err);
+    if (rc)
+        return rc;
+    if (decode_ms)
+        *decode_ms = p->timed_frames ? (float)(p->decode_ms / (double)p->timed_frames) : 0.0f;
+    if (develop_ms)
+        *develop_ms = p->timed_frames ? (float)(p->develop_ms / (double)p->timed_frames) : 0.0f;
+    if (frames)
+        *frames = p->timed_frames;
+    p->decode_ms = p->develop_ms = 0;
+    p->timed_frames = 0;
+    p->timing = enable != 0;
+    return PRR_OK;
+}
+
+/* ------------------------------------------------------------------ */
 /* Lossless JPEG tiles of a DNG (dng.rs:92-128, lj92.rs:14-93). */
 
 static uint64_t lj92_tiles(uint32_t width, uint32_t height)

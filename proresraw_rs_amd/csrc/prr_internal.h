@@ -109,6 +109,8 @@ struct OrientDesc {
     uint32_t in_stride, out_stride;
 };
 hipError_t launch_orient(const OrientDesc &d, int element_bytes, hipStream_t s);
+/* n <= 64 words copied by a kernel on stream s. */
+hipError_t launch_copy_words(const uint32_t *src, uint32_t *dst, uint32_t n, hipStream_t s);
 
 /* Lossless JPEG tiles of a DNG (prr_dng.hip). */
 enum { LJ92_TILE = 256 };

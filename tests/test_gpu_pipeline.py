@@ -1,0 +1,56 @@
+"""The frame pipeline (prr_pipeline_*): frames of different sizes,
+patterns and qscales submitted back to back, decoded on one stream and
+developed on another through a ring of buffer sets, give planes bit-equal to
+the oracle's develop of the oracle's decode (within the 1e-5 relative bar;
+exact in practice) and each frame's status."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from proresraw_rs_amd import synth as S
+from proresraw_rs_amd.workloads import config1
+
+pytestmark = pytest.mark.gpu
+
+
+def test_pipelined_frames_match_the_oracle(gpu):
+    import torch
+    P = gpu
+    from proresraw_rs_amd.pipeline import Pipeline
+    scenes = [S.scene(256, 128, 0, seed=61, version=0, alignment=2, qscale=(15, 15)),
+              config1(200, 120, seed=62),
+              S.scene(384, 160, 3, seed=63, version=1, alignment=4, qscale=(40, 40)),  # larger
+              config1(256, 136, seed=64)]
+    packets = [S.encode_scene(sc) for sc in scenes]
+    frames = []
+    for pk in packets:
+        hdr, info = P.parse_frame_info(pk)
+        pad = ((len(pk) + 3) & ~3) + 16
+        d = torch.zeros(pad, dtype=torch.uint8, device="cuda")
+        d[:len(pk)] = torch.frombuffer(bytearray(pk), dtype=torch.uint8).cuda()
+        frames.append((pk, hdr, info, d))
+    pipe = Pipeline(depth=2)
+    order = [0, 1, 0, 2, 3, 1, 2, 2, 0, 3]
+    outs = []
+    for i, f in enumerate(order):
+        pk, hdr, info, d = frames[f]
+        planes = [torch.full((hdr.width * hdr.height,), -7.0, device="cuda") for _ in range(3)]
+        status = torch.zeros(4, dtype=torch.int32, device="cuda")
+        print(f"pipeline frame {i}: scene {f} {hdr.width}x{hdr.height}", flush=True)
+        pipe.submit(hdr, info, d, len(pk), planes, stage=P.Stage.Rec2020, status=status)
+        outs.append((f, planes, status))
+    pipe.sync()
+    want = {}
+    for f, planes, status in outs:
+        pk = frames[f][0]
+        if f not in want:
+            h, samples, skipped, mq = O.decode_frame(pk)
+            want[f] = (O.develop_planar(h, samples, O.params(h, h.quant_matrix[0] * mq), 2), mq)
+        ref, mq = want[f]
+        st = status.cpu().numpy().view(np.uint32)
+        assert st[0] == 0xFFFFFFFF and st[1] == mq and st[3] == 0
+        for p, r in zip(planes, ref):
+            got = p.cpu().numpy()
+            rel = float(np.abs(got - r.reshape(-1)).max()) / max(float(np.abs(r).max()), 1e-30)
+            assert rel <= 1e-5
+    pipe.close()
