@@ -844,6 +844,22 @@ static void pipeline_release(prr_pipeline *p)
     delete p;
 }
 
+/* A stream on a hardware queue of its own: the runtime shares its few
+ * hardware queues among ordinary streams (two pipeline streams can land on
+ * one queue and then run one after the other), while a stream created with a
+ * CU mask gets a queue to itself. The mask enables every CU. */
+static hipError_t dedicated_stream(int device, hipStream_t *out)
+{
+    hipDeviceProp_t prop;
+    hipError_t e = hipGetDeviceProperties(&prop, device);
+    if (e != hipSuccess)
+        return e;
+    std::vector<uint32_t> mask(((size_t)prop.multiProcessorCount + 31) / 32, 0xFFFFFFFFu);
+    if (prop.multiProcessorCount % 32)
+        mask.back() = (1u << (prop.multiProcessorCount % 32)) - 1u;
+    return hipExtStreamCreateWithCUMask(out, (uint32_t)mask.size(), mask.data());
+}
+
 extern "C" int prr_pipeline_create(prr_context *ctx, uint32_t depth, prr_pipeline **out,
                                    prr_error *err)
 {
@@ -864,7 +880,15 @@ extern "C" int prr_pipeline_create(prr_context *ctx, uint32_t depth, prr_pipelin
         return rc;
     }
     p->dctx->launch = ctx->launch;
-    hipError_t e = hipStreamCreateWithFlags(&p->develop, hipStreamNonBlocking);
+    /* The decode and develop streams on hardware queues of their own, so
+     * they run side by side. */
+    hipStream_t decode_stream = nullptr;
+    hipError_t e = dedicated_stream(ctx->device, &decode_stream);
+    if (e == hipSuccess) {
+        (void)hipStreamDestroy(p->dctx->stream);
+        p->dctx->stream = decode_stream;
+        e = dedicated_stream(ctx->device, &p->develop);
+    }
     for (PipeSet &st : p->sets) {
         for (hipEvent_t *ev : {&st.decoded, &st.developed})
             if (e == hipSuccess)
