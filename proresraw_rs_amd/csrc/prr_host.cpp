@@ -524,11 +524,27 @@ extern "C" int prr_develop_frame_device(prr_context *ctx, const prr_header *hdr,
     return PRR_OK;
 }
 
+/* develop(&frame, stage) from the linear picture; the develop kernel also
+ * copies the frame's status to status_out when that is set. */
+static int develop_linear(prr_context *ctx, const prr_header *hdr, const uint16_t *d_linear,
+                          const prr_device_status *d_status, int explicit_wb, float wb_red,
+                          float wb_blue, int stage, float *d_r, float *d_g, float *d_b,
+                          prr_device_status *status_out, void *stream, prr_error *err);
+
 extern "C" int prr_develop_linear_device(prr_context *ctx, const prr_header *hdr,
                                          const uint16_t *d_linear,
                                          const prr_device_status *d_status, int explicit_wb,
                                          float wb_red, float wb_blue, int stage, float *d_r,
                                          float *d_g, float *d_b, void *stream, prr_error *err)
+{
+    return develop_linear(ctx, hdr, d_linear, d_status, explicit_wb, wb_red, wb_blue, stage, d_r,
+                          d_g, d_b, nullptr, stream, err);
+}
+
+static int develop_linear(prr_context *ctx, const prr_header *hdr, const uint16_t *d_linear,
+                          const prr_device_status *d_status, int explicit_wb, float wb_red,
+                          float wb_blue, int stage, float *d_r, float *d_g, float *d_b,
+                          prr_device_status *status_out, void *stream, prr_error *err)
 {
     if (!ctx || !hdr || !d_linear || !d_status || !d_r)
         return fail(err, PRR_E_INVALID_ARGUMENT, "null argument");
@@ -544,6 +560,7 @@ extern "C" int prr_develop_linear_device(prr_context *ctx, const prr_header *hdr
                                         stage);
     d.stride = hdr->width;
     d.linear_input = 1;
+    d.status_out = status_out;
     HIP_TRY(launch_develop(d, d_r, d_g, d_b, s), "develop kernel");
     return PRR_OK;
 }
@@ -979,16 +996,12 @@ extern "C" int prr_pipeline_submit(prr_pipeline *p, const prr_header *hdr,
     HIP_TRY(hipStreamWaitEvent(vs, st.decoded, 0), "hipStreamWaitEvent");
     if (timed)
         HIP_TRY(hipEventRecord(st.t2, vs), "hipEventRecord");
-    if ((rc = prr_develop_linear_device(p->dctx, hdr, (const uint16_t *)st.linear.ptr,
-                                        (const prr_device_status *)st.status.ptr, explicit_wb,
-                                        wb_red, wb_blue, stage, d_r, d_g, d_b, vs, err)))
+    if ((rc = develop_linear(p->dctx, hdr, (const uint16_t *)st.linear.ptr,
+                             (const prr_device_status *)st.status.ptr, explicit_wb, wb_red,
+                             wb_blue, stage, d_r, d_g, d_b, d_status_out, vs, err)))
         return rc;
     if (timed)
         HIP_TRY(hipEventRecord(st.t3, vs), "hipEventRecord");
-    if (d_status_out)
-        HIP_TRY(launch_copy_words((const uint32_t *)st.status.ptr, (uint32_t *)d_status_out,
-                                  sizeof(prr_device_status) / 4, vs),
-                "status copy");
     HIP_TRY(hipEventRecord(st.developed, vs), "hipEventRecord");
     st.used = true;
     st.timed = timed;

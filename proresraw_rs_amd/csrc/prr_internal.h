@@ -56,6 +56,9 @@ struct DevelopDesc {
     /* When set, `samples` holds linearized values (Frame::linearize) and the
      * curve is not applied again. */
     uint32_t linear_input;
+    /* When set, the first workgroup copies *qstatus here (a pipeline
+     * frame's status handed to its caller without a dispatch of its own). */
+    prr_device_status *status_out;
 };
 
 enum BatchMode : uint32_t {
@@ -109,8 +112,6 @@ struct OrientDesc {
     uint32_t in_stride, out_stride;
 };
 hipError_t launch_orient(const OrientDesc &d, int element_bytes, hipStream_t s);
-/* n <= 64 words copied by a kernel on stream s. */
-hipError_t launch_copy_words(const uint32_t *src, uint32_t *dst, uint32_t n, hipStream_t s);
 
 /* Lossless JPEG tiles of a DNG (prr_dng.hip). */
 enum { LJ92_TILE = 256 };

@@ -1600,6 +1600,9 @@ __global__ __launch_bounds__(256) void k_develop(DevelopDesc d, float *out_r, fl
     __shared__ DevelopLds lds;
     if (d.gate && *d.gate == 0)
         return;
+    if (d.status_out && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x < 4)
+        reinterpret_cast<uint32_t *>(d.status_out)[threadIdx.x] =
+            reinterpret_cast<const uint32_t *>(d.qstatus)[threadIdx.x];
     develop_tile(d, out_r, out_g, out_b, blockIdx.x, blockIdx.y, lds, 0, 0, d.width);
 }
 

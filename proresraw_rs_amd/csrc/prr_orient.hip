@@ -92,22 +92,6 @@ __global__ void __launch_bounds__(OLANES) k_orient(OrientDesc d)
     }
 }
 
-/* A few words copied by one wave (a frame's status out of a pipeline set):
- * a kernel on the stream, so the copy keeps the stream on the compute
- * queue instead of a copy engine. */
-__global__ void k_copy_words(const uint32_t *__restrict__ src, uint32_t *__restrict__ dst,
-                             uint32_t n)
-{
-    if (threadIdx.x < n)
-        dst[threadIdx.x] = src[threadIdx.x];
-}
-
-hipError_t launch_copy_words(const uint32_t *src, uint32_t *dst, uint32_t n, hipStream_t s)
-{
-    hipLaunchKernelGGL(k_copy_words, dim3(1), dim3(64), 0, s, src, dst, n);
-    return hipGetLastError();
-}
-
 hipError_t launch_orient(const OrientDesc &d, int element_bytes, hipStream_t s)
 {
     dim3 grid((d.ow + OT - 1) / OT, (d.oh + OT - 1) / OT, d.planes);
