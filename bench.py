@@ -44,7 +44,7 @@ def parse_args():
     ap.add_argument("--no-clip", action="store_true")
     ap.add_argument("--no-dng", action="store_true")
     ap.add_argument("--no-orient", action="store_true")
-    ap.add_argument("--clip-frames", type=int, default=48)
+    ap.add_argument("--clip-frames", type=int, default=160)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device to device 10 times (the reference reading "
                          "of the HBM byte counters) and exit")
@@ -546,6 +546,7 @@ def clip_bench(args):
             "host_read_gb_per_s": round(nbytes / read_s / 1e9, 2) if read_s > 0 else None,
             "host_read_s": round(read_s, 4), "clip_bytes": nbytes,
             "resident_window_value": round(16 * 5760 * 3240 / (resident_ms * 1e-3) / 1e9, 3),
+            "fill_ms": round((el - frames / 16 * resident_ms * 1e-3) * 1e3, 3),
             "workload": f"{frames}-frame .mov of 5760x3240 RGGB v0 a3 q45 (8 scenes cycled): "
                         "sample table + windowed pread into pinned memory + copy + batched "
                         "decode+develop(WhiteBalanced), 16 frames a window"}

@@ -1,5 +1,5 @@
 """The frame pipeline (prr_pipeline_*): frames of different sizes,
-patterns and qscales submitted back to back, decoded on one stream and
+patterns, qscales and inverse transforms submitted back to back, decoded on one stream and
 developed on another through a ring of buffer sets, give planes bit-equal to
 the oracle's develop of the oracle's decode (within the 1e-5 relative bar;
 exact in practice) and each frame's status."""
@@ -36,17 +36,20 @@ def test_pipelined_frames_match_the_oracle(gpu):
         pk, hdr, info, d = frames[f]
         planes = [torch.full((hdr.width * hdr.height,), -7.0, device="cuda") for _ in range(3)]
         status = torch.zeros(4, dtype=torch.int32, device="cuda")
-        print(f"pipeline frame {i}: scene {f} {hdr.width}x{hdr.height}", flush=True)
-        pipe.submit(hdr, info, d, len(pk), planes, stage=P.Stage.Rec2020, status=status)
-        outs.append((f, planes, status))
+        ffmpeg = i % 3 == 1
+        print(f"pipeline frame {i}: scene {f} {hdr.width}x{hdr.height} ffmpeg={ffmpeg}", flush=True)
+        pipe.submit(hdr, info, d, len(pk), planes, stage=P.Stage.Rec2020, status=status,
+                    idct=P.Idct.Ffmpeg if ffmpeg else P.Idct.Apple)
+        outs.append((f, ffmpeg, planes, status))
     pipe.sync()
     want = {}
-    for f, planes, status in outs:
+    for f, ffmpeg, planes, status in outs:
         pk = frames[f][0]
-        if f not in want:
-            h, samples, skipped, mq = O.decode_frame(pk)
-            want[f] = (O.develop_planar(h, samples, O.params(h, h.quant_matrix[0] * mq), 2), mq)
-        ref, mq = want[f]
+        if (f, ffmpeg) not in want:
+            h, samples, skipped, mq = O.decode_frame(pk, ffmpeg=ffmpeg)
+            want[(f, ffmpeg)] = (O.develop_planar(h, samples, O.params(h, h.quant_matrix[0] * mq),
+                                                  2), mq)
+        ref, mq = want[(f, ffmpeg)]
         st = status.cpu().numpy().view(np.uint32)
         assert st[0] == 0xFFFFFFFF and st[1] == mq and st[3] == 0
         for p, r in zip(planes, ref):
