@@ -162,11 +162,23 @@ __global__ void __launch_bounds__(LANES) k_lj92_encode(const uint16_t *__restric
 
     if (t < 17)
         s.hist[t] = 0;
-    /* 1. The padded tile, one row segment of 64 sensels per wave load. */
-    for (uint32_t i = t; i < TILE * TILE; i += LANES) {
-        uint32_t y = i / TILE, x = i % TILE;
-        uint32_t sy = clamp_phase(top + y, height), sx = clamp_phase(left + x, width);
-        s.tile[i] = values[(size_t)sy * stride + sx];
+    /* 1. The padded tile: inside the picture at a 16-byte aligned stride,
+     * 16-byte loads (a wave reads two rows of 512 bytes an instruction);
+     * at the picture's edges, one sensel a lane with the CFA-phase padding. */
+    if (left + TILE <= width && top + TILE <= height && (stride & 7) == 0 &&
+        (reinterpret_cast<uintptr_t>(values) & 15) == 0) {
+        uint4 *dst = reinterpret_cast<uint4 *>(s.tile);
+        for (uint32_t i = t; i < TILE * TILE / 8; i += LANES) {
+            const uint32_t y = i / (TILE / 8), c = i % (TILE / 8);
+            dst[i] = *reinterpret_cast<const uint4 *>(values + (size_t)(top + y) * stride + left +
+                                                      8 * c);
+        }
+    } else {
+        for (uint32_t i = t; i < TILE * TILE; i += LANES) {
+            uint32_t y = i / TILE, x = i % TILE;
+            uint32_t sy = clamp_phase(top + y, height), sx = clamp_phase(left + x, width);
+            s.tile[i] = values[(size_t)sy * stride + sx];
+        }
     }
     __syncthreads();
 
