@@ -92,6 +92,11 @@ def test_pictures_match_the_oracle(gpu):
     check(smooth + rng.integers(0, 12, smooth.shape))
     check(rng.integers(250, 4096, (17, 9)))            # one small tile, odd width
     check(rng.integers(0, 4096, (513, 2)))             # two columns
+    # dng.rs:241-252's 4x2 picture: its one tile decodes to the padded tile.
+    small = np.array([[i % 4 + 10 * (i // 4) for i in range(8)]], dtype=np.uint16).reshape(2, 4)
+    data, _, _ = check(small)
+    out, *_ = decode(data)
+    assert out == O.dng_tile(small, 0, 0).reshape(-1).tolist()
     t = skewed_tile(rng)
     assert huffman_depth(category_counts(t) + [1]) > 16  # the limit is exercised
     data, offsets, lengths = check(t)

@@ -139,3 +139,15 @@ def test_frame_tiles_decode_to_their_tiles():
         out, *_ = decode(stream)
         want = O.dng_tile(v, (t % 3) * 256, (t // 3) * 256)
         assert out == want.reshape(-1).tolist()
+
+
+def test_padding_repeats_the_edge_of_the_same_colour():
+    """dng.rs:241-252: a 4x2 picture whose value is its column plus 10 x its
+    row; past the right edge the last two columns alternate, past the
+    bottom the last two rows."""
+    values = np.array([[i % 4 + 10 * (i // 4) for i in range(8)]], dtype=np.uint16).reshape(2, 4)
+    tile = O.dng_tile(values, 0, 0)
+    assert tile[0, :8].tolist() == [0, 1, 2, 3, 2, 3, 2, 3]
+    assert tile[1, :4].tolist() == [10, 11, 12, 13]
+    assert tile[2, 0] == 0
+    assert tile[3, 0] == 10
