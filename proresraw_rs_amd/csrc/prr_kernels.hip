@@ -1196,19 +1196,36 @@ __device__ inline float estimate_pre(const float *xx, const float *gg, const flo
     float mean_x = dot4(xx[0], xx[1], xx[2], xx[3], 0.25f, 0.25f, 0.25f, 0.25f);
     if (!(cs < cg))
         return mean_x;
-    bool above = true;
+    float dx[4], dg[4];
 #pragma unroll
-    for (int i = 0; i < 4; i++)
-        above = above & (sx[i] < xx[i]) & (sg[i] < gg[i]);
-    if (!above)
-        return mean_x;
+    for (int i = 0; i < 4; i++) {
+        dx[i] = xx[i] - sx[i];
+        dg[i] = gg[i] - sg[i];
+    }
+    if (IN_RANGE) {
+        /* Every operand is finite here, and for finite a, b the IEEE
+         * difference a - b (with gradual underflow) is positive exactly when
+         * b < a: the eight tests are one minimum, over the differences the
+         * quotients below need anyway. */
+        const float m = fminf(fminf(fminf(dx[0], dx[1]), fminf(dx[2], dx[3])),
+                              fminf(fminf(dg[0], dg[1]), fminf(dg[2], dg[3])));
+        if (!(m > 0.0f))
+            return mean_x;
+    } else {
+        bool above = true;
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+            above = above & (sx[i] < xx[i]) & (sg[i] < gg[i]);
+        if (!above)
+            return mean_x;
+    }
     float var_x[4], lo[4], mid[4], hi[4];
 #pragma unroll
     for (int i = 0; i < 4; i++) {
         var_x[i] = variance(xx[i], off, n0, n1, n2);
-        lo[i] = div((cg - cs) * (xx[i] - sx[i]), sg[i] + gg[i]);
+        lo[i] = div((cg - cs) * dx[i], sg[i] + gg[i]);
         mid[i] = div(cg * xx[i], gg[i]);
-        hi[i] = div((cs + cg) * (sx[i] + xx[i]), gg[i] - sg[i]);
+        hi[i] = div((cs + cg) * (sx[i] + xx[i]), dg[i]);
     }
     float spread = dot4(mid[0] - lo[0], mid[1] - lo[1], mid[2] - lo[2], mid[3] - lo[3],
                         hi[0] - mid[0], hi[1] - mid[1], hi[2] - mid[2], hi[3] - mid[3]);
@@ -1569,10 +1586,15 @@ __device__ inline void develop_tile_as(const DevelopDesc &d, const DevK &k, floa
     auto grc = [&](uint32_t y, uint32_t x) {
         return lds.gr[(int32_t)y - (y0 - 1)][((int32_t)x - (x0 - 2)) >> 1];
     };
-    for (uint32_t i = tid; i < DTH * (DTW / 2); i += 256) {
-        const uint32_t ly = i / (DTW / 2), lp = i - ly * (DTW / 2);
-        const uint32_t y = (uint32_t)y0 + ly;
-        const uint32_t xb = (uint32_t)x0 + 2 * lp;
+    /* Lane tid takes pair column tid % 32 of rows tid / 32 + 8j: the column,
+     * its clamps and (rows 8 apart) the Bayer phase stay the same for the
+     * lane, so their arithmetic is done once. */
+    static_assert(DTW / 2 == 32 && DTH % 8 == 0, "chroma lanes cover 8 rows of 32 pairs");
+    const uint32_t lp = tid % (DTW / 2), ly0 = tid / (DTW / 2);
+    const uint32_t xb = (uint32_t)x0 + 2 * lp;
+#pragma unroll
+    for (uint32_t j = 0; j < DTH / 8; j++) {
+        const uint32_t y = (uint32_t)y0 + ly0 + 8 * j;
         if (y >= h || xb >= w)
             continue;
         float res[2][3];
