@@ -1169,6 +1169,23 @@ __device__ __forceinline__ float div_in_range(float n, float d)
     return fmaf(fmaf(-d, q1, n), r1, q1);
 }
 
+/* The reciprocal refinement of div_in_range: it depends on the divisor
+ * alone, so a divisor shared by several quotients is refined once. */
+__device__ __forceinline__ float refined_rcp(float d)
+{
+    const float r = __builtin_amdgcn_rcpf(d);
+    return fmaf(fmaf(-d, r, 1.0f), r, r);
+}
+
+/* div_in_range(n, d) given r1 = refined_rcp(d): the same operations on the
+ * same values, so the same bits. */
+__device__ __forceinline__ float div_in_range_r(float n, float d, float r1)
+{
+    const float q = n * r1;
+    const float q1 = fmaf(fmaf(-d, q, n), r1, q);
+    return fmaf(fmaf(-d, q1, n), r1, q1);
+}
+
 /* sqrt(x) correctly rounded, as the general square root sequence computes
  * it (scale below 2^-96, v_sqrt, the two one-ulp corrections by FMA
  * residuals, unscale, pass zeros and +inf through) without the steps that
@@ -1189,8 +1206,8 @@ __device__ __forceinline__ float sqrt_in_range(float x)
  * chroma_in_range, and the lo / mid / hi quotients use div_in_range. */
 template <bool IN_RANGE>
 __device__ inline float estimate_pre(const float *xx, const float *gg, const float *sx,
-                                     const float *sg, float cg, float cs, float off, float n0,
-                                     float n1, float n2)
+                                     const float *sg, const float4 *rc, float cg, float cs,
+                                     float off, float n0, float n1, float n2)
 {
     auto div = [](float n, float d) { return IN_RANGE ? div_in_range(n, d) : n / d; };
     float mean_x = dot4(xx[0], xx[1], xx[2], xx[3], 0.25f, 0.25f, 0.25f, 0.25f);
@@ -1223,9 +1240,16 @@ __device__ inline float estimate_pre(const float *xx, const float *gg, const flo
 #pragma unroll
     for (int i = 0; i < 4; i++) {
         var_x[i] = variance(xx[i], off, n0, n1, n2);
-        lo[i] = div((cg - cs) * dx[i], sg[i] + gg[i]);
-        mid[i] = div(cg * xx[i], gg[i]);
-        hi[i] = div((cs + cg) * (sx[i] + xx[i]), dg[i]);
+        if (IN_RANGE) {
+            /* rc[i] = refined reciprocals of gg, sg + gg, gg - sg (x, y, z). */
+            lo[i] = div_in_range_r((cg - cs) * dx[i], sg[i] + gg[i], rc[i].y);
+            mid[i] = div_in_range_r(cg * xx[i], gg[i], rc[i].x);
+            hi[i] = div_in_range_r((cs + cg) * (sx[i] + xx[i]), dg[i], rc[i].z);
+        } else {
+            lo[i] = div((cg - cs) * dx[i], sg[i] + gg[i]);
+            mid[i] = div(cg * xx[i], gg[i]);
+            hi[i] = div((cs + cg) * (sx[i] + xx[i]), dg[i]);
+        }
     }
     float spread = dot4(mid[0] - lo[0], mid[1] - lo[1], mid[2] - lo[2], mid[3] - lo[3],
                         hi[0] - mid[0], hi[1] - mid[1], hi[2] - mid[2], hi[3] - mid[3]);
@@ -1252,8 +1276,17 @@ enum { DTW = 64, DTH = 32, NORM_W = DTW + 6, NORM_H = DTH + 6, GRN_W = DTW + 4, 
  * need (sqrt(variance(rb)) and sqrt(variance(g)) at the chroma offset); a
  * green cell holds its value and sqrt(variance(g)). */
 struct DevelopLds {
-    float norm[NORM_H][NORM_W];
-    float dev[NORM_H][NORM_W];
+    union {
+        struct {
+            float norm[NORM_H][NORM_W];
+            float dev[NORM_H][NORM_W];
+        } in;
+        /* After the green pass (the normalized values are dead by then),
+         * on in-range frames: each red/blue cell's refined reciprocals of the
+         * three chroma denominators it is a neighbour with (green, green +
+         * its deviation, green - its deviation). */
+        float4 rcp[GRN_H][GRN_W / 2];
+    } u;
     float4 rb[GRN_H][GRN_W / 2];
     float2 gr[GRN_H][GRN_W / 2];
     uint32_t curve[9];
@@ -1420,11 +1453,19 @@ __device__ __forceinline__ float2 green_g(const DevK &k, float centre)
  * of pixels (xb, y), (xb + 1, y), xb even: `green(y, x)` and `grc(y, x)`
  * read the red/blue and the green green-pass cells at picture coordinates.
  * res[i] is pixel xb + i. */
-template <bool IN_RANGE, class RbCell, class GCell>
+template <bool IN_RANGE, class RbCell, class GCell, class RcpCell>
 __device__ __forceinline__ void chroma_pair(const DevelopDesc &d, const DevK &k, uint32_t y,
-                                            uint32_t xb, RbCell green, GCell grc,
+                                            uint32_t xb, RbCell green, GCell grc, RcpCell rcpc,
                                             float (&res)[2][3])
 {
+    /* The neighbour cells' refined reciprocals, read only where they were
+     * computed (in-range frames). */
+    auto rcp_of = [&](uint32_t yy, uint32_t xx_) {
+        if constexpr (IN_RANGE)
+            return rcpc(yy, xx_);
+        else
+            return make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    };
     const uint32_t w = k.w, h = k.h;
     const float n0 = k.n0, n1 = k.n1, n2 = k.n2, off = k.coff;
     const uint32_t ym = y > 0 ? y - 1 : y + 1, yp = y < h - 1 ? y + 1 : y - 1;
@@ -1457,7 +1498,9 @@ __device__ __forceinline__ void chroma_pair(const DevelopDesc &d, const DevK &k,
         const float gg[4] = {a.y, b.y, c2.y, e.y};
         const float sx[4] = {a.z, b.z, c2.z, e.z};
         const float sg[4] = {a.w, b.w, c2.w, e.w};
-        const float est = estimate_pre<IN_RANGE>(xx, gg, sx, sg, mid.y, mid.w, off, n0, n1, n2);
+        const float4 rc[4] = {rcp_of(ym, xm), rcp_of(yp, xm), rcp_of(ym, xp), rcp_of(yp, xp)};
+        const float est =
+            estimate_pre<IN_RANGE>(xx, gg, sx, sg, rc, mid.y, mid.w, off, n0, n1, n2);
         finish(on_red_row ? mid.x : est, mid.y, on_red_row ? est : mid.x, res_rb);
     }
     {
@@ -1473,8 +1516,13 @@ __device__ __forceinline__ void chroma_pair(const DevelopDesc &d, const DevK &k,
         const float hsx[4] = {l.z, r.z, l.z, r.z}, hsg[4] = {l.w, r.w, l.w, r.w};
         const float vx[4] = {u.x, dn.x, u.x, dn.x}, vg[4] = {u.y, dn.y, u.y, dn.y};
         const float vsx[4] = {u.z, dn.z, u.z, dn.z}, vsg[4] = {u.w, dn.w, u.w, dn.w};
-        const float eh = estimate_pre<IN_RANGE>(hx, hg, hsx, hsg, mid.y, mid.w, off, n0, n1, n2);
-        const float ev = estimate_pre<IN_RANGE>(vx, vg, vsx, vsg, mid.y, mid.w, off, n0, n1, n2);
+        const float4 rl = rcp_of(y, xm), rr = rcp_of(y, xp);
+        const float4 ru = rcp_of(ym, x), rd = rcp_of(yp, x);
+        const float4 hrc[4] = {rl, rr, rl, rr}, vrc[4] = {ru, rd, ru, rd};
+        const float eh =
+            estimate_pre<IN_RANGE>(hx, hg, hsx, hsg, hrc, mid.y, mid.w, off, n0, n1, n2);
+        const float ev =
+            estimate_pre<IN_RANGE>(vx, vg, vsx, vsg, vrc, mid.y, mid.w, off, n0, n1, n2);
         finish(on_red_row ? eh : ev, mid.y, on_red_row ? ev : eh, res_g);
     }
 #pragma unroll
@@ -1540,16 +1588,16 @@ __device__ inline void develop_tile_as(const DevelopDesc &d, const DevK &k, floa
         const bool red_row = ((uint32_t)py & 1u) == k.red_row;
         const bool rb = ((uint32_t)px & 1u) == (red_row ? k.red_col : 1u - k.red_col);
         const float cap = rb ? (red_row ? clip_red : clip_blue) : 1.0f;
-        lds.norm[ly][lx] = v;
-        lds.dev[ly][lx] = deviation<IN_RANGE>(fminf(v, cap), k.bs, k);
+        lds.u.in.norm[ly][lx] = v;
+        lds.u.in.dev[ly][lx] = deviation<IN_RANGE>(fminf(v, cap), k.bs, k);
     }
     __syncthreads();
 
     /* Green pass. Each lane takes one (green, red/blue) pair of a row and
      * does the green sensel, then the red/blue one, so every lane of a wave
      * runs the same code whatever the Bayer phase of its row. */
-    auto norm = [&](int32_t y, int32_t x) { return lds.norm[y - (y0 - 3)][x - (x0 - 3)]; };
-    auto dev = [&](int32_t y, int32_t x) { return lds.dev[y - (y0 - 3)][x - (x0 - 3)]; };
+    auto norm = [&](int32_t y, int32_t x) { return lds.u.in.norm[y - (y0 - 3)][x - (x0 - 3)]; };
+    auto dev = [&](int32_t y, int32_t x) { return lds.u.in.dev[y - (y0 - 3)][x - (x0 - 3)]; };
     /* Green-pass cells lie in [x0 - 1, x0 + DTW] x [y0 - 1, y0 + DTH]; the
      * mirroring acts only within two sensels of the picture's sides. */
     const bool edge_x = x0 - 1 <= 1 || x0 + DTW >= (int32_t)w - 2;
@@ -1578,6 +1626,20 @@ __device__ inline void develop_tile_as(const DevelopDesc &d, const DevK &k, floa
     }
     __syncthreads();
 
+    if constexpr (IN_RANGE) {
+        /* Each red/blue cell's refined reciprocals of its three chroma
+         * denominators, once per cell instead of once per quotient (a cell
+         * is a neighbour of up to eight pixels). Cells outside the picture
+         * hold whatever the LDS held; no quotient reads them. */
+        for (uint32_t i = tid; i < GRN_H * (GRN_W / 2); i += 256) {
+            const uint32_t ly = i / (GRN_W / 2), lp = i - ly * (GRN_W / 2);
+            const float4 c = lds.rb[ly][lp];
+            lds.u.rcp[ly][lp] = make_float4(refined_rcp(c.y), refined_rcp(c.w + c.y),
+                                            refined_rcp(c.y - c.w), 0.0f);
+        }
+        __syncthreads();
+    }
+
     /* Chroma pass, white balance and output matrix: each lane takes the
      * red/blue pixel and the green pixel of one pair. */
     auto rbc = [&](uint32_t y, uint32_t x) {
@@ -1585,6 +1647,9 @@ __device__ inline void develop_tile_as(const DevelopDesc &d, const DevK &k, floa
     };
     auto grc = [&](uint32_t y, uint32_t x) {
         return lds.gr[(int32_t)y - (y0 - 1)][((int32_t)x - (x0 - 2)) >> 1];
+    };
+    auto rcpc = [&](uint32_t y, uint32_t x) {
+        return lds.u.rcp[(int32_t)y - (y0 - 1)][((int32_t)x - (x0 - 2)) >> 1];
     };
     /* Lane tid takes pair column tid % 32 of rows tid / 32 + 8j: the column,
      * its clamps and (rows 8 apart) the Bayer phase stay the same for the
@@ -1598,7 +1663,7 @@ __device__ inline void develop_tile_as(const DevelopDesc &d, const DevK &k, floa
         if (y >= h || xb >= w)
             continue;
         float res[2][3];
-        chroma_pair<IN_RANGE>(d, k, y, xb, rbc, grc, res);
+        chroma_pair<IN_RANGE>(d, k, y, xb, rbc, grc, rcpc, res);
         store_pair(out_r, out_g, out_b, (size_t)(y - oy) * ostride + (xb - ox), xb, w, res);
     }
 }
