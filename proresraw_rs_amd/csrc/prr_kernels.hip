@@ -1579,17 +1579,37 @@ __device__ inline void develop_tile_as(const DevelopDesc &d, const DevK &k, floa
      * one's deviation as a green-pass neighbour (outside the picture, by the
      * Bayer phase of the position: those entries are never used unmirrored). */
     const float clip_red = d.clip[0], clip_blue = d.clip[1];
-    for (uint32_t i = tid; i < NORM_H * NORM_W; i += 256) {
-        const int32_t ly = (int32_t)(i / NORM_W), lx = (int32_t)(i - (uint32_t)ly * NORM_W);
+    auto load = [&](int32_t ly, int32_t lx, float cap) {
         const int32_t py = y0 - 3 + ly, px = x0 - 3 + lx;
         const int32_t y = py < 0 ? 0 : py > (int32_t)h - 1 ? (int32_t)h - 1 : py;
         const int32_t x = px < 0 ? 0 : px > (int32_t)w - 1 ? (int32_t)w - 1 : px;
         const float v = norm_of(d, lds.curve, d.samples[(size_t)y * d.stride + (uint32_t)x]);
-        const bool red_row = ((uint32_t)py & 1u) == k.red_row;
-        const bool rb = ((uint32_t)px & 1u) == (red_row ? k.red_col : 1u - k.red_col);
-        const float cap = rb ? (red_row ? clip_red : clip_blue) : 1.0f;
         lds.u.in.norm[ly][lx] = v;
         lds.u.in.dev[ly][lx] = deviation<IN_RANGE>(fminf(v, cap), k.bs, k);
+    };
+    /* The clip of a neighbour by its Bayer phase (a red/blue sensel at its
+     * row's clip, a green one at 1). */
+    auto cap_of = [&](int32_t ly, int32_t lx) {
+        const int32_t py = y0 - 3 + ly, px = x0 - 3 + lx;
+        const bool red_row = ((uint32_t)py & 1u) == k.red_row;
+        const bool rb = ((uint32_t)px & 1u) == (red_row ? k.red_col : 1u - k.red_col);
+        return rb ? (red_row ? clip_red : clip_blue) : 1.0f;
+    };
+    {
+        /* The first 64 columns: lane tid keeps column tid % 64 and takes
+         * rows tid / 64 + 4j, so its column clamp and (rows 4 apart) its
+         * Bayer phase are fixed. */
+        static_assert(NORM_W >= 64, "the halo row covers 64 lanes");
+        const int32_t lx = (int32_t)(tid & 63), ly0 = (int32_t)(tid >> 6);
+        const float cap = cap_of(ly0, lx);
+        for (int32_t ly = ly0; ly < NORM_H; ly += 4)
+            load(ly, lx, cap);
+        /* The remaining columns, one sensel a lane. */
+        for (uint32_t i = tid; i < NORM_H * (NORM_W - 64); i += 256) {
+            const int32_t ly = (int32_t)(i / (NORM_W - 64));
+            const int32_t lx2 = 64 + (int32_t)(i - (uint32_t)ly * (NORM_W - 64));
+            load(ly, lx2, cap_of(ly, lx2));
+        }
     }
     __syncthreads();
 
