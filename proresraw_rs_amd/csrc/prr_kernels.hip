@@ -1239,13 +1239,15 @@ __device__ inline float estimate_pre(const float *xx, const float *gg, const flo
     float var_x[4], lo[4], mid[4], hi[4];
 #pragma unroll
     for (int i = 0; i < 4; i++) {
-        var_x[i] = variance(xx[i], off, n0, n1, n2);
         if (IN_RANGE) {
-            /* rc[i] = refined reciprocals of gg, sg + gg, gg - sg (x, y, z). */
+            /* rc[i] = refined reciprocals of gg, sg + gg, gg - sg (x, y, z)
+             * and variance(xx, off) (w). */
+            var_x[i] = rc[i].w;
             lo[i] = div_in_range_r((cg - cs) * dx[i], sg[i] + gg[i], rc[i].y);
             mid[i] = div_in_range_r(cg * xx[i], gg[i], rc[i].x);
             hi[i] = div_in_range_r((cs + cg) * (sx[i] + xx[i]), dg[i], rc[i].z);
         } else {
+            var_x[i] = variance(xx[i], off, n0, n1, n2);
             lo[i] = div((cg - cs) * dx[i], sg[i] + gg[i]);
             mid[i] = div(cg * xx[i], gg[i]);
             hi[i] = div((cs + cg) * (sx[i] + xx[i]), dg[i]);
@@ -1284,7 +1286,8 @@ struct DevelopLds {
         /* After the green pass (the normalized values are dead by then),
          * on in-range frames: each red/blue cell's refined reciprocals of the
          * three chroma denominators it is a neighbour with (green, green +
-         * its deviation, green - its deviation). */
+         * its deviation, green - its deviation) and the variance of its
+         * value at the chroma offset. */
         float4 rcp[GRN_H][GRN_W / 2];
     } u;
     float4 rb[GRN_H][GRN_W / 2];
@@ -1648,14 +1651,16 @@ __device__ inline void develop_tile_as(const DevelopDesc &d, const DevK &k, floa
 
     if constexpr (IN_RANGE) {
         /* Each red/blue cell's refined reciprocals of its three chroma
-         * denominators, once per cell instead of once per quotient (a cell
-         * is a neighbour of up to eight pixels). Cells outside the picture
-         * hold whatever the LDS held; no quotient reads them. */
+         * denominators and the variance of its value at the chroma offset,
+         * once per cell instead of once per use (a cell is a neighbour of up
+         * to eight pixels). Cells outside the picture hold whatever the LDS
+         * held; no estimate reads them. */
         for (uint32_t i = tid; i < GRN_H * (GRN_W / 2); i += 256) {
             const uint32_t ly = i / (GRN_W / 2), lp = i - ly * (GRN_W / 2);
             const float4 c = lds.rb[ly][lp];
             lds.u.rcp[ly][lp] = make_float4(refined_rcp(c.y), refined_rcp(c.w + c.y),
-                                            refined_rcp(c.y - c.w), 0.0f);
+                                            refined_rcp(c.y - c.w),
+                                            variance(c.x, k.coff, k.n0, k.n1, k.n2));
         }
         __syncthreads();
     }
