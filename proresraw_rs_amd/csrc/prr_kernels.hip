@@ -1625,27 +1625,39 @@ __device__ inline void develop_tile_as(const DevelopDesc &d, const DevK &k, floa
      * mirroring acts only within two sensels of the picture's sides. */
     const bool edge_x = x0 - 1 <= 1 || x0 + DTW >= (int32_t)w - 2;
     const bool edge_y = y0 - 1 <= 1 || y0 + DTH >= (int32_t)h - 2;
-    for (uint32_t i = tid; i < GRN_H * (GRN_W / 2); i += 256) {
-        const int32_t ly = (int32_t)(i / (GRN_W / 2));
-        const int32_t lp = (int32_t)(i - (uint32_t)ly * (GRN_W / 2));
+    auto cell_in_tile = [&](int32_t gx) {
+        return gx >= x0 - 1 && gx <= x0 + DTW && gx >= 0 && gx < (int32_t)w;
+    };
+    /* Green-pass cells of pair column lp of row ly: the green sensel's own
+     * cell and the red/blue one's (green_row_impl). */
+    auto green_cells = [&](int32_t ly, int32_t lp) {
         const int32_t gy = y0 - 1 + ly;
         if (gy < 0 || gy >= (int32_t)h)
-            continue;
+            return;
         const bool phase = ((uint32_t)gy & 1u) == k.red_row;
         const int32_t first = (int32_t)(phase ? k.red_col : 1 - k.red_col);
         const float clip = phase ? d.clip[0] : d.clip[1];
-        auto cell_in_tile = [&](int32_t gx) {
-            return gx >= x0 - 1 && gx <= x0 + DTW && gx >= 0 && gx < (int32_t)w;
-        };
-        {
-            const int32_t gx = x0 - 2 + 2 * lp + (1 - first);
-            if (cell_in_tile(gx))
-                lds.gr[ly][lp] = green_g<IN_RANGE>(k, norm(gy, gx));
-        }
+        const int32_t gxg = x0 - 2 + 2 * lp + (1 - first);
+        if (cell_in_tile(gxg))
+            lds.gr[ly][lp] = green_g<IN_RANGE>(k, norm(gy, gxg));
         const int32_t gx = x0 - 2 + 2 * lp + first;
-        if (!cell_in_tile(gx))
-            continue;
-        lds.rb[ly][lp] = green_rb<IN_RANGE>(k, gx, gy, clip, norm, dev, edge_x, edge_y);
+        if (cell_in_tile(gx))
+            lds.rb[ly][lp] = green_rb<IN_RANGE>(k, gx, gy, clip, norm, dev, edge_x, edge_y);
+    };
+    {
+        /* Pair columns 0..31: lane tid keeps column tid % 32 over rows
+         * tid / 32 + 8j (one Bayer phase); columns 32 and 33 go to lanes
+         * 64..131, so no wave takes more than 5 cells (as 1,156 cells over
+         * 256 lanes need). */
+        static_assert(GRN_W / 2 == 34 && GRN_H == 34, "green-pass cells of a 64x32 tile");
+        const int32_t lp = (int32_t)(tid & 31), ly0 = (int32_t)(tid >> 5);
+#pragma unroll 1
+        for (int32_t ly = ly0; ly < GRN_H; ly += 8)
+            green_cells(ly, lp);
+        if (tid >= 64 && tid < 64 + 2 * GRN_H) {
+            const int32_t i = (int32_t)tid - 64;
+            green_cells(i >> 1, 32 + (i & 1));
+        }
     }
     __syncthreads();
 
