@@ -1451,29 +1451,37 @@ __device__ __forceinline__ float2 green_g(const DevK &k, float centre)
     return make_float2(centre, deviation<IN_RANGE>(centre, k.coff, k));
 }
 
+/* LDS cell indices (row * GRN_W / 2 + column) of one chroma pair: the rows
+ * above, at and below it (mirrored at the picture's top and bottom, as
+ * chroma_row_with does) and the cells of its red/blue pixel, its green
+ * pixel and their left and right neighbours (mirrored at the sides). */
+struct PairCells {
+    uint32_t rm, r0, rp;          /* row offsets, in cells */
+    uint32_t rb_x, rb_xm, rb_xp;  /* columns of the red/blue pixel's cells */
+    uint32_t g_x, g_xm, g_xp;     /* columns of the green pixel's cells */
+};
+
 /* Chroma pass (chroma_row_with, demosaic.rs:648-688), white balance and
  * the output matrix (matrix_planes_impl, demosaic.rs:711-720) of the pair
- * of pixels (xb, y), (xb + 1, y), xb even: `green(y, x)` and `grc(y, x)`
- * read the red/blue and the green green-pass cells at picture coordinates.
+ * of pixels (xb, y), (xb + 1, y), xb even, whose red/blue pixel is at xb +
+ * rb_col: `pc` locates its green-pass cells in the red/blue (`rbcell`) and
+ * green (`gcell`) cell arrays and the reciprocal cells (`rcpcell`).
  * res[i] is pixel xb + i. */
-template <bool IN_RANGE, class RbCell, class GCell, class RcpCell>
-__device__ __forceinline__ void chroma_pair(const DevelopDesc &d, const DevK &k, uint32_t y,
-                                            uint32_t xb, RbCell green, GCell grc, RcpCell rcpc,
-                                            float (&res)[2][3])
+template <bool IN_RANGE>
+__device__ __forceinline__ void chroma_pair(const DevelopDesc &d, const DevK &k, bool on_red_row,
+                                            uint32_t rb_col, const PairCells &pc,
+                                            const float4 *rbcell, const float2 *gcell,
+                                            const float4 *rcpcell, float (&res)[2][3])
 {
+    const float n0 = k.n0, n1 = k.n1, n2 = k.n2, off = k.coff;
     /* The neighbour cells' refined reciprocals, read only where they were
      * computed (in-range frames). */
-    auto rcp_of = [&](uint32_t yy, uint32_t xx_) {
+    auto rcp_of = [&](uint32_t i) {
         if constexpr (IN_RANGE)
-            return rcpc(yy, xx_);
+            return rcpcell[i];
         else
             return make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     };
-    const uint32_t w = k.w, h = k.h;
-    const float n0 = k.n0, n1 = k.n1, n2 = k.n2, off = k.coff;
-    const uint32_t ym = y > 0 ? y - 1 : y + 1, yp = y < h - 1 ? y + 1 : y - 1;
-    const bool on_red_row = (y & 1u) == k.red_row;
-    const uint32_t rb_col = on_red_row ? k.red_col : 1 - k.red_col;
     auto finish = [&](float rr, float gval, float bb, float *out) {
         float r = fminf(rr * d.wb[0], 1.0f);
         float g = fminf(gval, 1.0f);
@@ -1493,15 +1501,15 @@ __device__ __forceinline__ void chroma_pair(const DevelopDesc &d, const DevK &k,
     float res_rb[3], res_g[3];
     {
         /* The red or blue pixel: diagonal neighbours. */
-        const uint32_t x = xb + rb_col;
-        const uint32_t xm = x > 0 ? x - 1 : x + 1, xp = x < w - 1 ? x + 1 : x - 1;
-        const float4 mid = green(y, x);
-        const float4 a = green(ym, xm), b = green(yp, xm), c2 = green(ym, xp), e = green(yp, xp);
+        const float4 mid = rbcell[pc.r0 + pc.rb_x];
+        const float4 a = rbcell[pc.rm + pc.rb_xm], b = rbcell[pc.rp + pc.rb_xm];
+        const float4 c2 = rbcell[pc.rm + pc.rb_xp], e = rbcell[pc.rp + pc.rb_xp];
         const float xx[4] = {a.x, b.x, c2.x, e.x};
         const float gg[4] = {a.y, b.y, c2.y, e.y};
         const float sx[4] = {a.z, b.z, c2.z, e.z};
         const float sg[4] = {a.w, b.w, c2.w, e.w};
-        const float4 rc[4] = {rcp_of(ym, xm), rcp_of(yp, xm), rcp_of(ym, xp), rcp_of(yp, xp)};
+        const float4 rc[4] = {rcp_of(pc.rm + pc.rb_xm), rcp_of(pc.rp + pc.rb_xm),
+                              rcp_of(pc.rm + pc.rb_xp), rcp_of(pc.rp + pc.rb_xp)};
         const float est =
             estimate_pre<IN_RANGE>(xx, gg, sx, sg, rc, mid.y, mid.w, off, n0, n1, n2);
         finish(on_red_row ? mid.x : est, mid.y, on_red_row ? est : mid.x, res_rb);
@@ -1509,18 +1517,16 @@ __device__ __forceinline__ void chroma_pair(const DevelopDesc &d, const DevK &k,
     {
         /* The green pixel: horizontal and vertical neighbour pairs (red or
          * blue cells). */
-        const uint32_t x = xb + (1 - rb_col);
-        const uint32_t xm = x > 0 ? x - 1 : x + 1, xp = x < w - 1 ? x + 1 : x - 1;
-        const float2 gm = grc(y, x);
+        const float2 gm = gcell[pc.r0 + pc.g_x];
         const float4 mid = make_float4(0.0f, gm.x, 0.0f, gm.y);
-        const float4 l = green(y, xm), r = green(y, xp);
-        const float4 u = green(ym, x), dn = green(yp, x);
+        const float4 l = rbcell[pc.r0 + pc.g_xm], r = rbcell[pc.r0 + pc.g_xp];
+        const float4 u = rbcell[pc.rm + pc.g_x], dn = rbcell[pc.rp + pc.g_x];
         const float hx[4] = {l.x, r.x, l.x, r.x}, hg[4] = {l.y, r.y, l.y, r.y};
         const float hsx[4] = {l.z, r.z, l.z, r.z}, hsg[4] = {l.w, r.w, l.w, r.w};
         const float vx[4] = {u.x, dn.x, u.x, dn.x}, vg[4] = {u.y, dn.y, u.y, dn.y};
         const float vsx[4] = {u.z, dn.z, u.z, dn.z}, vsg[4] = {u.w, dn.w, u.w, dn.w};
-        const float4 rl = rcp_of(y, xm), rr = rcp_of(y, xp);
-        const float4 ru = rcp_of(ym, x), rd = rcp_of(yp, x);
+        const float4 rl = rcp_of(pc.r0 + pc.g_xm), rr = rcp_of(pc.r0 + pc.g_xp);
+        const float4 ru = rcp_of(pc.rm + pc.g_x), rd = rcp_of(pc.rp + pc.g_x);
         const float4 hrc[4] = {rl, rr, rl, rr}, vrc[4] = {ru, rd, ru, rd};
         const float eh =
             estimate_pre<IN_RANGE>(hx, hg, hsx, hsg, hrc, mid.y, mid.w, off, n0, n1, n2);
@@ -1678,29 +1684,42 @@ __device__ inline void develop_tile_as(const DevelopDesc &d, const DevK &k, floa
     }
 
     /* Chroma pass, white balance and output matrix: each lane takes the
-     * red/blue pixel and the green pixel of one pair. */
-    auto rbc = [&](uint32_t y, uint32_t x) {
-        return lds.rb[(int32_t)y - (y0 - 1)][((int32_t)x - (x0 - 2)) >> 1];
-    };
-    auto grc = [&](uint32_t y, uint32_t x) {
-        return lds.gr[(int32_t)y - (y0 - 1)][((int32_t)x - (x0 - 2)) >> 1];
-    };
-    auto rcpc = [&](uint32_t y, uint32_t x) {
-        return lds.u.rcp[(int32_t)y - (y0 - 1)][((int32_t)x - (x0 - 2)) >> 1];
-    };
-    /* Lane tid takes pair column tid % 32 of rows tid / 32 + 8j: the column,
-     * its clamps and (rows 8 apart) the Bayer phase stay the same for the
-     * lane, so their arithmetic is done once. */
+     * red/blue pixel and the green pixel of one pair. Lane tid takes pair
+     * column tid % 32 of rows tid / 32 + 8j: the column, its cells and
+     * (rows 8 apart) the Bayer phase stay the same for the lane. */
     static_assert(DTW / 2 == 32 && DTH % 8 == 0, "chroma lanes cover 8 rows of 32 pairs");
+    constexpr uint32_t RW = GRN_W / 2; /* cells a row */
     const uint32_t lp = tid % (DTW / 2), ly0 = tid / (DTW / 2);
     const uint32_t xb = (uint32_t)x0 + 2 * lp;
+    const bool on_red_row = ((uint32_t)(y0 + (int32_t)ly0) & 1u) == k.red_row;
+    const uint32_t rb_col = on_red_row ? k.red_col : 1 - k.red_col;
+    /* Cell column of picture column x (mirrored neighbours included). */
+    auto col = [&](uint32_t x) { return (uint32_t)(((int32_t)x - (x0 - 2)) >> 1); };
+    PairCells pc;
+    {
+        const uint32_t x = xb + rb_col;
+        pc.rb_x = col(x);
+        pc.rb_xm = col(x > 0 ? x - 1 : x + 1);
+        pc.rb_xp = col(x < w - 1 ? x + 1 : x - 1);
+        const uint32_t gxp = xb + (1 - rb_col);
+        pc.g_x = col(gxp);
+        pc.g_xm = col(gxp > 0 ? gxp - 1 : gxp + 1);
+        pc.g_xp = col(gxp < w - 1 ? gxp + 1 : gxp - 1);
+    }
+    const float4 *rbcell = &lds.rb[0][0];
+    const float2 *gcell = &lds.gr[0][0];
+    const float4 *rcpcell = &lds.u.rcp[0][0];
 #pragma unroll
     for (uint32_t j = 0; j < DTH / 8; j++) {
         const uint32_t y = (uint32_t)y0 + ly0 + 8 * j;
         if (y >= h || xb >= w)
             continue;
+        /* Row y is cell row ly0 + 8j + 1. */
+        pc.r0 = (ly0 + 8 * j + 1) * RW;
+        pc.rm = y > 0 ? pc.r0 - RW : pc.r0 + RW;
+        pc.rp = y < h - 1 ? pc.r0 + RW : pc.r0 - RW;
         float res[2][3];
-        chroma_pair<IN_RANGE>(d, k, y, xb, rbc, grc, rcpc, res);
+        chroma_pair<IN_RANGE>(d, k, on_red_row, rb_col, pc, rbcell, gcell, rcpcell, res);
         store_pair(out_r, out_g, out_b, (size_t)(y - oy) * ostride + (xb - ox), xb, w, res);
     }
 }
