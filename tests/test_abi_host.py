@@ -175,3 +175,37 @@ def test_output_size_is_checked_before_any_device_work():
 def test_fourccs():
     assert P.is_prores_raw_fourcc(b"aprn") and P.is_prores_raw_fourcc(b"APRH")
     assert not P.is_prores_raw_fourcc(b"apch")
+
+
+def test_new_entry_points_reject_bad_arguments_before_device_work():
+    """Argument checks of the pipeline, DNG-tile and orientation entry points
+    come before any HIP call, so they hold without a device."""
+    lib = N.lib()
+    err = N.PrrError()
+    h = ctypes.c_void_p()
+    assert lib.prr_pipeline_create(None, 3, ctypes.byref(h), ctypes.byref(err)) == 101
+    assert lib.prr_pipeline_submit(None, None, None, None, 0, 0, 0, 0.0, 0.0, 2, None, None, None,
+                                   None, ctypes.byref(err)) == 101
+    assert lib.prr_pipeline_sync(None, ctypes.byref(err)) == 101
+    lib.prr_pipeline_destroy(None)
+    # DNG tiles: a picture narrower than two sensels, a stride below the width.
+    buf = (ctypes.c_uint16 * 16)()
+    out = (ctypes.c_uint8 * 16)()
+    offs = (ctypes.c_uint32 * 1)()
+    total = ctypes.c_size_t()
+    fake_ctx = ctypes.c_void_p(1)  # never dereferenced: the size check comes first
+    for w, h_, stride in ((1, 4, 1), (4, 1, 4), (4, 4, 3)):
+        rc = lib.prr_dng_lj92_tiles(fake_ctx, buf, w, h_, stride, out, 16, offs, offs, 1,
+                                    ctypes.byref(total), ctypes.byref(err))
+        assert rc == 101, (w, h_, stride)
+    assert lib.prr_dng_lj92_capacity(5760, 3240) == 299 * lib.prr_dng_lj92_capacity(256, 256)
+    # Orientation: no context, an orientation outside 1..8.
+    crop = (ctypes.c_uint8 * 4)()
+    p = ctypes.c_void_p(16)
+    assert lib.prr_orient_planes_device(None, p, p, p, 8, 8, 8, crop, 1, p, p, p, 8, None,
+                                        ctypes.byref(err)) == 101
+    ow, oh = ctypes.c_uint32(), ctypes.c_uint32()
+    assert lib.prr_oriented_size(8, 8, crop, 0, ctypes.byref(ow), ctypes.byref(oh)) == 101
+    assert lib.prr_oriented_size(8, 8, crop, 9, ctypes.byref(ow), ctypes.byref(oh)) == 101
+    assert lib.prr_oriented_size(8, 6, crop, 6, ctypes.byref(ow), ctypes.byref(oh)) == 0
+    assert (ow.value, oh.value) == (6, 8)
