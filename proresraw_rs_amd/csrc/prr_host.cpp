@@ -909,7 +909,8 @@ extern "C" int prr_pipeline_create(prr_context *ctx, uint32_t depth, prr_pipelin
     for (PipeSet &st : p->sets) {
         for (hipEvent_t *ev : {&st.decoded, &st.developed})
             if (e == hipSuccess)
-                e = hipEventCreateWithFlags(ev, hipEventDisableTiming);
+                e = hipEventCreateWithFlags(ev, hipEventDisableTiming |
+                                                        hipEventDisableSystemFence);
         for (hipEvent_t *ev : {&st.t0, &st.t1, &st.t2, &st.t3})
             if (e == hipSuccess)
                 e = hipEventCreate(ev);
