@@ -355,10 +355,25 @@ __device__ inline bool decode_stream(Bits<P> &b, uint32_t lb, uint32_t log2_coef
 /* ------------------------------------------------------------------ */
 /* Inverse transforms (idct.rs). d holds dequantized coefficients. */
 
+/* c is a 16-bit coefficient and q a product of two bytes (quantization
+ * matrix entry and tile qscale), so both fit the 24-bit multiplier and the
+ * product fits 32 bits: __mul24 is exact and runs at full rate, where a
+ * 32-bit multiply issues at quarter rate. */
 __device__ inline int32_t dequantize(int32_t c, int32_t q)
 {
-    int32_t v = c * q;
+    int32_t v = __mul24(c, q);
     return v < -32768 ? -32768 : v > 32767 ? 32767 : v;
+}
+
+/* The quantizer step of a coefficient: matrix entry qm times the tile's
+ * qscale, both bytes. The empty asm makes q one opaque value; otherwise the
+ * compiler reassociates c * (qm * qs) into (c * qs) * qm and issues the
+ * second product as a quarter-rate 32-bit multiply. */
+__device__ inline int32_t quant_step(uint32_t qm, uint32_t qs)
+{
+    int32_t q = (int32_t)((qm & 0xFFu) * (qs & 0xFFu));
+    asm("" : "+v"(q));
+    return q;
 }
 
 /* trunc_to_i32 + output_code (idct.rs:108-123). */
@@ -910,7 +925,8 @@ __device__ inline void decode_group_full(const FrameDesc &f, const TileSet &ts, 
                 int32_t d[64];
 #pragma unroll
                 for (int k = 0; k < 64; k++)
-                    d[k] = dequantize(src[kInv8.v[k] << amax], (int32_t)g.qmat[k] * (int32_t)ti.qscale);
+                    d[k] = dequantize(src[kInv8.v[k] << amax],
+                                      quant_step(g.qmat[k], ti.qscale));
                 uint32_t v[64];
                 if (FFMPEG)
                     ffmpeg_8x8(d, v);
@@ -996,21 +1012,23 @@ __device__ inline void decode_group_reduced(const FrameDesc &f, const TileSet &t
                     int32_t d[64];
 #pragma unroll
                     for (int k = 0; k < 64; k++)
-                        d[k] = dequantize(src[kInv8.v[k] << amax], (int32_t)g.qmat[k] * (int32_t)ti.qscale);
+                        d[k] = dequantize(src[kInv8.v[k] << amax],
+                                          quant_step(g.qmat[k], ti.qscale));
                     apple_8x8(d, v);
                 } else if constexpr (shift == 2) {
                     int32_t d[16];
 #pragma unroll
                     for (int k = 0; k < 16; k++)
                         d[k] = dequantize(src[kInv4.v[k] << amax],
-                                          (int32_t)g.qmat[8 * (k >> 2) + (k & 3)] * (int32_t)ti.qscale);
+                                          quant_step(g.qmat[8 * (k >> 2) + (k & 3)], ti.qscale));
                     apple_4x4(d, v);
                 } else {
                     int32_t d[4];
                     const int qi[4] = {0, 1, 8, 9};
 #pragma unroll
                     for (int k = 0; k < 4; k++)
-                        d[k] = dequantize(src[kInv2.v[k] << amax], (int32_t)g.qmat[qi[k]] * (int32_t)ti.qscale);
+                        d[k] = dequantize(src[kInv2.v[k] << amax],
+                                          quant_step(g.qmat[qi[k]], ti.qscale));
                     apple_2x2(d, v);
                 }
                 uint16_t *dst = stage + ((tc * 4 + s) * side) * W + b * side;
