@@ -15,7 +15,7 @@
  *                 linear interleaved RGB (reduced.rs:217-273)
  *  k_linearize    Frame::linearize_into (decoder.rs:197-216)
  *  k_develop      green pass + chroma pass + white balance + output matrix
- *                 in one kernel over 64x32 pixel tiles with LDS halos
+ *                 in one kernel over 64x16 pixel tiles with LDS halos
  *                 (demosaic.rs:356-720)
  *
  * Floating point follows the crate operation for operation: this file is
@@ -1284,7 +1284,10 @@ __device__ inline float estimate_pre(const float *xx, const float *gg, const flo
     return fmaf(mean_mid - mean_x, weight, mean_x);
 }
 
-enum { DTW = 64, DTH = 32, NORM_W = DTW + 6, NORM_H = DTH + 6, GRN_W = DTW + 4, GRN_H = DTH + 2 };
+/* Develop tiles of 64 x 16 pixels: a tile's LDS (27 KB) lets six workgroups
+ * share a CU. 64 x 32 tiles (49 KB, three a CU) recompute less halo but
+ * ran 10 % slower; 64 x 8 tiles recompute more halo than occupancy repays. */
+enum { DTW = 64, DTH = 16, NORM_W = DTW + 6, NORM_H = DTH + 6, GRN_W = DTW + 4, GRN_H = DTH + 2 };
 
 /* LDS of one develop tile: normalized values, and beside each the noise
  * deviation it has as a green-pass neighbour (sqrt(variance) at the green
@@ -1585,7 +1588,7 @@ __device__ __forceinline__ void store_pair(float *out_r, float *out_g, float *ou
     }
 }
 
-/* One 64x32 tile of output per 256-lane workgroup: normalized code values
+/* One 64x16 tile of output per 256-lane workgroup: normalized code values
  * with a 3-sensel halo and green-pass cells with a 1-sensel halo live in
  * LDS; each lane handles a red/blue + green pair of sensels so every lane
  * does the same work. Output sensel (x, y) goes to (y - oy) * ostride +
@@ -1671,9 +1674,9 @@ __device__ inline void develop_tile_as(const DevelopDesc &d, const DevK &k, floa
     {
         /* Pair columns 0..31: lane tid keeps column tid % 32 over rows
          * tid / 32 + 8j (one Bayer phase); columns 32 and 33 go to lanes
-         * 64..131, so no wave takes more than 5 cells (as 1,156 cells over
-         * 256 lanes need). */
-        static_assert(GRN_W / 2 == 34 && GRN_H == 34, "green-pass cells of a 64x32 tile");
+         * 64..99, so no wave takes more than 3 cells (as 612 cells over 256
+         * lanes need). */
+        static_assert(GRN_W / 2 == 34 && GRN_H == 18, "green-pass cells of a 64x16 tile");
         const int32_t lp = (int32_t)(tid & 31), ly0 = (int32_t)(tid >> 5);
 #pragma unroll 1
         for (int32_t ly = ly0; ly < GRN_H; ly += 8)
@@ -1742,7 +1745,7 @@ __device__ inline void develop_tile_as(const DevelopDesc &d, const DevK &k, floa
     }
 }
 
-/* One 64x32 tile of develop: the in-range form when the frame's constants
+/* One 64x16 tile of develop: the in-range form when the frame's constants
  * allow it (the same bits either way). */
 __device__ inline void develop_tile(const DevelopDesc &d, float *out_r, float *out_g,
                                     float *out_b, uint32_t bx, uint32_t by, DevelopLds &lds,
@@ -1767,19 +1770,21 @@ __global__ __launch_bounds__(256) void k_develop(DevelopDesc d, float *out_r, fl
     develop_tile(d, out_r, out_g, out_b, blockIdx.x, blockIdx.y, lds, 0, 0, d.width);
 }
 
-/* Plain develop of sampled 64x64 tiles (two 64x32 develop tiles each) into
- * scratch[k] = r, g, b planes of 64x64, for the learned path's check. */
+/* Plain develop of sampled 64x64 tiles (SAMPLE_PARTS develop tiles each)
+ * into scratch[k] = r, g, b planes of 64x64, for the learned path's check. */
+constexpr uint32_t SAMPLE_PARTS = 64 / DTH;
+static_assert(DTW == 64 && 64 % DTH == 0, "a 64x64 sample is whole develop tiles");
 __global__ __launch_bounds__(256) void k_develop_sampled(DevelopDesc d, const uint32_t *tiles,
                                                          float *scratch)
 {
     __shared__ DevelopLds lds;
-    const uint32_t k = blockIdx.x >> 1, half = blockIdx.x & 1;
-    const uint32_t bx = tiles[2 * k], by2 = tiles[2 * k + 1];
-    if (2 * by2 + half >= (d.height + DTH - 1) / DTH)
+    const uint32_t k = blockIdx.x / SAMPLE_PARTS, part = blockIdx.x % SAMPLE_PARTS;
+    const uint32_t bx = tiles[2 * k], by64 = tiles[2 * k + 1];
+    const uint32_t by = SAMPLE_PARTS * by64 + part;
+    if (by >= (d.height + DTH - 1) / DTH)
         return;
     float *base = scratch + (size_t)k * 3 * 4096;
-    develop_tile(d, base, base + 4096, base + 8192, bx, 2 * by2 + half, lds, DTW * bx,
-                 2 * DTH * by2, 64);
+    develop_tile(d, base, base + 4096, base + 8192, bx, by, lds, DTW * bx, 64 * by64, 64);
 }
 
 /* Load a struct the whole wave agrees on into scalar registers. */
@@ -2176,7 +2181,8 @@ hipError_t launch_develop_sampled(const DevelopDesc &d, const uint32_t *tiles, u
 {
     if (n_tiles == 0)
         return hipSuccess;
-    hipLaunchKernelGGL(k_develop_sampled, dim3(2 * n_tiles), dim3(256), 0, s, d, tiles, scratch);
+    hipLaunchKernelGGL(k_develop_sampled, dim3(SAMPLE_PARTS * n_tiles), dim3(256), 0, s, d, tiles,
+                       scratch);
     return hipGetLastError();
 }
 } // namespace net
