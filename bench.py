@@ -74,17 +74,17 @@ def measured_traffic(kernel, width, height):
 UTIL_FILE = os.path.join(ROOT, "profiles", "issue_utilization.txt")
 
 
-def measured_valu_util(kernel, width, height):
-    """VALU issue utilization of `kernel` on this frame size from the
-    committed counter summary (4 x SQ_INSTS_VALU over cycles x SIMDs), or
-    None."""
+def measured_valu_rate(kernel, width, height):
+    """VALU instructions `kernel` issues per SIMD-cycle on this frame size,
+    from the committed counter summary (SQ_INSTS_VALU over cycles x SIMDs),
+    or None."""
     if (width, height) != (4096, 2160) or not os.path.exists(UTIL_FILE):
         return None
     for line in open(UTIL_FILE):
         f = line.split()
         if len(f) > 2 and f[0] == kernel and f[1] == "plain4096x2160":
             kv = dict(x.split("=") for x in f[2:])
-            return float(kv["valu_util"])
+            return float(kv["valu_per_simd_cycle"])
     return None
 
 
@@ -256,8 +256,8 @@ def main():
                 "frac": round(achieved / 8000.0, 4),
                 "traffic": measured_traffic("k_develop", w, h),
                 "traffic_source": os.path.relpath(TRAFFIC_FILE, ROOT),
-                "valu_issue_util": measured_valu_util("k_develop", w, h),
-                "valu_issue_util_source": os.path.relpath(UTIL_FILE, ROOT),
+                "valu_insts_per_simd_cycle": measured_valu_rate("k_develop", w, h),
+                "valu_source": os.path.relpath(UTIL_FILE, ROOT),
                 "kernel": "k_develop", "kernel_ms": round(pipe_develop_ms, 4),
                 "kernel_ms_alone": round(t_develop_ms, 4),
                 "bytes_per_launch": int(dev_bytes),
