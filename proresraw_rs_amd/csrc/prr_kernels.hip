@@ -1564,26 +1564,31 @@ __device__ __forceinline__ void chroma_pair(const DevelopDesc &d, const DevK &k,
 
 /* Store the pair (xb, xb + 1) at `at` (the index of xb): planar with
  * float2 stores, or interleaved RGB when out_g is NULL. */
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+
+/* The planes are written once and not read again by the kernel: streaming
+ * (non-temporal) stores keep them from filling the L2 with dirty lines the
+ * end of the launch would have to write back. */
 __device__ __forceinline__ void store_pair(float *out_r, float *out_g, float *out_b, size_t at,
                                            uint32_t xb, uint32_t w, const float (&res)[2][3])
 {
     if (out_g) {
         if (xb + 1 < w) {
-            *(float2 *)(out_r + at) = make_float2(res[0][0], res[1][0]);
-            *(float2 *)(out_g + at) = make_float2(res[0][1], res[1][1]);
-            *(float2 *)(out_b + at) = make_float2(res[0][2], res[1][2]);
+            __builtin_nontemporal_store((f32x2){res[0][0], res[1][0]}, (f32x2 *)(out_r + at));
+            __builtin_nontemporal_store((f32x2){res[0][1], res[1][1]}, (f32x2 *)(out_g + at));
+            __builtin_nontemporal_store((f32x2){res[0][2], res[1][2]}, (f32x2 *)(out_b + at));
         } else if (xb < w) {
-            out_r[at] = res[0][0];
-            out_g[at] = res[0][1];
-            out_b[at] = res[0][2];
+            __builtin_nontemporal_store(res[0][0], out_r + at);
+            __builtin_nontemporal_store(res[0][1], out_g + at);
+            __builtin_nontemporal_store(res[0][2], out_b + at);
         }
     } else {
 #pragma unroll
         for (uint32_t k = 0; k < 2; k++)
             if (xb + k < w) {
-                out_r[3 * (at + k) + 0] = res[k][0];
-                out_r[3 * (at + k) + 1] = res[k][1];
-                out_r[3 * (at + k) + 2] = res[k][2];
+                __builtin_nontemporal_store(res[k][0], out_r + 3 * (at + k) + 0);
+                __builtin_nontemporal_store(res[k][1], out_r + 3 * (at + k) + 1);
+                __builtin_nontemporal_store(res[k][2], out_r + 3 * (at + k) + 2);
             }
     }
 }
