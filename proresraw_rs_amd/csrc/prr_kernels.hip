@@ -1772,7 +1772,15 @@ __global__ __launch_bounds__(256) void k_develop(DevelopDesc d, float *out_r, fl
     if (d.status_out && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x < 4)
         reinterpret_cast<uint32_t *>(d.status_out)[threadIdx.x] =
             reinterpret_cast<const uint32_t *>(d.qstatus)[threadIdx.x];
-    develop_tile(d, out_r, out_g, out_b, blockIdx.x, blockIdx.y, lds, 0, 0, d.width);
+    /* Workgroups go to the 8 XCDs in turn (workgroup b to XCD b % 8); tile
+     * t = xcd_tile(b) gives each XCD one contiguous run of tiles in raster
+     * order, so the halo rows and edge lines neighbouring tiles share are
+     * read into one XCD's L2. A bijection whatever the dispatch does. */
+    const uint32_t nb = gridDim.x * gridDim.y, b = blockIdx.y * gridDim.x + blockIdx.x;
+    const uint32_t per = nb / 8, extra = nb % 8, xcd = b % 8;
+    const uint32_t t = (xcd < extra ? xcd * (per + 1) : extra * (per + 1) + (xcd - extra) * per) +
+                       b / 8;
+    develop_tile(d, out_r, out_g, out_b, t % gridDim.x, t / gridDim.x, lds, 0, 0, d.width);
 }
 
 /* Plain develop of sampled 64x64 tiles (SAMPLE_PARTS develop tiles each)
