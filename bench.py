@@ -1,18 +1,22 @@
 """Benchmark: plain GPU decode + develop of ProRes RAW frames on one MI355X.
 
-Workload (one step = one frame): the 4096x2160 BGGR frame of the
-plain-GPU-at-scale configuration (bitstream v1, tile alignment 3, qscale 45,
-seeded quant matrix and 8-point curve, 2% of tiles corrupted) is decoded with
-the Apple transform, linearized in the same kernel, and developed to Rec.2020
-planes. The packet is resident in device memory before timing starts. Frames
-stream through the native pipeline (prr_pipeline): frame i + 1's decode runs
-on one HIP stream beside frame i's develop on another. The serial step
-(decode then develop, one HIP graph a frame) is reported beside it.
+Workload (one step = one frame): the 5760x3240 frame kind of the
+end-to-end stream (RGGB, bitstream v0 with the default curve, tile
+alignment 3, qscale 45, seeded scene of 8 gradients + 200 discs with
+Poisson-Gaussian noise) is decoded with the Apple transform, linearized in
+the same kernel, and developed to Rec.2020 planes. The packet is resident
+in device memory before timing starts. Frames stream through the native
+pipeline (prr_pipeline): frame i + 1's decode runs on one HIP stream beside
+frame i's develop on another. The serial step (decode then develop, one HIP
+graph a frame) is reported beside it, and the 4096x2160 frame (BGGR v1 a3
+q45, seeded quant matrix and curve, 2% corrupt tiles) as a leg in both
+inverse transforms.
 
 Prints one JSON line. `value` is Gsensels/s (visible width x height per
-second). `roofline` covers the develop kernel (the dominant one) against
-HBM bandwidth; `cpu_baseline` times the oracle (a C restatement of the
-crate's CPU path, threaded like its rayon code) on a bounded sample.
+second). `roofline` covers the kernel that takes longest inside the stream
+against HBM bandwidth, with its VALU issue rate beside it; `cpu_baseline`
+times the oracle (a C restatement of the crate's CPU path, threaded like its
+rayon code) on the same packet.
 """
 import argparse
 import ctypes
@@ -30,13 +34,14 @@ sys.path.insert(0, ROOT)
 def parse_args():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--width", type=int, default=4096)
-    ap.add_argument("--height", type=int, default=2160)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=30)
+    ap.add_argument("--width", type=int, default=5760)
+    ap.add_argument("--height", type=int, default=3240)
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-c1", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-frames", type=int, default=64)
     ap.add_argument("--no-learned", action="store_true")
@@ -45,47 +50,89 @@ def parse_args():
     ap.add_argument("--no-dng", action="store_true")
     ap.add_argument("--no-orient", action="store_true")
     ap.add_argument("--clip-frames", type=int, default=160)
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="batches keep the default launch shape (counter runs: one shape)")
+    ap.add_argument("--only", choices=["plain", "c1", "batch", "learned", "clip", "dng", "orient"],
+                    help="run one leg only (profiling); no CPU baseline")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device to device 10 times (the reference reading "
                          "of the HBM byte counters) and exit")
-    return ap.parse_args()
+    args = ap.parse_args()
+    if args.only:
+        for leg in ("c1", "batch", "learned", "clip", "dng", "orient"):
+            setattr(args, "no_" + leg, args.only != leg)
+        args.no_cpu = True
+    return args
 
 
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_plain4096x2160.txt")
+PROFILES = os.path.join(ROOT, "profiles")
+TRAFFIC_FILE = os.path.join(PROFILES, "hbm_traffic.txt")
+UTIL_FILE = os.path.join(PROFILES, "issue_utilization.txt")
+N_CU = 256          # MI355X compute units
+CLOCK_HZ = 2.4e9    # MI355X peak engine clock
+HBM_PEAK = 8000.0   # GB/s
 
 
-def measured_traffic(kernel, width, height):
-    """HBM bytes per launch of `kernel` from the committed PMC summary
-    (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by what the
-    counter reads for a copy of known size), when it covers this frame
-    size; else None."""
-    if (width, height) != (4096, 2160) or not os.path.exists(TRAFFIC_FILE):
+T_START = time.perf_counter()
+
+
+def log(msg):
+    """A progress line on stderr as each leg ends (stdout holds the JSON
+    line only)."""
+    print(f"[bench {time.perf_counter() - T_START:7.1f} s] {msg}", file=sys.stderr, flush=True)
+
+
+def profile_fields(path, kernel, workload):
+    """The key=value fields of `kernel` on `workload` in a committed counter
+    summary (lines `kernel workload key=value ...`), or None."""
+    if not os.path.exists(path):
         return None
-    for line in open(TRAFFIC_FILE):
+    for line in open(path):
         f = line.split()
-        if not f or f[0] != kernel:
-            continue
-        kv = dict(x.split("=") for x in f[1:])
-        return int(1024 * (float(kv["fetch_kib"]) * float(kv["fetch_scale"]) +
-                           float(kv["write_kib"]) * float(kv["write_scale"])))
+        if len(f) > 2 and f[0] == kernel and f[1] == workload:
+            return dict(x.split("=", 1) for x in f[2:])
     return None
 
 
-UTIL_FILE = os.path.join(ROOT, "profiles", "issue_utilization.txt")
-
-
-def measured_valu_rate(kernel, width, height):
-    """VALU instructions `kernel` issues per SIMD-cycle on this frame size,
-    from the committed counter summary (SQ_INSTS_VALU over cycles x SIMDs),
-    or None."""
-    if (width, height) != (4096, 2160) or not os.path.exists(UTIL_FILE):
+def measured_traffic(kernel, workload):
+    """HBM bytes per launch: FETCH_SIZE and WRITE_SIZE per dispatch, each
+    scaled by what the counter reads for a copy of known size."""
+    kv = profile_fields(TRAFFIC_FILE, kernel, workload)
+    if kv is None:
         return None
-    for line in open(UTIL_FILE):
-        f = line.split()
-        if len(f) > 2 and f[0] == kernel and f[1] == "plain4096x2160":
-            kv = dict(x.split("=") for x in f[2:])
-            return float(kv["valu_per_simd_cycle"])
-    return None
+    return int(1024 * (float(kv["fetch_kib"]) * float(kv["fetch_scale"]) +
+                       float(kv["write_kib"]) * float(kv["write_scale"])))
+
+
+def measured_valu(kernel, workload):
+    """(SQ_INSTS_VALU per dispatch, VALU instructions per SIMD-cycle) of
+    `kernel` on `workload`, or (None, None)."""
+    kv = profile_fields(UTIL_FILE, kernel, workload)
+    if kv is None:
+        return None, None
+    return float(kv["SQ_INSTS_VALU"]), float(kv["valu_per_simd_cycle"])
+
+
+def decode_bound(kernel, workload, coded_bits, tiles, kernel_ms):
+    """The decode against its per-CU bit-rate bound: the kernel's own VALU
+    instruction stream (wave64 instructions per dispatch, from the counter
+    summary) spread over every CU at one wave instruction per CU-cycle (four
+    SIMDs, four cycles a wave64 instruction) at the peak clock gives the
+    shortest time those coded bits could take with this kernel's
+    instructions per bit; the fraction is that time over the measured one."""
+    insts, _ = measured_valu(kernel, workload)
+    out = {"kernel": kernel, "coded_bits": int(coded_bits), "kernel_ms": round(kernel_ms, 4),
+           "tiles_per_s": round(tiles / (kernel_ms * 1e-3), 1), "valu_source":
+           os.path.relpath(UTIL_FILE, ROOT)}
+    if insts is None:
+        return out
+    t_bound = insts / (N_CU * CLOCK_HZ)
+    out.update({"valu_insts": insts, "lane_ops_per_bit": round(64.0 * insts / coded_bits, 2),
+                "bound_ms": round(t_bound * 1e3, 4),
+                "bound_gbit_per_s": round(coded_bits / t_bound / 1e9, 1),
+                "bound_tiles_per_s": round(tiles / t_bound, 1),
+                "frac": round(t_bound / (kernel_ms * 1e-3), 4)})
+    return out
 
 
 def calibrate_copy(mib):
@@ -102,37 +149,72 @@ def calibrate_copy(mib):
     print(json.dumps({"calibrate_copy_mib": mib, "copies": 10, "ok": bool((dst == src).all())}))
 
 
-def main():
-    args = parse_args()
-    if args.calibrate_copy:
-        calibrate_copy(args.calibrate_copy)
-        return
-    if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are out of scope "
-                                   "(frames shard across GPUs with no exchange)"}))
-        sys.exit(2)
+def device_packet(packet):
+    import torch
+    padded = ((len(packet) + 3) & ~3) + 16
+    d = torch.zeros(padded, dtype=torch.uint8, device="cuda")
+    d[:len(packet)] = torch.frombuffer(bytearray(packet), dtype=torch.uint8).to("cuda")
+    return d
+
+
+def pipelined(packet, d_packet, hdr, info, planes, stage, idct, steps, warmup, time_kernels=True):
+    """The pipelined stream of frames (prr_pipeline, four buffer sets): one
+    step = one frame submitted; the timed region is bracketed by device
+    synchronization, so it holds exactly `steps` whole frames. Returns
+    (seconds, mean decode ms, mean develop ms inside the stream)."""
+    import torch
+
+    from proresraw_rs_amd.pipeline import Pipeline
+    pipe = Pipeline(depth=4)
+    status = torch.zeros(4, dtype=torch.int32, device="cuda")
+
+    def submit():
+        pipe.submit(hdr, info, d_packet, len(packet), planes, stage=stage, idct=idct,
+                    status=status)
+
+    for _ in range(warmup):
+        submit()
+    pipe.sync()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        submit()
+    pipe.sync()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    st = status.cpu().numpy().view(np.uint32)
+    if st[0] != 0xFFFFFFFF or st[3]:
+        raise RuntimeError(f"device status {st}")
+    dec_ms = dev_ms = None
+    if time_kernels:
+        # Kernel durations inside the stream, with HIP events on the streams
+        # the kernels run on (each shares the chip with the other there).
+        pipe.timing(True)
+        for _ in range(min(steps, 60)):
+            submit()
+        dec_ms, dev_ms, _ = pipe.timing(False)
+    pipe.close()
+    return elapsed, dec_ms, dev_ms
+
+
+def plain_leg(args, packet, workload):
+    """The plain path on one frame kind: the serial step (one HIP graph a
+    frame), per-kernel times alone, and the pipelined stream."""
     import torch
 
     import proresraw_rs_amd as P
     from proresraw_rs_amd import _native as N
-    from proresraw_rs_amd import synth as S
-    from proresraw_rs_amd.workloads import config1
 
-    dev = torch.device("cuda:0")
     lib = N.lib()
     ctx = P.context()
     stream = torch.cuda.Stream()
-    sc = config1(args.width, args.height)
-    packet = S.encode_scene(sc)
     hdr, info = P.parse_frame_info(packet)
     raw_hdr = hdr._raw
     w, h = hdr.width, hdr.height
     cw, ch = info.coded_width, info.coded_height
     npix = w * h
-
-    padded = ((len(packet) + 3) & ~3) + 16
-    d_packet = torch.zeros(padded, dtype=torch.uint8, device=dev)
-    d_packet[:len(packet)] = torch.frombuffer(bytearray(packet), dtype=torch.uint8).to(dev)
+    dev = torch.device("cuda:0")
+    d_packet = device_packet(packet)
     d_samples = torch.empty(cw * ch, dtype=torch.int16, device=dev)
     d_linear = torch.empty(npix, dtype=torch.int16, device=dev)
     d_flags = torch.empty(max(info.nb_tiles, 1), dtype=torch.uint8, device=dev)
@@ -170,7 +252,6 @@ def main():
     if st[0] != 0xFFFFFFFF or st[3]:
         raise RuntimeError(f"device status {st}")
 
-    # Per-kernel durations with HIP events on the launch stream.
     def timed(fn, n=30):
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         fn()
@@ -184,8 +265,6 @@ def main():
     t_decode_ms = timed(decode)
     t_develop_ms = timed(develop)
 
-    # The serial step (decode then develop on one stream, one HIP graph a
-    # frame), reported beside the pipelined stream of frames below.
     graph = None
     if not args.no_graph:
         graph = torch.cuda.CUDAGraph()
@@ -200,80 +279,138 @@ def main():
         else:
             step()
 
+    serial_steps = max(args.steps // 2, 10)
     with torch.cuda.stream(stream):
         for _ in range(args.warmup):
             run_step()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        for _ in range(args.steps):
+        for _ in range(serial_steps):
             run_step()
         torch.cuda.synchronize()
-        elapsed = time.perf_counter() - t0
-    serial_ms = 1e3 * elapsed / args.steps
-    serial_value = npix * args.steps / elapsed / 1e9
+        serial_s = time.perf_counter() - t0
 
-    # The pipelined stream of frames (prr_pipeline): frame i + 1's decode on
-    # one HIP stream beside frame i's develop on another, four buffer sets.
-    # One step = one frame submitted; the timed region is bracketed by device
-    # synchronization, so it holds exactly `steps` whole frames.
-    from proresraw_rs_amd.pipeline import Pipeline
-    pipe = Pipeline(depth=4)
-    pipe_status = torch.zeros(4, dtype=torch.int32, device=dev)
+    elapsed, pipe_dec_ms, pipe_dev_ms = pipelined(packet, d_packet, hdr, info, planes,
+                                                  P.Stage.Rec2020, P.Idct.Apple, args.steps,
+                                                  args.warmup)
+    table_bytes = 2 * info.nb_tiles
+    coded_bits = 8 * (len(packet) - info.header_end - table_bytes)
+    return {
+        "npix": npix, "width": w, "height": h, "packet_bytes": len(packet),
+        "nb_tiles": info.nb_tiles, "coded_width": cw, "coded_height": ch,
+        "value": npix * args.steps / elapsed / 1e9, "ms_per_step": 1e3 * elapsed / args.steps,
+        "serial_value": npix * serial_steps / serial_s / 1e9,
+        "serial_ms": 1e3 * serial_s / serial_steps, "serial_graph": graph is not None,
+        "decode_ms_alone": t_decode_ms, "develop_ms_alone": t_develop_ms,
+        "decode_ms": pipe_dec_ms, "develop_ms": pipe_dev_ms,
+        "coded_bits": coded_bits, "workload_key": workload,
+    }
 
-    def submit():
-        pipe.submit(hdr, info, d_packet, len(packet), planes, stage=P.Stage.Rec2020,
-                    status=pipe_status)
 
-    for _ in range(args.warmup):
-        submit()
-    pipe.sync()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        submit()
-    pipe.sync()
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    st = pipe_status.cpu().numpy().view(np.uint32)
-    if st[0] != 0xFFFFFFFF or st[3]:
-        raise RuntimeError(f"device status {st}")
-    # Kernel durations inside the pipelined stream, with HIP events on the
-    # streams the kernels run on (the develop shares the chip with the next
-    # frame's decode there).
-    pipe.timing(True)
-    for _ in range(min(args.steps, 60)):
-        submit()
-    pipe_decode_ms, pipe_develop_ms, _ = pipe.timing(False)
-    pipe.close()
-    ms_per_step = 1e3 * elapsed / args.steps
-    value = npix * args.steps / elapsed / 1e9
-
-    # Develop roofline: algorithmic bytes = linear values read (2 B per visible
-    # sensel) + three f32 planes written (12 B per visible sensel).
+def roofline_of(r):
+    """The roofline object for the kernel that takes longest inside the
+    stream. Algorithmic bytes a launch: develop reads the linear picture
+    (2 B/sensel) and writes three f32 planes (12 B/sensel); the decode
+    (prepare + decode) reads the packet and writes code values (2 B per
+    coded sensel) and linear values (2 B per visible sensel)."""
+    npix, key = r["npix"], r["workload_key"]
     dev_bytes = 14.0 * npix
-    achieved = dev_bytes / (pipe_develop_ms * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": 8000.0, "unit": "GB/s",
-                "frac": round(achieved / 8000.0, 4),
-                "traffic": measured_traffic("k_develop", w, h),
-                "traffic_source": os.path.relpath(TRAFFIC_FILE, ROOT),
-                "valu_insts_per_simd_cycle": measured_valu_rate("k_develop", w, h),
-                "valu_source": os.path.relpath(UTIL_FILE, ROOT),
-                "kernel": "k_develop", "kernel_ms": round(pipe_develop_ms, 4),
-                "kernel_ms_alone": round(t_develop_ms, 4),
-                "bytes_per_launch": int(dev_bytes),
-                "decode_ms": round(pipe_decode_ms, 4),
-                "decode_ms_alone": round(t_decode_ms, 4),
-                "decode_tiles_per_s": round(info.nb_tiles / (t_decode_ms * 1e-3), 1)}
+    dec_bytes = float(r["packet_bytes"]) + 2.0 * r["coded_width"] * r["coded_height"] + 2.0 * npix
+    develop_dominant = r["develop_ms"] >= r["decode_ms"]
+    kernel = "k_develop" if develop_dominant else "k_decode"
+    nbytes = dev_bytes if develop_dominant else dec_bytes
+    ms = r["develop_ms"] if develop_dominant else r["decode_ms"]
+    achieved = nbytes / (ms * 1e-3) / 1e9
+    insts, rate = measured_valu(kernel, key)
+    out = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK, "unit": "GB/s",
+           "frac": round(achieved / HBM_PEAK, 4), "traffic": measured_traffic(kernel, key),
+           "traffic_source": os.path.relpath(TRAFFIC_FILE, ROOT), "kernel": kernel,
+           "kernel_ms": round(ms, 4),
+           "kernel_ms_alone": round(r["develop_ms_alone" if develop_dominant else
+                                      "decode_ms_alone"], 4),
+           "bytes_per_launch": int(nbytes), "bytes_per_unit": "14 B/visible sensel"
+           if develop_dominant else "packet + 2 B/coded sensel + 2 B/visible sensel"}
+    # The kernel's limiter is its VALU issue, not HBM: instructions per
+    # SIMD-cycle from the counter summary (a stream of wave64 FP32
+    # instructions tops out at 0.25).
+    if rate is not None:
+        out.update({"limiter": "valu_issue", "valu_insts_per_simd_cycle": rate,
+                    "valu_issue_frac_of_0.25": round(rate / 0.25, 3),
+                    "valu_source": os.path.relpath(UTIL_FILE, ROOT)})
+    other = "k_decode" if develop_dominant else "k_develop"
+    out["other_kernel"] = {"kernel": other, "kernel_ms": round(r["decode_ms" if develop_dominant
+                                                                 else "develop_ms"], 4),
+                           "traffic": measured_traffic(other, key)}
+    out["decode_bitrate"] = decode_bound("k_decode", key, r["coded_bits"], r["nb_tiles"],
+                                         r["decode_ms_alone"])
+    out["decode_bitrate_frac"] = out["decode_bitrate"].get("frac")
+    return out
 
-    batched = None if args.no_batch else batched_bench(args)
-    learned = None if args.no_learned else learned_bench(args, value)
-    clip = None if args.no_clip else clip_bench(args)
-    dng = None if args.no_dng else dng_bench(args)
-    orient = None if args.no_orient else orient_bench(args)
 
-    cpu = None
-    if not args.no_cpu:
-        cpu = cpu_baseline(packet, args.cpu_seconds)
+def c1_leg(args):
+    """The 4096x2160 frame (BGGR v1 a3 q45, seeded quant matrix and curve, 2%
+    corrupt tiles) through the pipelined stream in each inverse transform,
+    developed to Rec.2020 (the transforms are bit-exact against the oracle
+    in the GPU tests)."""
+    import torch
+
+    import proresraw_rs_amd as P
+    from proresraw_rs_amd import synth as S
+    from proresraw_rs_amd.workloads import config1
+
+    packet = S.encode_scene(config1(4096, 2160))
+    hdr, info = P.parse_frame_info(packet)
+    d_packet = device_packet(packet)
+    npix = hdr.width * hdr.height
+    planes = [torch.empty(npix, dtype=torch.float32, device="cuda") for _ in range(3)]
+    out = {"workload": "4096x2160 BGGR v1 a3 q45, seeded qmat + curve, 2% corrupt tiles; "
+                       "pipelined decode+linearize+develop(Rec2020)", "packet_bytes": len(packet)}
+    steps = max(args.steps // 2, 10)
+    for idct in (P.Idct.Apple, P.Idct.Ffmpeg):
+        el, dec, dv = pipelined(packet, d_packet, hdr, info, planes, P.Stage.Rec2020, idct,
+                                steps, args.warmup)
+        out[idct.name.lower()] = {"value": round(npix * steps / el / 1e9, 3),
+                                  "ms_per_frame": round(1e3 * el / steps, 4),
+                                  "decode_ms": round(dec, 4), "develop_ms": round(dv, 4)}
+    return out
+
+
+def main():
+    args = parse_args()
+    if args.calibrate_copy:
+        calibrate_copy(args.calibrate_copy)
+        return
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are out of scope "
+                                   "(frames shard across GPUs with no exchange)"}))
+        sys.exit(2)
+    import torch  # noqa: F401
+
+    from proresraw_rs_amd import synth as S
+    from proresraw_rs_amd.workloads import stream_frame
+
+    packet = S.encode_scene(stream_frame(args.width, args.height))
+    key = f"plain{args.width}x{args.height}"
+    log(f"encoded the {args.width}x{args.height} packet ({len(packet)} bytes)")
+    r = plain_leg(args, packet, key)
+    value = r["value"]
+    roofline = roofline_of(r)
+    log(f"plain leg: {value:.2f} Gsensels/s pipelined, {r['serial_value']:.2f} serial")
+
+    def leg(name, skip, fn, *a):
+        if skip:
+            return None
+        out = fn(*a)
+        log(f"{name} leg done")
+        return out
+
+    c1 = leg("4096x2160 both transforms", args.no_c1, c1_leg, args)
+    batched = leg("batched", args.no_batch, batched_bench, args)
+    learned = leg("batched + learned", args.no_learned, learned_bench, args, value)
+    clip = leg("clip", args.no_clip, clip_bench, args)
+    dng = leg("DNG tiles", args.no_dng, dng_bench, args)
+    orient = leg("orientation", args.no_orient, orient_bench, args)
+    cpu = leg("CPU baseline", args.no_cpu, cpu_baseline, packet, args.cpu_seconds)
 
     line = {
         "metric": "Gsensels/s decode+develop, plain vs batched+learned demosaic at >=45 dB PSNR",
@@ -282,20 +419,25 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 4),
+        "ms_per_step": round(r["ms_per_step"], 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "u16+f32",
         "data": "synthetic",
-        "config": {"workload": f"plain GPU decode(Apple)+linearize+develop(Rec2020), {w}x{h} BGGR "
-                               "v1 a3 q45, seeded qmat+curve, 2% corrupt tiles; a stream of "
-                               "frames, decode of frame i+1 beside develop of frame i",
-                   "packet_bytes": len(packet), "pipeline_depth": 4,
-                   "serial_value": round(serial_value, 3), "serial_ms_per_step": round(serial_ms, 4),
-                   "serial_graph": graph is not None},
+        "config": {"workload": f"plain GPU decode(Apple)+linearize+develop(Rec2020), "
+                               f"{r['width']}x{r['height']} RGGB v0 a3 q45 (the end-to-end "
+                               "stream's 5760x3240 frame kind); a stream of frames, decode of "
+                               "frame i+1 beside develop of frame i",
+                   "packet_bytes": r["packet_bytes"], "tiles": r["nb_tiles"],
+                   "coded_bits_per_sensel": round(r["coded_bits"] / r["npix"], 3),
+                   "pipeline_depth": 4,
+                   "serial_value": round(r["serial_value"], 3),
+                   "serial_ms_per_step": round(r["serial_ms"], 4),
+                   "serial_graph": r["serial_graph"]},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "c1_4096x2160": c1,
         "batched": batched,
         "batched_learned": learned,
         "clip": clip,
@@ -304,6 +446,7 @@ def main():
     }
     if cpu:
         line["gpu_over_cpu"] = round(value / cpu["value"], 1)
+        line["serial_gpu_over_cpu"] = round(r["serial_value"] / cpu["value"], 1)
     print(json.dumps(line), flush=True)
 
 
@@ -333,26 +476,33 @@ def batched_bench(args):
         distinct.append(S.encode_scene(sc))
     b = Batch()
     sensels = 0
+    out_pixels = 0
+    full_sensels = 0
     n = args.batch_frames
     n_full = 0
     for k in range(n):
         pk = distinct[k % 16]
-        hdr, _ = P.parse_frame_info(pk)
+        hdr, info = P.parse_frame_info(pk)
         sensels += hdr.width * hdr.height
         if k % 2 == 0:
             b.add_full(pk, P.Idct.Apple, develop=P.Stage.WhiteBalanced)
             n_full += 1
+            out_pixels += hdr.width * hdr.height
+            full_sensels += hdr.width * hdr.height
         else:
             j = k // 2
             scale = [P.Scale.Half, P.Scale.Quarter, P.Scale.Eighth][j % 3]
             greens = P.Greens.Average if scale == P.Scale.Half and j % 4 == 0 else P.Greens.First
             b.add_reduced(pk, scale, greens)
+            # A reduced frame's output: (coded w >> s) x (coded h >> s) RGB pixels.
+            out_pixels += (info.coded_width >> int(scale)) * (info.coded_height >> int(scale))
     s = torch.cuda.Stream()
     b.run(s)
     s.synchronize()
     for slot in b.slots:
         slot.check()
-    tuned = b.autotune(s)
+    tuned = b.autotune(s) if not args.no_autotune else {"spw": 64, "chunk": 1, "waves": 0,
+                                                         "cached": None}
     b.run(s)
     s.synchronize()
     g = torch.cuda.CUDAGraph()
@@ -370,8 +520,14 @@ def batched_bench(args):
         torch.cuda.synchronize()
         el = (time.perf_counter() - t0) / reps
     info = b.info()
-    return {"value": round(sensels / el / 1e9, 3), "unit": "Gsensels/s", "frames": n,
-            "full_develop_frames": n_full, "ms_per_batch": round(el * 1e3, 3),
+    return {"value": round(sensels / el / 1e9, 3), "unit": "Gsensels/s",
+            "value_counts": "visible sensels of every frame's packet, reduced frames included "
+                            "(not comparable with full-resolution legs; see output_value)",
+            "output_value": round(out_pixels / el / 1e9, 3),
+            "output_unit": "Gpixels/s of output (full frames w*h developed, reduced frames "
+                           "(coded w >> s)(coded h >> s) RGB)",
+            "full_develop_gsensels": round(full_sensels / 1e9, 4),
+            "frames": n, "full_develop_frames": n_full, "ms_per_batch": round(el * 1e3, 3),
             "gsensels": round(sensels / 1e9, 4), "tile_groups": info["tile_groups"],
             "persistent_grid": info["grid"], "lds_bytes": info["lds_bytes"],
             "autotuned": {"spw": tuned["spw"], "chunk": tuned["chunk"], "waves": tuned["waves"],
@@ -409,11 +565,15 @@ def learned_bench(args, plain_value):
     b = Batch()
     frames = []
     sensels = 0
+    coded_bits = 0
+    tiles = 0
     for j in range(args.learned_frames):
         pk = distinct[j % 4]
         slot = b.add_full(pk, P.Idct.Apple, linear=True)
         h = slot.header
         sensels += h.width * h.height
+        coded_bits += 8 * (len(pk) - slot.info.header_end - 2 * slot.info.nb_tiles)
+        tiles += slot.info.nb_tiles
         planes = [torch.empty(h.width * h.height, dtype=torch.float32, device="cuda")
                   for _ in range(3)]
         frames.append((slot, P.DemosaicParams.new(h, 0), planes))
@@ -447,7 +607,10 @@ def learned_bench(args, plain_value):
         s.synchronize()
         for slot, _, _ in frames:
             slot.check()
-        b.autotune(s)
+        tuned = b.autotune(s) if not args.no_autotune else {"spw": 64, "chunk": 1}
+        # The batched decode alone (scan, queue, persistent decode with the
+        # fused linearize), HIP events on its stream.
+        batch_ms = b.time_run(s, reps=5)
         # Fallbacks and sampled-tile PSNR, frame by frame (eager, once).
         fallbacks, worst = 0, float("inf")
         b.run(s)
@@ -480,6 +643,11 @@ def learned_bench(args, plain_value):
             "vs_batched_plain": round(v_learned / v_plain, 3),
             "vs_plain_gpu_path": round(v_learned / plain_value, 3),
             "fallback_frames": fallbacks, "min_sampled_tile_psnr_db": round(worst, 2),
+            "batch_decode_ms": round(batch_ms, 3),
+            "batch_decode_autotuned": {"spw": tuned["spw"], "chunk": tuned["chunk"]},
+            "coded_bits_per_sensel": round(coded_bits / sensels, 3),
+            "decode_bitrate": decode_bound("k_batch_decode", "learned32", coded_bits, tiles,
+                                           batch_ms),
             "workload": "32 frames alternating 8192x4320 v1 a4 q15 and 5760x3240 v0 a3 q45, "
                         "batched decode+linearize, learned demosaic+WB+Rec2020 with fallback"}
 
@@ -643,26 +811,56 @@ def orient_bench(args):
     return {"workload": "crop (8, 8, 4, 4) + orientation of three 5760x3240 f32 planes", **res}
 
 
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(packet, seconds):
-    """The oracle's decode + linearize + develop(Rec2020) of the same packet,
-    repeated for about `seconds`, on every host core (capped at 16)."""
+    """The oracle (the C restatement of the crate's rayon path: tile rows
+    for decoding, 32-row bands for develop) on the same packet:
+    decode_frame then develop(&frame, Rec2020), which linearizes inside as
+    the crate's does. Method of the crate's examples/bench.rs:121-140: one
+    warm-up run, then timed runs (at least 5, for about `seconds`),
+    reporting the median and the fastest. Threads: every CPU this process
+    may run on, up to 16, the host's share for one GPU; one more run on a
+    single thread gives the per-core rate."""
     from oracle import oracle as O
-    threads = O.THREADS
-    t0 = time.perf_counter()
-    n = 0
-    while True:
-        h, s, sk, mq = O.decode_frame(packet, threads=threads)
-        O.linearize(h, s, threads=threads)
+    affinity = len(os.sched_getaffinity(0))
+    threads = min(affinity, 16)
+
+    def run(th):
+        h, s, sk, mq = O.decode_frame(packet, threads=th)
         prm = O.params(h, h.quant_matrix[0] * mq)
-        O.develop_planar(h, s, prm, 2, threads=threads)
-        n += 1
-        if time.perf_counter() - t0 >= seconds:
-            break
-    el = time.perf_counter() - t0
+        O.develop_planar(h, s, prm, 2, threads=th)
+        return h
+
+    h = run(threads)
+    times = []
+    t_start = time.perf_counter()
+    while len(times) < 5 or (time.perf_counter() - t_start < seconds and len(times) < 200):
+        t0 = time.perf_counter()
+        run(threads)
+        times.append(time.perf_counter() - t0)
+    t0 = time.perf_counter()
+    run(1)
+    single = time.perf_counter() - t0
     npix = h.width * h.height
-    return {"value": round(npix * n / el / 1e9, 5), "unit": "Gsensels/s", "cores": threads,
-            "kind": "port", "sample": f"{n} frames of the same {h.width}x{h.height} packet, "
-                                      f"{el:.1f} s, median-free mean"}
+    med = float(np.median(times))
+    fastest = min(times)
+    return {"value": round(npix / med / 1e9, 5), "unit": "Gsensels/s", "cores": threads,
+            "kind": "port", "median_ms": round(med * 1e3, 2), "fastest_ms": round(fastest * 1e3, 2),
+            "fastest_value": round(npix / fastest / 1e9, 5), "runs": len(times),
+            "single_thread_value": round(npix / single / 1e9, 5),
+            "nproc": os.cpu_count(), "affinity_cpus": affinity, "cpu_model": cpu_model(),
+            "sample": f"decode_frame + develop(Rec2020) of the {h.width}x{h.height} packet: one "
+                      f"warm-up, then {len(times)} timed runs ({sum(times):.1f} s) on {threads} "
+                      "threads, median reported; plus one run on 1 thread"}
 
 
 if __name__ == "__main__":

@@ -237,11 +237,22 @@ int prr_pipeline_create(prr_context *ctx, uint32_t depth, prr_pipeline **out, pr
  * must stay valid until the pipeline is joined or synchronized): decode with
  * `idct`, then develop(&frame, stage) (explicit_wb as
  * prr_develop_frame_device) into the planes. d_status_out, when not NULL,
- * receives the frame's prr_device_status after its develop. Returns at once. */
+ * receives the frame's prr_device_status after its develop. Returns at once;
+ * an argument it refuses queues no work.
+ * Ordering: the decode runs on a stream of the pipeline's own and the
+ * develop on another, so neither waits for work the caller queued on its
+ * streams. A caller that writes the packet, or still reads earlier planes,
+ * on a stream of its own calls prr_pipeline_wait(p, that stream) first;
+ * then prr_pipeline_join (or _sync) before reading the planes. */
 int prr_pipeline_submit(prr_pipeline *p, const prr_header *hdr, const prr_frame_info *info,
                         const uint8_t *d_packet, size_t len, int idct, int explicit_wb,
                         float wb_red, float wb_blue, int stage, float *d_r, float *d_g,
                         float *d_b, prr_device_status *d_status_out, prr_error *err);
+/* Make the pipeline wait, on the device, for the work queued on `stream`
+ * so far: frames submitted after this call start their decode and their
+ * develop only after it (a packet uploaded or planes read on the caller's
+ * stream). */
+int prr_pipeline_wait(prr_pipeline *p, void *stream, prr_error *err);
 /* Make `stream` wait, on the device, for every frame submitted so far. */
 int prr_pipeline_join(prr_pipeline *p, void *stream, prr_error *err);
 /* Wait on the host for every frame submitted so far. */

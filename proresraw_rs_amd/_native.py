@@ -167,6 +167,7 @@ SIGNATURES = {
     "prr_pipeline_create": (i32, [c_vp, u32, ctypes.POINTER(c_vp), E]),
     "prr_pipeline_submit": (i32, [c_vp, H, FI, c_vp, sz, i32, i32, ctypes.c_float, ctypes.c_float,
                                   i32, c_vp, c_vp, c_vp, c_vp, E]),
+    "prr_pipeline_wait": (i32, [c_vp, c_vp, E]),
     "prr_pipeline_join": (i32, [c_vp, c_vp, E]),
     "prr_pipeline_sync": (i32, [c_vp, E]),
     "prr_pipeline_timing": (i32, [c_vp, i32, ctypes.POINTER(ctypes.c_float),

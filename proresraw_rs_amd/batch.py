@@ -88,9 +88,12 @@ class Batch:
         self.slots.append(slot)
         return slot
 
-    def add_full(self, packet, idct=Idct.Apple, develop=None, wb=None, linear=False):
+    def add_full(self, packet, idct=Idct.Apple, develop=None, wb=None, linear=False,
+                 interleaved=False):
         """A full decode (decode_frame_with); with develop=Stage, also
-        develop(&frame, stage) (or with_white_balance(wb) when wb is given)."""
+        develop(&frame, stage) (or with_white_balance(wb) when wb is given)
+        into three planes, or with interleaved=True into one interleaved RGB
+        buffer (develop_into's layout)."""
         torch = _torch()
         header, info = parse_frame_info(packet)
         slot = Slot(header, info, packet, KIND_DEVELOP if develop is not None else KIND_DECODE)
@@ -105,12 +108,17 @@ class Batch:
         bf.d_samples = slot.samples.data_ptr()
         bf.d_linear = slot.linear.data_ptr() if linear else None
         if develop is not None:
-            slot.planes = [torch.zeros(npix, dtype=torch.float32, device=dev) for _ in range(3)]
             bf.stage = int(develop)
             bf.explicit_wb = 1 if wb is not None else 0
             if wb is not None:
                 bf.wb_red, bf.wb_blue = wb
-            bf.d_r, bf.d_g, bf.d_b = [p.data_ptr() for p in slot.planes]
+            if interleaved:
+                slot.planes = [torch.zeros(3 * npix, dtype=torch.float32, device=dev)]
+                bf.d_r, bf.d_g, bf.d_b = slot.planes[0].data_ptr(), None, None
+            else:
+                slot.planes = [torch.zeros(npix, dtype=torch.float32, device=dev)
+                               for _ in range(3)]
+                bf.d_r, bf.d_g, bf.d_b = [p.data_ptr() for p in slot.planes]
         return self._add(slot, bf)
 
     def add_reduced(self, packet, scale: Scale, greens: Greens = Greens.First):

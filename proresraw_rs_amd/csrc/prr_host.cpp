@@ -833,6 +833,7 @@ struct prr_pipeline {
     prr_context *ctx = nullptr;   /* the caller's: device and launch shape */
     prr_context *dctx = nullptr;  /* decode stream and the decode's scratch */
     hipStream_t develop = nullptr;
+    hipEvent_t caller = nullptr;  /* recorded on a caller's stream by prr_pipeline_wait */
     std::vector<PipeSet> sets;
     uint64_t submitted = 0;
     bool timing = false;
@@ -854,6 +855,8 @@ static void pipeline_release(prr_pipeline *p)
             if (e)
                 (void)hipEventDestroy(e);
     }
+    if (p->caller)
+        (void)hipEventDestroy(p->caller);
     if (p->develop)
         (void)hipStreamDestroy(p->develop);
     if (p->dctx)
@@ -906,6 +909,8 @@ extern "C" int prr_pipeline_create(prr_context *ctx, uint32_t depth, prr_pipelin
         p->dctx->stream = decode_stream;
         e = dedicated_stream(ctx->device, &p->develop);
     }
+    if (e == hipSuccess)
+        e = hipEventCreateWithFlags(&p->caller, hipEventDisableTiming);
     for (PipeSet &st : p->sets) {
         for (hipEvent_t *ev : {&st.decoded, &st.developed})
             if (e == hipSuccess)
@@ -959,6 +964,14 @@ extern "C" int prr_pipeline_submit(prr_pipeline *p, const prr_header *hdr,
         return fail(err, PRR_E_INVALID_ARGUMENT, "planes");
     if (stage < PRR_STAGE_DEMOSAICED || stage > PRR_STAGE_REC2020)
         return fail(err, PRR_E_INVALID_ARGUMENT, "stage", (uint64_t)stage);
+    /* Every argument the decode and the develop check, checked before
+     * either is queued: a refused submit queues nothing. */
+    if (((uintptr_t)d_r | (uintptr_t)d_g | (uintptr_t)d_b) & 7)
+        return fail(err, PRR_E_INVALID_ARGUMENT, "plane alignment");
+    if ((uintptr_t)d_packet & 3)
+        return fail(err, PRR_E_INVALID_ARGUMENT, "packet alignment");
+    if (idct != PRR_IDCT_APPLE && idct != PRR_IDCT_FFMPEG)
+        return fail(err, PRR_E_INVALID_ARGUMENT, "samples or idct");
     HIP_TRY(hipSetDevice(p->ctx->device), "hipSetDevice");
     PipeSet &st = p->sets[p->submitted % p->sets.size()];
     hipStream_t ds = p->dctx->stream, vs = p->develop;
@@ -1007,6 +1020,20 @@ extern "C" int prr_pipeline_submit(prr_pipeline *p, const prr_header *hdr,
     st.used = true;
     st.timed = timed;
     p->submitted++;
+    return PRR_OK;
+}
+
+extern "C" int prr_pipeline_wait(prr_pipeline *p, void *stream, prr_error *err)
+{
+    if (!p)
+        return fail(err, PRR_E_INVALID_ARGUMENT, "pipeline");
+    HIP_TRY(hipSetDevice(p->ctx->device), "hipSetDevice");
+    /* An event at the end of the caller's stream; both pipeline streams
+     * wait for it on the device (the wait takes the event as recorded now,
+     * so the one event serves every call). */
+    HIP_TRY(hipEventRecord(p->caller, (hipStream_t)stream), "hipEventRecord");
+    HIP_TRY(hipStreamWaitEvent(p->dctx->stream, p->caller, 0), "hipStreamWaitEvent");
+    HIP_TRY(hipStreamWaitEvent(p->develop, p->caller, 0), "hipStreamWaitEvent");
     return PRR_OK;
 }
 

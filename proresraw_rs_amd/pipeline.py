@@ -36,7 +36,16 @@ class Pipeline:
         if rc:
             raise Error(err)
 
+    def wait(self, stream):
+        """Frames submitted from now on start after the work queued on
+        `stream` so far (a packet uploaded there, planes read there)."""
+        err = N.PrrError()
+        if self.lib.prr_pipeline_wait(self.handle, ctypes.c_void_p(stream.cuda_stream),
+                                      ctypes.byref(err)):
+            raise Error(err)
+
     def join(self, stream):
+        """`stream` waits for every frame submitted so far."""
         err = N.PrrError()
         if self.lib.prr_pipeline_join(self.handle, ctypes.c_void_p(stream.cuda_stream),
                                       ctypes.byref(err)):

@@ -44,3 +44,17 @@ def config1(width=4096, height=2160, seed=2):
                    qmat=seeded_qmat(seed + 100), curve=seeded_curve(seed + 200),
                    white_level=4095, corrupt_fraction=0.02, color_matrix=seeded_matrix(seed + 300),
                    gain=1.0, wb=(1.9, 1.4))
+
+
+def stream_frame(width=5760, height=3240, seed=3001):
+    """The 5760x3240 frame kind of the end-to-end stream: RGGB, v0 (default
+    curve), alignment 3, qscale 45, default quant matrix, white 4095."""
+    return S.scene(width, height, 0, seed=seed, version=0, alignment=3, qscale=(45, 45),
+                   white_level=4095)
+
+
+def stream_frame_8k(width=8192, height=4320, seed=3000):
+    """The 8192x4320 frame kind of the end-to-end stream: RGGB, v1 with a
+    seeded 8-point curve, alignment 4, qscale 15."""
+    return S.scene(width, height, 0, seed=seed, version=1, alignment=4, qscale=(15, 15),
+                   curve=seeded_curve(40), white_level=4095)

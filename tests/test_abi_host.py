@@ -187,7 +187,23 @@ def test_new_entry_points_reject_bad_arguments_before_device_work():
     assert lib.prr_pipeline_submit(None, None, None, None, 0, 0, 0, 0.0, 0.0, 2, None, None, None,
                                    None, ctypes.byref(err)) == 101
     assert lib.prr_pipeline_sync(None, ctypes.byref(err)) == 101
+    assert lib.prr_pipeline_wait(None, None, ctypes.byref(err)) == 101
     lib.prr_pipeline_destroy(None)
+    # A submit whose planes, packet or transform are refused queues nothing:
+    # every check comes before the pipeline itself is touched (the handle
+    # below is never dereferenced).
+    fake_pipe = ctypes.c_void_p(8)
+    hdr, info = N.PrrHeader(), N.PrrFrameInfo()
+    p8, p4 = ctypes.c_void_p(64), ctypes.c_void_p(68)
+    for packet, idct, r, g, b, what in ((p8, 0, p4, p8, p8, b"plane alignment"),
+                                        (p8, 0, p8, p8, p4, b"plane alignment"),
+                                        (ctypes.c_void_p(66), 0, p8, p8, p8, b"packet alignment"),
+                                        (p8, 2, p8, p8, p8, b"samples or idct")):
+        err = N.PrrError()
+        assert lib.prr_pipeline_submit(fake_pipe, ctypes.byref(hdr), ctypes.byref(info), packet,
+                                       16, idct, 0, 0.0, 0.0, 2, r, g, b, None,
+                                       ctypes.byref(err)) == 101
+        assert err.what == what
     # DNG tiles: a picture narrower than two sensels, a stride below the width.
     buf = (ctypes.c_uint16 * 16)()
     out = (ctypes.c_uint8 * 16)()
