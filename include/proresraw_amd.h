@@ -187,6 +187,23 @@ int prr_decode_frame_device(prr_context *ctx, const prr_header *hdr, const prr_f
                             const uint8_t *d_packet, size_t len, int idct, uint16_t *d_samples,
                             uint16_t *d_linear, uint8_t *d_tile_flags,
                             prr_device_status *d_status, void *stream, prr_error *err);
+/* No counterpart in the crate: prr_decode_frame_device for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc). Same arguments, rules
+ * and results; it calls no hipMalloc, hipFree or allocating routine. Its
+ * scratch (tile offsets and tile order) is a static device array of
+ * 3 MiB per GPU, set up by prr_context_create and held until the process
+ * ends, so frames of more than PRR_NOALLOC_MAX_TILES tiles are refused
+ * with PRR_E_INVALID_ARGUMENT (a 8192x4320 frame has at most 138,240).
+ * Every call on one GPU shares that array: calls that overlap on one GPU
+ * (on different streams, or from several threads) race, and the caller
+ * serializes them. */
+#define PRR_NOALLOC_MAX_TILES 262144u
+int prr_decode_frame_device_noalloc(prr_context *ctx, const prr_header *hdr,
+                                    const prr_frame_info *info, const uint8_t *d_packet,
+                                    size_t len, int idct, uint16_t *d_samples,
+                                    uint16_t *d_linear, uint8_t *d_tile_flags,
+                                    prr_device_status *d_status, void *stream, prr_error *err);
 /* Frame::linearize_into on device buffers. */
 int prr_linearize_device(prr_context *ctx, const prr_header *hdr, const uint16_t *d_samples,
                          uint16_t *d_out, void *stream, prr_error *err);

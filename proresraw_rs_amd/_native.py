@@ -156,6 +156,8 @@ SIGNATURES = {
     "prr_develop_planar_into": (i32, [c_vp, H, c_vp, sz, DP, i32, c_vp, c_vp, c_vp, sz, E]),
     "prr_develop_into": (i32, [c_vp, H, c_vp, sz, DP, i32, c_vp, sz, E]),
     "prr_decode_frame_device": (i32, [c_vp, H, FI, c_vp, sz, i32, c_vp, c_vp, c_vp, c_vp, c_vp, E]),
+    "prr_decode_frame_device_noalloc": (i32, [c_vp, H, FI, c_vp, sz, i32, c_vp, c_vp, c_vp, c_vp,
+                                              c_vp, E]),
     "prr_linearize_device": (i32, [c_vp, H, c_vp, c_vp, c_vp, E]),
     "prr_decode_frame_reduced_device": (i32, [c_vp, H, FI, c_vp, sz, i32, i32, c_vp, c_vp, c_vp,
                                               c_vp, E]),

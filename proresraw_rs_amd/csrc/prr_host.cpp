@@ -267,6 +267,7 @@ struct prr_context {
     DevBuf packet, offsets, queue, status, flags, samples, out0, out1, out2;
     DevBuf lj_raw, lj_records, lj_values, lj_out, lj_meta; /* DNG tile streams */
     std::vector<uint8_t> host_flags;
+    void *noalloc = nullptr; /* this device's static scratch (prr_decode_frame_device_noalloc) */
 };
 
 static int ensure(prr_context *ctx, DevBuf &b, size_t bytes, prr_error *err)
@@ -303,6 +304,14 @@ extern "C" int prr_context_create(int device, prr_context **out, prr_error *err)
     if (e != hipSuccess) {
         delete ctx;
         return device_fail(err, e, "hipStreamCreate");
+    }
+    /* The allocation-free decode's static scratch: its address on this
+     * device, looked up once per device with the device current. */
+    e = noalloc_scratch(device, &ctx->noalloc);
+    if (e != hipSuccess) {
+        (void)hipStreamDestroy(ctx->stream);
+        delete ctx;
+        return device_fail(err, e, "hipGetSymbolAddress");
     }
     ctx->launch.lds_limit = (uint32_t)std::min<size_t>(prop.sharedMemPerBlock, 65536);
     if (const char *v = std::getenv("PRR_STREAMS_PER_WG"))
@@ -404,6 +413,36 @@ extern "C" int prr_decode_frame_device(prr_context *ctx, const prr_header *hdr,
     FrameDesc f = frame_desc(hdr, info, d_packet, len, (uint64_t *)ctx->offsets.ptr);
     HIP_TRY(launch_decode(f, idct, d_samples, d_linear, d_tile_flags, d_status,
                           (uint32_t *)ctx->queue.ptr, ctx->launch, s),
+            "decode kernels");
+    return PRR_OK;
+}
+
+extern "C" int prr_decode_frame_device_noalloc(prr_context *ctx, const prr_header *hdr,
+                                               const prr_frame_info *info,
+                                               const uint8_t *d_packet, size_t len, int idct,
+                                               uint16_t *d_samples, uint16_t *d_linear,
+                                               uint8_t *d_tile_flags,
+                                               prr_device_status *d_status, void *stream,
+                                               prr_error *err)
+{
+    int rc = check_device_args(ctx, hdr, info, d_packet, d_status, err);
+    if (rc)
+        return rc;
+    if (!d_samples || (idct != PRR_IDCT_APPLE && idct != PRR_IDCT_FFMPEG))
+        return fail(err, PRR_E_INVALID_ARGUMENT, "samples or idct");
+    if (info->nb_tiles > PRR_NOALLOC_MAX_TILES)
+        return fail(err, PRR_E_INVALID_ARGUMENT, "tiles above static scratch", info->nb_tiles,
+                    PRR_NOALLOC_MAX_TILES);
+    if (!ctx->noalloc)
+        return fail(err, PRR_E_DEVICE, "no static scratch");
+    HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+    /* The static scratch, by the address looked up for this device: the
+     * offsets, then the tile order. */
+    uint64_t *offsets = (uint64_t *)ctx->noalloc;
+    uint32_t *order = (uint32_t *)((uint8_t *)ctx->noalloc + NOALLOC_OFFSETS_BYTES);
+    FrameDesc f = frame_desc(hdr, info, d_packet, len, offsets);
+    HIP_TRY(launch_decode(f, idct, d_samples, d_linear, d_tile_flags, d_status, order,
+                          ctx->launch, (hipStream_t)stream),
             "decode kernels");
     return PRR_OK;
 }

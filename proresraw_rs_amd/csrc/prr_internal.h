@@ -149,6 +149,18 @@ hipError_t launch_batch(const BatchFrame *d_frames, uint32_t n_frames, uint32_t 
                         uint32_t chunk, size_t lds, hipStream_t s);
 size_t decode_lds_bytes(uint32_t spw, uint32_t per_block, uint32_t amax);
 
+/* Layout of the allocation-free decode's static device scratch: tile
+ * offsets (PRR_NOALLOC_MAX_TILES + 1 uint64, padded to 256 bytes), then the
+ * tile order (PRR_NOALLOC_MAX_TILES uint32). */
+enum : size_t {
+    NOALLOC_OFFSETS_BYTES = ((size_t)8 * (PRR_NOALLOC_MAX_TILES + 1) + 255) & ~(size_t)255,
+    NOALLOC_SCRATCH_BYTES = NOALLOC_OFFSETS_BYTES + (size_t)4 * PRR_NOALLOC_MAX_TILES,
+};
+enum { NOALLOC_MAX_DEVICES = 64 };
+/* The device address of `device`'s copy of the static scratch (looked up
+ * once per device; `device` must be current). */
+hipError_t noalloc_scratch(int device, void **out);
+
 /* Host side (prr_host.cpp). */
 /* Conversion::new (demosaic.rs:286-304): the stage folded into the params. */
 DevelopDesc develop_desc(const prr_header *hdr, const uint16_t *d_samples,

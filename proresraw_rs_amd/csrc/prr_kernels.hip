@@ -26,6 +26,8 @@
 #include "prr_internal.h"
 #include "prr_net.h"
 
+#include <mutex>
+
 namespace prr {
 
 /* ------------------------------------------------------------------ */
@@ -2212,6 +2214,42 @@ hipError_t launch_develop_sampled(const DevelopDesc &d, const uint32_t *tiles, u
 size_t decode_lds_bytes(uint32_t spw, uint32_t per_block, uint32_t amax)
 {
     return group_lds_bytes(spw, stream_stride(per_block, amax));
+}
+
+/* ------------------------------------------------------------------ */
+/* Device scratch of the allocation-free decode
+ * (prr_decode_frame_device_noalloc): one frame's tile offsets (nb_tiles + 1
+ * uint64, written whole by k_prepare) followed by its tile order (nb_tiles
+ * uint32, likewise), for frames of at most PRR_NOALLOC_MAX_TILES tiles:
+ * 3 MiB. Each GPU holds its own copy from the first context created on it
+ * until the process ends; every call on one GPU shares that copy, so calls
+ * that overlap on one GPU race. Nothing in it has to start at zero. */
+static __device__ __align__(256) uint8_t scratch[NOALLOC_SCRATCH_BYTES];
+
+namespace {
+std::mutex g_scratch_mutex;
+void *g_scratch_ptr[NOALLOC_MAX_DEVICES];
+} // namespace
+
+/* The device address of `device`'s copy of `scratch`, looked up once per
+ * device with that device current (the caller has made it current). On the
+ * host the array's name is only a stand-in for the symbol; kernels and
+ * copies take this address. */
+hipError_t noalloc_scratch(int device, void **out)
+{
+    *out = nullptr;
+    if (device < 0 || device >= NOALLOC_MAX_DEVICES)
+        return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[device]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess)
+            return e;
+        g_scratch_ptr[device] = ptr;
+    }
+    *out = g_scratch_ptr[device];
+    return hipSuccess;
 }
 
 } // namespace prr

@@ -204,6 +204,9 @@ def test_new_entry_points_reject_bad_arguments_before_device_work():
                                        16, idct, 0, 0.0, 0.0, 2, r, g, b, None,
                                        ctypes.byref(err)) == 101
         assert err.what == what
+    # The allocation-free decode checks its arguments before its context.
+    assert lib.prr_decode_frame_device_noalloc(None, None, None, None, 0, 0, None, None, None,
+                                               None, None, ctypes.byref(err)) == 101
     # DNG tiles: a picture narrower than two sensels, a stride below the width.
     buf = (ctypes.c_uint16 * 16)()
     out = (ctypes.c_uint8 * 16)()
