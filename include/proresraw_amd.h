@@ -127,8 +127,11 @@ void *prr_context_stream(prr_context *ctx);
 /* Version string of this library. */
 const char *prr_version(void);
 /* Decode launch shape (no counterpart in the crate, which has no launch to
- * shape): lanes decoding streams per workgroup (8, 16, 32 or 64; 8 by
- * default for single frames); 0 keeps the current value. */
+ * shape): lanes decoding streams per workgroup, 8, 16, 32 or 64, or 0 for
+ * the automatic width (the default): the narrowest that keeps the decode at
+ * 2.5 waves a SIMD or fewer, counting every frame decoding at once (a
+ * 4096x2160 frame decodes 8 wide, a 5760x3240 one 16 wide, two 5760x3240
+ * frames of a pipeline 32 wide). */
 int prr_set_decode_config(prr_context *ctx, uint32_t streams_per_wg, prr_error *err);
 
 /* ---- host-side frame structure (no device work) ----------------------- */
@@ -243,10 +246,13 @@ int prr_develop_linear_device(prr_context *ctx, const prr_header *hdr, const uin
  * device memory, decoded (with the linearize fused) on one HIP stream and
  * developed from the linear picture on a second, so frame i + 1's decode
  * (a latency-bound entropy chain) runs beside frame i's develop (VALU
- * bound). The pipeline owns `depth` sets of intermediate buffers (code
- * values, linear values, tile flags, status) used in turn; a decode waits
- * on the device for the develop that last used its set. Each frame's planes
- * equal prr_develop_frame_device's after prr_decode_frame_device. ---- */
+ * bound). Consecutive frames decode on two streams of their own, so two
+ * frames' decodes overlap (one frame has too few streams to fill the chip
+ * with wide decode waves). The pipeline owns `depth` sets of intermediate
+ * buffers (code values, linear values, tile flags, status) used in turn; a
+ * decode waits on the device for the develop that last used its set. Each
+ * frame's planes equal prr_develop_frame_device's after
+ * prr_decode_frame_device. ---- */
 typedef struct prr_pipeline prr_pipeline;
 /* depth: buffer sets, 2..8 (0: 3). */
 int prr_pipeline_create(prr_context *ctx, uint32_t depth, prr_pipeline **out, prr_error *err);

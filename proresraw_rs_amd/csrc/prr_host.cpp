@@ -264,6 +264,12 @@ struct prr_context {
     int device = 0;
     hipStream_t stream = nullptr;
     DecodeLaunch launch{8, 65536, 256};
+    /* The decode width follows the frame (frame_launch) unless the caller
+     * fixed it; `in_flight` frames' decodes share the chip (2 in a
+     * pipeline's decode contexts). */
+    bool spw_auto = true;
+    uint32_t in_flight = 1;
+    uint32_t simds = 1024;
     DevBuf packet, offsets, queue, status, flags, samples, out0, out1, out2;
     DevBuf lj_raw, lj_records, lj_values, lj_out, lj_meta; /* DNG tile streams */
     std::vector<uint8_t> host_flags;
@@ -314,8 +320,14 @@ extern "C" int prr_context_create(int device, prr_context **out, prr_error *err)
         return device_fail(err, e, "hipGetSymbolAddress");
     }
     ctx->launch.lds_limit = (uint32_t)std::min<size_t>(prop.sharedMemPerBlock, 65536);
-    if (const char *v = std::getenv("PRR_STREAMS_PER_WG"))
-        ctx->launch.streams_per_wg = (uint32_t)std::atoi(v);
+    ctx->simds = 4u * (uint32_t)prop.multiProcessorCount;
+    if (const char *v = std::getenv("PRR_STREAMS_PER_WG")) {
+        const uint32_t spw = (uint32_t)std::atoi(v);
+        if (spw == 8 || spw == 16 || spw == 32 || spw == 64) {
+            ctx->launch.streams_per_wg = spw;
+            ctx->spw_auto = false;
+        }
+    }
     *out = ctx;
     return PRR_OK;
 }
@@ -339,19 +351,43 @@ extern "C" void *prr_context_stream(prr_context *ctx) { return ctx ? ctx->stream
 
 int prr::context_device(const prr_context *ctx) { return ctx->device; }
 
-/* Decode launch parameter: streams per workgroup (16, 32 or 64); 0 keeps
- * the current value. */
+/* Decode launch parameter: streams per workgroup (8, 16, 32 or 64); 0
+ * returns to the automatic width. */
 extern "C" int prr_set_decode_config(prr_context *ctx, uint32_t streams_per_wg, prr_error *err)
 {
     if (!ctx)
         return fail(err, PRR_E_INVALID_ARGUMENT, "ctx");
-    if (streams_per_wg) {
-        if (streams_per_wg != 8 && streams_per_wg != 16 && streams_per_wg != 32 &&
-            streams_per_wg != 64)
-            return fail(err, PRR_E_INVALID_ARGUMENT, "streams_per_wg", streams_per_wg);
-        ctx->launch.streams_per_wg = streams_per_wg;
+    if (streams_per_wg == 0) {
+        ctx->spw_auto = true;
+        return PRR_OK;
     }
+    if (streams_per_wg != 8 && streams_per_wg != 16 && streams_per_wg != 32 &&
+        streams_per_wg != 64)
+        return fail(err, PRR_E_INVALID_ARGUMENT, "streams_per_wg", streams_per_wg);
+    ctx->launch.streams_per_wg = streams_per_wg;
+    ctx->spw_auto = false;
     return PRR_OK;
+}
+
+/* The decode launch of a frame of nb_tiles tiles. The automatic width is
+ * the narrowest of 8, 16, 32 and 64 streams per workgroup that keeps the
+ * decode's waves at 2.5 a SIMD or fewer, counting the streams of every
+ * frame decoding at once: a wave's loop runs as long as its longest
+ * stream, so wider groups issue fewer wave instructions for the same
+ * symbols (the streams of a frame are of similar length), while the waves
+ * left must still hide the loop's latency from one another. */
+static DecodeLaunch frame_launch(const prr_context *ctx, uint32_t nb_tiles)
+{
+    DecodeLaunch l = ctx->launch;
+    if (ctx->spw_auto) {
+        const uint64_t streams = 4ull * nb_tiles * ctx->in_flight;
+        const uint64_t max_waves = (uint64_t)ctx->simds * 5 / 2;
+        uint32_t spw = 8;
+        while (spw < 64 && streams / spw > max_waves)
+            spw <<= 1;
+        l.streams_per_wg = spw;
+    }
+    return l;
 }
 
 /* ------------------------------------------------------------------ */
@@ -412,7 +448,7 @@ extern "C" int prr_decode_frame_device(prr_context *ctx, const prr_header *hdr,
         return rc;
     FrameDesc f = frame_desc(hdr, info, d_packet, len, (uint64_t *)ctx->offsets.ptr);
     HIP_TRY(launch_decode(f, idct, d_samples, d_linear, d_tile_flags, d_status,
-                          (uint32_t *)ctx->queue.ptr, ctx->launch, s),
+                          (uint32_t *)ctx->queue.ptr, frame_launch(ctx, info->nb_tiles), s),
             "decode kernels");
     return PRR_OK;
 }
@@ -442,7 +478,7 @@ extern "C" int prr_decode_frame_device_noalloc(prr_context *ctx, const prr_heade
     uint32_t *order = (uint32_t *)((uint8_t *)ctx->noalloc + NOALLOC_OFFSETS_BYTES);
     FrameDesc f = frame_desc(hdr, info, d_packet, len, offsets);
     HIP_TRY(launch_decode(f, idct, d_samples, d_linear, d_tile_flags, d_status, order,
-                          ctx->launch, (hipStream_t)stream),
+                          frame_launch(ctx, info->nb_tiles), (hipStream_t)stream),
             "decode kernels");
     return PRR_OK;
 }
@@ -470,7 +506,8 @@ extern "C" int prr_decode_frame_reduced_device(prr_context *ctx, const prr_heade
         return rc;
     FrameDesc f = frame_desc(hdr, info, d_packet, len, (uint64_t *)ctx->offsets.ptr);
     HIP_TRY(launch_decode_reduced(f, scale, greens, d_rgb, d_tile_flags, d_status,
-                                  (uint32_t *)ctx->queue.ptr, ctx->launch, s),
+                                  (uint32_t *)ctx->queue.ptr, frame_launch(ctx, info->nb_tiles),
+                                  s),
             "reduced kernels");
     return PRR_OK;
 }
@@ -868,9 +905,15 @@ struct PipeSet {
 };
 } // namespace
 
+/* Decode contexts of a pipeline: consecutive frames decode on alternate
+ * ones, so two frames' decodes overlap (a frame alone has too few streams
+ * to fill the chip with wide decode waves). */
+enum { PIPE_DECODERS = 2 };
+
 struct prr_pipeline {
     prr_context *ctx = nullptr;   /* the caller's: device and launch shape */
-    prr_context *dctx = nullptr;  /* decode stream and the decode's scratch */
+    /* Decode streams and their scratch; frame k decodes on dctx[k % 2]. */
+    prr_context *dctx[PIPE_DECODERS] = {nullptr, nullptr};
     hipStream_t develop = nullptr;
     hipEvent_t caller = nullptr;  /* recorded on a caller's stream by prr_pipeline_wait */
     std::vector<PipeSet> sets;
@@ -882,8 +925,9 @@ struct prr_pipeline {
 
 static void pipeline_release(prr_pipeline *p)
 {
-    if (p->dctx)
-        (void)hipStreamSynchronize(p->dctx->stream);
+    for (prr_context *d : p->dctx)
+        if (d)
+            (void)hipStreamSynchronize(d->stream);
     if (p->develop)
         (void)hipStreamSynchronize(p->develop);
     for (PipeSet &st : p->sets) {
@@ -898,8 +942,9 @@ static void pipeline_release(prr_pipeline *p)
         (void)hipEventDestroy(p->caller);
     if (p->develop)
         (void)hipStreamDestroy(p->develop);
-    if (p->dctx)
-        prr_context_destroy(p->dctx);
+    for (prr_context *d : p->dctx)
+        if (d)
+            prr_context_destroy(d);
     delete p;
 }
 
@@ -933,21 +978,28 @@ extern "C" int prr_pipeline_create(prr_context *ctx, uint32_t depth, prr_pipelin
     prr_pipeline *p = new prr_pipeline();
     p->ctx = ctx;
     p->sets.resize(depth);
-    int rc = prr_context_create(ctx->device, &p->dctx, err);
-    if (rc) {
-        pipeline_release(p);
-        return rc;
+    hipError_t e = hipSuccess;
+    for (prr_context *&d : p->dctx) {
+        int rc = prr_context_create(ctx->device, &d, err);
+        if (rc) {
+            pipeline_release(p);
+            return rc;
+        }
+        d->launch = ctx->launch;
+        d->spw_auto = ctx->spw_auto;
+        d->in_flight = PIPE_DECODERS;
+        /* Decode and develop streams on hardware queues of their own, so
+         * they run side by side. */
+        hipStream_t decode_stream = nullptr;
+        if (e == hipSuccess)
+            e = dedicated_stream(ctx->device, &decode_stream);
+        if (e == hipSuccess) {
+            (void)hipStreamDestroy(d->stream);
+            d->stream = decode_stream;
+        }
     }
-    p->dctx->launch = ctx->launch;
-    /* The decode and develop streams on hardware queues of their own, so
-     * they run side by side. */
-    hipStream_t decode_stream = nullptr;
-    hipError_t e = dedicated_stream(ctx->device, &decode_stream);
-    if (e == hipSuccess) {
-        (void)hipStreamDestroy(p->dctx->stream);
-        p->dctx->stream = decode_stream;
+    if (e == hipSuccess)
         e = dedicated_stream(ctx->device, &p->develop);
-    }
     if (e == hipSuccess)
         e = hipEventCreateWithFlags(&p->caller, hipEventDisableTiming);
     for (PipeSet &st : p->sets) {
@@ -1013,7 +1065,8 @@ extern "C" int prr_pipeline_submit(prr_pipeline *p, const prr_header *hdr,
         return fail(err, PRR_E_INVALID_ARGUMENT, "samples or idct");
     HIP_TRY(hipSetDevice(p->ctx->device), "hipSetDevice");
     PipeSet &st = p->sets[p->submitted % p->sets.size()];
-    hipStream_t ds = p->dctx->stream, vs = p->develop;
+    prr_context *dctx = p->dctx[p->submitted % PIPE_DECODERS];
+    hipStream_t ds = dctx->stream, vs = p->develop;
     int rc;
     if (p->timing && st.timed && (rc = pipeline_collect(p, err)))
         return rc;
@@ -1024,12 +1077,12 @@ extern "C" int prr_pipeline_submit(prr_pipeline *p, const prr_header *hdr,
         /* Growing a set: its last frame must be done with it. */
         if (st.used)
             HIP_TRY(hipEventSynchronize(st.developed), "hipEventSynchronize");
-        if ((rc = ensure(p->dctx, st.samples, coded * 2, err)) ||
-            (rc = ensure(p->dctx, st.linear, visible * 2, err)) ||
-            (rc = ensure(p->dctx, st.flags, std::max<size_t>(info->nb_tiles, 1), err)))
+        if ((rc = ensure(dctx, st.samples, coded * 2, err)) ||
+            (rc = ensure(dctx, st.linear, visible * 2, err)) ||
+            (rc = ensure(dctx, st.flags, std::max<size_t>(info->nb_tiles, 1), err)))
             return rc;
     }
-    if ((rc = ensure(p->dctx, st.status, sizeof(prr_device_status), err)))
+    if ((rc = ensure(dctx, st.status, sizeof(prr_device_status), err)))
         return rc;
     /* The decode waits (on the device) for the develop that last read this
      * set; the develop waits for this decode. */
@@ -1038,7 +1091,7 @@ extern "C" int prr_pipeline_submit(prr_pipeline *p, const prr_header *hdr,
     const bool timed = p->timing;
     if (timed)
         HIP_TRY(hipEventRecord(st.t0, ds), "hipEventRecord");
-    if ((rc = prr_decode_frame_device(p->dctx, hdr, info, d_packet, len, idct,
+    if ((rc = prr_decode_frame_device(dctx, hdr, info, d_packet, len, idct,
                                       (uint16_t *)st.samples.ptr, (uint16_t *)st.linear.ptr,
                                       (uint8_t *)st.flags.ptr, (prr_device_status *)st.status.ptr,
                                       ds, err)))
@@ -1049,7 +1102,7 @@ extern "C" int prr_pipeline_submit(prr_pipeline *p, const prr_header *hdr,
     HIP_TRY(hipStreamWaitEvent(vs, st.decoded, 0), "hipStreamWaitEvent");
     if (timed)
         HIP_TRY(hipEventRecord(st.t2, vs), "hipEventRecord");
-    if ((rc = develop_linear(p->dctx, hdr, (const uint16_t *)st.linear.ptr,
+    if ((rc = develop_linear(dctx, hdr, (const uint16_t *)st.linear.ptr,
                              (const prr_device_status *)st.status.ptr, explicit_wb, wb_red,
                              wb_blue, stage, d_r, d_g, d_b, d_status_out, vs, err)))
         return rc;
@@ -1071,7 +1124,8 @@ extern "C" int prr_pipeline_wait(prr_pipeline *p, void *stream, prr_error *err)
      * wait for it on the device (the wait takes the event as recorded now,
      * so the one event serves every call). */
     HIP_TRY(hipEventRecord(p->caller, (hipStream_t)stream), "hipEventRecord");
-    HIP_TRY(hipStreamWaitEvent(p->dctx->stream, p->caller, 0), "hipStreamWaitEvent");
+    for (prr_context *d : p->dctx)
+        HIP_TRY(hipStreamWaitEvent(d->stream, p->caller, 0), "hipStreamWaitEvent");
     HIP_TRY(hipStreamWaitEvent(p->develop, p->caller, 0), "hipStreamWaitEvent");
     return PRR_OK;
 }
@@ -1093,7 +1147,8 @@ extern "C" int prr_pipeline_sync(prr_pipeline *p, prr_error *err)
         return fail(err, PRR_E_INVALID_ARGUMENT, "pipeline");
     HIP_TRY(hipSetDevice(p->ctx->device), "hipSetDevice");
     HIP_TRY(hipStreamSynchronize(p->develop), "hipStreamSynchronize");
-    HIP_TRY(hipStreamSynchronize(p->dctx->stream), "hipStreamSynchronize");
+    for (prr_context *d : p->dctx)
+        HIP_TRY(hipStreamSynchronize(d->stream), "hipStreamSynchronize");
     return PRR_OK;
 }
 

@@ -277,4 +277,4 @@ def test_every_decode_launch_shape_is_bit_exact(gpu, spw):
             assert r.skipped_tiles == sk
             np.testing.assert_array_equal(r.samples, rgb)
     finally:
-        assert lib.prr_set_decode_config(ctx.handle, 8, ctypes.byref(err)) == 0
+        assert lib.prr_set_decode_config(ctx.handle, 0, ctypes.byref(err)) == 0
