@@ -77,7 +77,7 @@ typedef struct {
     float gain;
 } prr_header;
 
-/* Sizes derived from a header (header.rs:249-256, tile.rs:547-561,
+/* Sizes derived from a header (header.rs:249-256, tile.rs:47-72,
  * reduced.rs:99-111). */
 typedef struct {
     uint32_t coded_width;

@@ -85,6 +85,11 @@ void orc_codeword(uint32_t p, uint32_t cb, uint32_t *value, uint32_t *len);
  * returns the count read (bits.rs tests drive BitReader this way). */
 size_t orc_read_symbols(const uint8_t *data, size_t len, unsigned align,
                         const uint16_t *cbs, size_t n, uint32_t *out);
+int orc_count_symbols(const uint8_t *data, size_t len, unsigned align, unsigned log2_blocks,
+                      unsigned log2_coefs, uint64_t cap, uint64_t *ac_symbols,
+                      uint64_t *consumed_bits);
+size_t orc_frame_stream_census(const uint8_t *packet, size_t len, uint32_t *bytes,
+                               uint32_t *symbols, size_t cap);
 void orc_decode_component_reduced(const uint8_t *data, size_t len, unsigned align,
                                   unsigned log2_blocks, unsigned log2_coefs,
                                   int16_t *coefs);

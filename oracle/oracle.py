@@ -51,6 +51,9 @@ _SIG = {
     "orc_codeword": (None, [u32, u32, ctypes.POINTER(u32), ctypes.POINTER(u32)]),
     "orc_read_symbols": (sz, [vp, sz, ctypes.c_uint, vp, sz, vp]),
     "orc_decode_component_reduced": (None, [vp, sz, ctypes.c_uint, ctypes.c_uint, ctypes.c_uint, vp]),
+    "orc_count_symbols": (i32, [vp, sz, ctypes.c_uint, ctypes.c_uint, ctypes.c_uint, u64,
+                                ctypes.POINTER(u64), ctypes.POINTER(u64)]),
+    "orc_frame_stream_census": (sz, [vp, sz, vp, vp, sz]),
     "orc_apple_block": (None, [vp, vp, vp]),
     "orc_ffmpeg_block": (None, [vp, vp, vp]),
     "orc_apple_block_4x4": (None, [vp, vp, vp]),
@@ -152,6 +155,28 @@ def decode_frame(packet, ffmpeg=False, threads=None):
     if rc:
         raise OracleError(e)
     return h, samples, [int(x) for x in skipped[:nsk.value]], mq.value
+
+
+def count_symbols(data, align, log2_blocks, log2_coefs=6, cap=1 << 26):
+    """(AC symbols, bits consumed, gave up) of decode_component_reduced on
+    one stream (an instrumented copy of its loop)."""
+    a, p = _arr(data)
+    n, bits = ctypes.c_uint64(), ctypes.c_uint64()
+    capped = lib().orc_count_symbols(p, a.size, align, log2_blocks, log2_coefs, cap,
+                                     ctypes.byref(n), ctypes.byref(bits))
+    return n.value, bits.value, bool(capped)
+
+
+def stream_census(packet):
+    """Per stream of a frame (4 per tile, decode order): (bytes, symbols)."""
+    a, p = _arr(packet)
+    h = parse_frame(packet)[0]
+    n = lib().orc_tile_count(h.width, h.height, h.alignment)
+    nbytes = np.zeros(4 * n, np.uint32)
+    syms = np.zeros(4 * n, np.uint32)
+    got = lib().orc_frame_stream_census(p, a.size, nbytes.ctypes.data, syms.ctypes.data, n)
+    assert got == n
+    return nbytes, syms
 
 
 def linearize(h, samples, threads=None):

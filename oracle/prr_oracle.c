@@ -83,7 +83,7 @@ static const uint16_t RN_CB[28] = {0x200, 0x100, 0x000, 0x000, 0x211, 0x211, 0x1
 static const uint16_t LN_CB[15] = {0x100, 0x111, 0x222, 0x222, 0x122, 0x122, 0x433, 0x433,
                                    0x233, 0x233, 0x233, 0x233, 0x233, 0x233, 0x033};
 
-/* AC_CB as runs of (count, value), codebook.rs:327-355. */
+/* AC_CB as runs of (count, value), codebook.rs:57-85. */
 static uint16_t ac_cb_entry(uint32_t i)
 {
     static const struct { uint8_t count; uint16_t value; } runs[10] = {
@@ -321,6 +321,100 @@ void orc_decode_component_reduced(const uint8_t *data, size_t len, unsigned alig
             return;
     }
 #undef PLACE
+}
+
+/* Instrumented copy of orc_decode_component_reduced (coefficients.rs:37-146):
+ * the same loop, storing nothing, counting the symbols of its AC part (one
+ * per codeword read after the DC values, the unit the device decoder's
+ * symbol budget counts) and the bits its consumes ask for. It gives up after
+ * `cap` AC symbols (returns 1), so a stream the loop would not leave cannot
+ * hang a test. */
+int orc_count_symbols(const uint8_t *data, size_t len, unsigned align, unsigned log2_blocks,
+                      unsigned log2_coefs, uint64_t cap, uint64_t *ac_symbols,
+                      uint64_t *consumed_bits)
+{
+    uint32_t per_block = 1u << log2_coefs;
+    uint32_t blocks = 1u << log2_blocks;
+    uint32_t codes = per_block * blocks;
+    uint64_t count = 0, bits = 0;
+    orc_reader r;
+    reader_init(&r, data, len, align);
+    *ac_symbols = 0;
+    *consumed_bits = 0;
+#define CONSUME(l) (bits += (l), reader_consume(&r, (l)))
+    if (reader_top(&r) == 0)
+        return 0;
+    uint32_t first, l;
+    orc_codeword(reader_top(&r), FIRST_DC, &first, &l);
+    CONSUME(l);
+    uint16_t dc_cb = DC_RESET;
+    uint32_t sign = 0;
+    for (uint32_t block = 1; block < blocks; block++) {
+        if (reader_top(&r) == 0)
+            goto out;
+        uint32_t symbol;
+        orc_codeword(reader_top(&r), dc_cb, &symbol, &l);
+        sign ^= symbol & 1;
+        uint32_t magnitude = ((symbol & 1) + symbol) >> 1;
+        int32_t delta = (int32_t)(((0u - sign) ^ magnitude) + sign);
+        dc_cb = next_cb(DC_CB, 13, magnitude);
+        sign = delta < 0;
+        CONSUME(l);
+    }
+    uint16_t ac_cb = AC_INITIAL, rn_cb = RN_INITIAL, ln_cb = LN_INITIAL;
+    uint32_t n = blocks;
+    if (n >= codes)
+        goto out;
+    while (reader_top(&r) != 0) {
+        uint32_t run;
+        if (++count > cap)
+            goto capped;
+        orc_codeword(reader_top(&r), ln_cb, &run, &l);
+        CONSUME(l);
+        for (uint32_t k = 0; k < run; k++) {
+            if (reader_top(&r) == 0)
+                goto out;
+            if (++count > cap)
+                goto capped;
+            uint32_t symbol;
+            orc_codeword(reader_top(&r), ac_cb, &symbol, &l);
+            CONSUME(l + 1);
+            ac_cb = next_ac(symbol);
+            n += 1;
+            if (n == codes)
+                goto out;
+        }
+        if (reader_top(&r) == 0)
+            goto out;
+        uint32_t zeros;
+        if (++count > cap)
+            goto capped;
+        orc_codeword(reader_top(&r), rn_cb, &zeros, &l);
+        rn_cb = next_cb(RN_CB, 28, zeros);
+        CONSUME(l);
+        n = n + 1 + zeros;
+        if (n >= codes || reader_top(&r) == 0)
+            goto out;
+        uint32_t symbol;
+        if (++count > cap)
+            goto capped;
+        orc_codeword(reader_top(&r), ac_cb, &symbol, &l);
+        CONSUME(l + 1);
+        ac_cb = next_ac(symbol);
+        ln_cb = next_cb(LN_CB, 15, symbol);
+        n += 1;
+        if (n >= codes)
+            goto out;
+    }
+out:
+    *ac_symbols = count;
+    *consumed_bits = bits;
+    return 0;
+capped:
+    *ac_symbols = cap;
+    *consumed_bits = bits;
+    return 1;
+#undef CONSUME
 }
 
 /* ------------------------------------------------------------------ */
@@ -899,6 +993,51 @@ int orc_parse_frame(const uint8_t *packet, size_t len, orc_header *hdr,
     }
     *max_qscale = mq;
     return ORC_OK;
+}
+
+/* Stream census of a frame: for tile t and component s (decode order of
+ * decoder.rs:286-306), entry 4t+s receives the stream's bytes and its
+ * symbol count (DC values + AC symbols, counted by orc_count_symbols);
+ * invalid tiles get 0 bytes and 0 symbols. Returns the tile count, or 0 on
+ * a frame error. */
+size_t orc_frame_stream_census(const uint8_t *packet, size_t len, uint32_t *bytes,
+                               uint32_t *symbols, size_t cap)
+{
+    orc_header h;
+    orc_error e;
+    size_t n;
+    uint8_t mq;
+    if (orc_parse_frame(packet, len, &h, NULL, NULL, 0, &n, &mq, &e) || n > cap)
+        return 0;
+    uint64_t *off = (uint64_t *)malloc(sizeof(uint64_t) * (n + 1));
+    uint32_t *sz = (uint32_t *)malloc(sizeof(uint32_t) * (n + 1));
+    uint32_t *tx = (uint32_t *)malloc(sizeof(uint32_t) * (n + 1));
+    uint32_t *ty = (uint32_t *)malloc(sizeof(uint32_t) * (n + 1));
+    uint8_t *tl = (uint8_t *)malloc(n + 1);
+    orc_parse_frame(packet, len, &h, off, sz, n, &n, &mq, &e);
+    orc_tile_geometry(h.width, h.height, h.alignment, tx, ty, tl);
+    for (size_t t = 0; t < n; t++) {
+        uint8_t qs;
+        uint32_t so[4], sl[4];
+        if (orc_tile_parse(packet + off[t], sz[t], &qs, so, sl)) {
+            for (int k = 0; k < 4; k++)
+                bytes[4 * t + k] = symbols[4 * t + k] = 0;
+            continue;
+        }
+        for (int k = 0; k < 4; k++) {
+            uint64_t start = off[t] + so[k], ac, bits;
+            orc_count_symbols(packet + start, sl[k], (unsigned)(start & 3), tl[t], 6,
+                              (uint64_t)1 << 40, &ac, &bits);
+            bytes[4 * t + k] = sl[k];
+            symbols[4 * t + k] = (uint32_t)(ac + ((size_t)1 << tl[t]));
+        }
+    }
+    free(off);
+    free(sz);
+    free(tx);
+    free(ty);
+    free(tl);
+    return n;
 }
 
 /* ------------------------------------------------------------------ */

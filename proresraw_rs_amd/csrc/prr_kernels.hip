@@ -128,7 +128,7 @@ __device__ inline uint32_t linearize(const uint32_t *c, uint32_t code)
     return v < 65535u ? v : 65535u;
 }
 
-/* Tile geometry from its frame-order index (TileGrid::row, tile.rs:569-591). */
+/* Tile geometry from its frame-order index (TileGrid::row, tile.rs:78-102). */
 __device__ inline void tile_geometry(const FrameDesc &f, uint32_t t, uint32_t &x, uint32_t &y,
                                      uint32_t &log2)
 {
@@ -683,7 +683,11 @@ __device__ inline bool front_end(const FrameDesc &f, Group &g, const TileSet &ts
         uint64_t stop = s < 3 ? start + sz[s] : tend;
         uint32_t align = (uint32_t)(start & 3);
         uint32_t codes = 1u << (tlog2 + log2_coefs);
-        uint32_t budget = 16u * (8u * (uint32_t)(stop - start) + codes) + 1024u;
+        /* The symbol budget: above the most AC symbols the crate's loop can
+         * read from a stream of this many bytes and coefficients (8.25
+         * bytes + codes + 1250, from the reader's refill and shift rules),
+         * so it ends only streams the crate would not leave. */
+        uint32_t budget = 16u * (8u * (uint32_t)(stop - start) + codes) + 2048u;
         PRR_LDS int16_t *mine = (PRR_LDS int16_t *)(g.coefs + lane * stride);
         const PRR_LDS uint32_t *tbl = (const PRR_LDS uint32_t *)g.tbl;
         const uint32_t dummy = (1u << log2_coefs) << f.alignment; /* first padding slot */

@@ -1,5 +1,5 @@
 """The oracle against the crate's known-answer tests: bits.rs:202-269,
-coefficients.rs:153-229, curve.rs:270-300, tile.rs:654-740,
+coefficients.rs:153-229, curve.rs:36-70, tile.rs:156-250,
 header.rs:333-489, decoder.rs:313-427, synth.rs:228-274 and the wrapped
 zero run of gpu.rs:1410-1443."""
 import ctypes
@@ -166,7 +166,7 @@ def test_valid_streams_round_trip():
 
 def test_reduced_decodes_keep_the_leading_scan_positions():
     # Quarter and eighth decodes read the same stream and stop after scan
-    # position 16 / 4 of every block (coefficients.rs:29-33, codebook.rs:374-380).
+    # position 16 / 4 of every block (coefficients.rs:29-33, codebook.rs:104-110).
     rng = np.random.default_rng(8)
     for trial in range(60):
         log2 = trial % 5
@@ -210,7 +210,7 @@ def test_rounds_to_nearest():
 
 def test_curve_matches_a_rational_interpolation():
     # Round-to-nearest of the exact interpolation, with the 16-bit wrap of
-    # a falling segment (curve.rs:249-253).
+    # a falling segment (curve.rs:18-22).
     c = [0, 224, 288, 468, 1162, 3339, 9431, 25377]
     for code in range(0, 65536, 97):
         seg, frac = code >> 13, code & 0x1FFF
