@@ -142,8 +142,30 @@ int prr_parse_header(const uint8_t *packet, size_t len, prr_header *out, prr_err
 /* Prefix checks of parse_frame (decoder.rs:76-103) plus the grid sizes. */
 int prr_parse_frame_info(const uint8_t *packet, size_t len, prr_header *hdr, prr_frame_info *info,
                          prr_error *err);
+/* ParsedFrame (decoder.rs:15-29): the header, the tile grid's sizes and
+ * the largest tile qscale byte; the tile payloads are the
+ * tile_offsets / tile_sizes prr_parse_frame writes. */
+typedef struct {
+    prr_header header;
+    prr_frame_info info;
+    uint8_t max_qscale;
+} prr_parsed_frame;
+/* parse_frame (decoder.rs:76-123) on the host, for callers that only parse:
+ * the prefix and header checks of prr_parse_frame_info, then the size-table
+ * walk: tile i's payload is bytes [tile_offsets[i], tile_offsets[i] +
+ * tile_sizes[i]) of the packet, max_qscale is the largest byte after each
+ * tile's start (read for invalid and short tiles too, 0 past the packet),
+ * and the first tile running past the packet is PRR_E_TILE_OUT_OF_BOUNDS
+ * {tile = a}. tile_offsets and tile_sizes may both be NULL; otherwise they
+ * hold `cap` entries, and cap < nb_tiles returns PRR_E_OUTPUT_SIZE (a =
+ * nb_tiles) with out->header and out->info filled, so a caller can size
+ * them. */
+int prr_parse_frame(const uint8_t *packet, size_t len, prr_parsed_frame *out,
+                    uint64_t *tile_offsets, uint32_t *tile_sizes, size_t cap, prr_error *err);
 /* quantizer / Frame::quantizer (decoder.rs:33-35, 150-152). */
 uint32_t prr_quantizer(const prr_header *hdr, uint8_t max_qscale);
+/* frame_quantizer (demosaic.rs:50-52): the quantizer of a parsed frame. */
+uint32_t prr_frame_quantizer(const prr_parsed_frame *parsed);
 /* DemosaicParams::new / with_white_balance (demosaic.rs:80-106). */
 void prr_demosaic_params_new(const prr_header *hdr, uint32_t quantizer,
                              prr_demosaic_params *out);

@@ -15,6 +15,7 @@ from .api import (  # noqa: F401
     Frame,
     FrameHeader,
     Greens,
+    ParsedFrame,
     Idct,
     RgbFrame,
     RgbImage,
@@ -31,9 +32,11 @@ from .api import (  # noqa: F401
     develop_planar,
     develop_planar_into,
     develop_with,
+    frame_quantizer,
     header_len,
     is_prores_raw_fourcc,
     output_matrix,
+    parse_frame,
     parse_frame_info,
     parse_header,
 )

@@ -59,6 +59,14 @@ class PrrFrameInfo(ctypes.Structure):
     ]
 
 
+class PrrParsedFrame(ctypes.Structure):
+    _fields_ = [
+        ("header", PrrHeader),
+        ("info", PrrFrameInfo),
+        ("max_qscale", ctypes.c_uint8),
+    ]
+
+
 class PrrDemosaicParams(ctypes.Structure):
     _fields_ = [
         ("cfa_phase", ctypes.c_int32),
@@ -144,7 +152,9 @@ SIGNATURES = {
     "prr_header_len": (i32, [c_vp, sz, ctypes.POINTER(sz), E]),
     "prr_parse_header": (i32, [c_vp, sz, H, E]),
     "prr_parse_frame_info": (i32, [c_vp, sz, H, FI, E]),
+    "prr_parse_frame": (i32, [c_vp, sz, ctypes.POINTER(PrrParsedFrame), c_vp, c_vp, sz, E]),
     "prr_quantizer": (u32, [H, ctypes.c_uint8]),
+    "prr_frame_quantizer": (u32, [ctypes.POINTER(PrrParsedFrame)]),
     "prr_demosaic_params_new": (None, [H, u32, DP]),
     "prr_demosaic_params_with_white_balance": (None, [H, u32, ctypes.c_float, ctypes.c_float, DP]),
     "prr_output_matrix": (None, [H, i32, c_vp]),

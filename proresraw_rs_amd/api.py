@@ -228,6 +228,47 @@ def parse_frame_info(packet):
     return FrameHeader._from_raw(h), info
 
 
+@dataclass
+class ParsedFrame:
+    """ParsedFrame (decoder.rs:15-29): the header, the tile grid's sizes,
+    each tile's payload as (offset, size) in the packet, and the largest
+    tile qscale byte."""
+    header: FrameHeader
+    info: N.PrrFrameInfo = field(repr=False)
+    tile_offsets: np.ndarray
+    tile_sizes: np.ndarray
+    max_qscale: int
+    _raw: N.PrrParsedFrame = field(repr=False, compare=False, default=None)
+
+    def tiles(self, packet) -> List[bytes]:
+        """The tile payloads (ParsedFrame::tiles) as slices of `packet`."""
+        b = bytes(packet)
+        return [b[int(o):int(o) + int(n)] for o, n in zip(self.tile_offsets, self.tile_sizes)]
+
+
+def parse_frame(packet) -> ParsedFrame:
+    """parse_frame (decoder.rs:76-123): the size table walked on the host
+    (no device work)."""
+    p, n, keep = _buf(packet)
+    out = N.PrrParsedFrame()
+    err = N.PrrError()
+    lib = N.lib()
+    _check(lib.prr_parse_frame(p, n, ctypes.byref(out), None, None, 0, ctypes.byref(err)), err)
+    count = out.info.nb_tiles
+    offsets = np.zeros(max(count, 1), dtype=np.uint64)
+    sizes = np.zeros(max(count, 1), dtype=np.uint32)
+    _check(lib.prr_parse_frame(p, n, ctypes.byref(out), ctypes.c_void_p(offsets.ctypes.data),
+                               ctypes.c_void_p(sizes.ctypes.data), offsets.size,
+                               ctypes.byref(err)), err)
+    return ParsedFrame(FrameHeader._from_raw(out.header), out.info, offsets[:count],
+                       sizes[:count], int(out.max_qscale), out)
+
+
+def frame_quantizer(parsed: ParsedFrame) -> int:
+    """frame_quantizer (demosaic.rs:50-52)."""
+    return int(N.lib().prr_frame_quantizer(ctypes.byref(parsed._raw)))
+
+
 class Frame:
     """Frame (decoder.rs:131-217): code values over the coded size,
     skipped tile indices and the largest tile qscale."""

@@ -202,6 +202,46 @@ extern "C" uint32_t prr_quantizer(const prr_header *hdr, uint8_t max_qscale)
     return (uint32_t)hdr->quant_matrix[0] * (uint32_t)max_qscale;
 }
 
+/* parse_frame (decoder.rs:76-123): the size table walked on the host. */
+extern "C" int prr_parse_frame(const uint8_t *packet, size_t len, prr_parsed_frame *out,
+                               uint64_t *tile_offsets, uint32_t *tile_sizes, size_t cap,
+                               prr_error *err)
+{
+    if (!out || ((tile_offsets == nullptr) != (tile_sizes == nullptr)))
+        return fail(err, PRR_E_INVALID_ARGUMENT, "null argument");
+    std::memset(out, 0, sizeof *out);
+    int rc = prr_parse_frame_info(packet, len, &out->header, &out->info, err);
+    if (rc)
+        return rc;
+    const size_t n = out->info.nb_tiles;
+    if (tile_offsets && cap < n)
+        return fail(err, PRR_E_OUTPUT_SIZE, "tile capacity", n, cap);
+    const uint8_t *table = packet + out->info.header_end;
+    uint64_t offset = out->info.header_end + 2 * (uint64_t)n;
+    uint8_t mq = 0;
+    for (size_t i = 0; i < n; i++) {
+        const uint64_t size = rd16(table + 2 * i);
+        /* The byte after the tile start, even for a tile too short to hold
+         * it (0 past the packet). */
+        const uint8_t q = offset + 1 < len ? packet[offset + 1] : 0;
+        mq = q > mq ? q : mq;
+        if (offset + size > len)
+            return fail(err, PRR_E_TILE_OUT_OF_BOUNDS, nullptr, i);
+        if (tile_offsets) {
+            tile_offsets[i] = offset;
+            tile_sizes[i] = (uint32_t)size;
+        }
+        offset += size;
+    }
+    out->max_qscale = mq;
+    return PRR_OK;
+}
+
+extern "C" uint32_t prr_frame_quantizer(const prr_parsed_frame *parsed)
+{
+    return prr_quantizer(&parsed->header, parsed->max_qscale);
+}
+
 /* DemosaicParams::with_white_balance (demosaic.rs:86-106). */
 extern "C" void prr_demosaic_params_with_white_balance(const prr_header *hdr, uint32_t quantizer,
                                                        float red, float blue,

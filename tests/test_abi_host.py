@@ -128,6 +128,69 @@ def test_frame_level_errors_match_the_oracle():
             assert e.points == o.points
 
 
+def test_parse_frame_matches_the_oracle():
+    """prr_parse_frame (parse_frame, decoder.rs:76-123, on the host): tile
+    offsets and sizes, max_qscale (counting invalid and short tiles) and
+    frame_quantizer (demosaic.rs:50-52) equal the oracle's on random frames,
+    frames with corrupt tiles, and frames whose size table runs tiles past
+    the packet (TileOutOfBounds naming the same tile); every frame-level
+    error as parse_frame_info's."""
+    rng = S.Rng(33)
+    pr = np.random.default_rng(33)
+    checked = oob = 0
+    for i, spec in enumerate(S.specs(rng)):
+        spec.valid_tiles = i % 2
+        pk = S.random_frame(spec, rng)
+        variants = [pk]
+        h, info = P.parse_frame_info(pk)
+        if info.nb_tiles:
+            # A tile entry grown past the packet, and a packet cut short.
+            k = int(pr.integers(0, info.nb_tiles))
+            bad = bytearray(pk)
+            bad[info.header_end + 2 * k:info.header_end + 2 * k + 2] = b"\xff\xff"
+            variants.append(bytes(bad))
+            variants.append(pk[:max(info.header_end + 2 * info.nb_tiles,
+                                    len(pk) - int(pr.integers(1, 200)))])
+        for v in variants:
+            try:
+                oh, offs, sizes, mq = O.parse_frame(v)
+                want = None
+            except O.OracleError as e:
+                want = e
+            try:
+                got = P.parse_frame(v)
+                err = None
+            except P.Error as e:
+                err = e
+            if want is not None:
+                assert err is not None and (err.code, err.a, err.b) == (want.code, want.a, want.b)
+                oob += want.code == 11
+                continue
+            assert err is None, err
+            assert np.array_equal(got.tile_offsets, offs) and np.array_equal(got.tile_sizes, sizes)
+            assert got.max_qscale == mq
+            assert P.frame_quantizer(got) == oh.quant_matrix[0] * mq
+            tiles = got.tiles(v)
+            assert all(len(t) == n for t, n in zip(tiles, got.tile_sizes))
+            checked += 1
+    print(f"{checked} frames parsed, {oob} TileOutOfBounds")
+    assert checked >= 15 and oob >= 5
+    for pk in bad_packets():
+        o = oracle_err(pk)
+        e = err_of(P.parse_frame, pk)
+        assert (e.code, e.a, e.b) == (o.code, o.a, o.b)
+    # A short tile table capacity is refused with the count to allocate.
+    pk = S.encode_scene(S.scene(64, 32, 0, seed=2, version=0, alignment=1, threads=1))
+    out = N.PrrParsedFrame()
+    offs = np.zeros(2, np.uint64)
+    sizes = np.zeros(2, np.uint32)
+    err = N.PrrError()
+    buf = np.frombuffer(pk, np.uint8)
+    rc = N.lib().prr_parse_frame(buf.ctypes.data, buf.size, ctypes.byref(out), offs.ctypes.data,
+                                 sizes.ctypes.data, 2, ctypes.byref(err))
+    assert rc == 100 and err.a == out.info.nb_tiles > 2
+
+
 def test_error_payloads_carry_the_crate_fields():
     e = frame_info_err(bad_packets()[0])
     assert e.kind == "Truncated" and (e.needed, e.available) == (10, 4) and e.what == "frame prefix"
