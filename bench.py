@@ -46,6 +46,10 @@ def parse_args():
     ap.add_argument("--batch-frames", type=int, default=64)
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--learned-frames", type=int, default=32)
+    ap.add_argument("--learned-distinct", type=int, default=8,
+                    help="distinct packets (half of each frame kind) cycled in the learned leg")
+    ap.add_argument("--net-weights", default=None,
+                    help="weights of the learned demosaic (default: the shipped ones)")
     ap.add_argument("--no-clip", action="store_true")
     ap.add_argument("--no-dng", action="store_true")
     ap.add_argument("--no-orient", action="store_true")
@@ -543,7 +547,8 @@ def learned_bench(args, plain_value):
     balance + Rec.2020 matrix with the on-device fallback to plain develop
     (noise outside the trained range, or any of 8 sampled 64x64 tiles below
     45 dB against plain develop). The same batch with plain develop is timed
-    beside it. Two distinct packets of each size are encoded and cycled."""
+    beside it. --learned-distinct packets (half of each kind) are encoded and
+    cycled."""
     import torch
 
     import proresraw_rs_amd as P
@@ -556,7 +561,8 @@ def learned_bench(args, plain_value):
     kinds = [dict(size=(8192, 4320), version=1, alignment=4, q=15, vendor=b"aprh"),
              dict(size=(5760, 3240), version=0, alignment=3, q=45, vendor=b"aprn")]
     distinct = []
-    for i in range(4):
+    nd = max(2, args.learned_distinct & ~1)
+    for i in range(nd):
         k = kinds[i % 2]
         sc = S.scene(*k["size"], 0, seed=3000 + i, version=k["version"],
                      alignment=k["alignment"], qscale=(k["q"], k["q"]),
@@ -568,7 +574,7 @@ def learned_bench(args, plain_value):
     coded_bits = 0
     tiles = 0
     for j in range(args.learned_frames):
-        pk = distinct[j % 4]
+        pk = distinct[j % nd]
         slot = b.add_full(pk, P.Idct.Apple, linear=True)
         h = slot.header
         sensels += h.width * h.height
@@ -577,7 +583,7 @@ def learned_bench(args, plain_value):
         planes = [torch.empty(h.width * h.height, dtype=torch.float32, device="cuda")
                   for _ in range(3)]
         frames.append((slot, P.DemosaicParams.new(h, 0), planes))
-    params, lo, hi = L.load_weights()
+    params, lo, hi = L.load_weights(args.net_weights) if args.net_weights else L.load_weights()
     net = L.Net(params)
     net.set_range(lo, hi)
     lib, ctx = N.lib(), P.context()
@@ -648,6 +654,7 @@ def learned_bench(args, plain_value):
             "coded_bits_per_sensel": round(coded_bits / sensels, 3),
             "decode_bitrate": decode_bound("k_batch_decode", "learned32", coded_bits, tiles,
                                            batch_ms),
+            "distinct_packets": nd,
             "workload": "32 frames alternating 8192x4320 v1 a4 q15 and 5760x3240 v0 a3 q45, "
                         "batched decode+linearize, learned demosaic+WB+Rec2020 with fallback"}
 

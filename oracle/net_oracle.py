@@ -221,17 +221,26 @@ def unshuffle(dpred, channels=C):
     return d
 
 
-def loss_and_dout(out, base, target, margin, emulate=True):
+def loss_and_dout(out, base, target, margin, emulate=True, borders=None):
     """Mean squared error over the cells [margin, H2 - margin) x [margin,
     W2 - margin) (all 12 sub-pixel colours) and its gradient with respect to
-    out; the per-element derivative 2 * diff / count is rounded to bf16."""
+    out; the per-element derivative 2 * diff / count is rounded to bf16.
+    borders[k] = (top, bottom, left, right): the sides of crop k that lie on
+    the picture's border, where the margin is 0; count stays the interior
+    one."""
     N, H2, W2, _ = out.shape
     ft = np.float32 if emulate else np.float64
     pred = shuffle(np.asarray(out, ft)[..., :OUT_CHANNELS]) + np.asarray(base, ft)
     diff = pred - np.asarray(target, ft)
-    mask = np.zeros((2 * H2, 2 * W2), bool)
-    mask[2 * margin:2 * (H2 - margin), 2 * margin:2 * (W2 - margin)] = True
-    count = N * 3 * int(mask.sum())
+    interior = np.zeros((2 * H2, 2 * W2), bool)
+    interior[2 * margin:2 * (H2 - margin), 2 * margin:2 * (W2 - margin)] = True
+    count = N * 3 * int(interior.sum())
+    mask = np.zeros((N, 1, 2 * H2, 2 * W2), bool)
+    for k in range(N):
+        t, b, l, r = borders[k] if borders is not None else (False,) * 4
+        m0, m1 = (0 if t else margin), (0 if b else margin)
+        m2, m3 = (0 if l else margin), (0 if r else margin)
+        mask[k, 0, 2 * m0:2 * (H2 - m1), 2 * m2:2 * (W2 - m3)] = True
     d = np.where(mask, diff, 0)
     loss = float(np.sum(np.asarray(d, np.float64) ** 2) / count)
     if emulate:

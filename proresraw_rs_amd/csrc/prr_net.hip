@@ -613,7 +613,11 @@ __global__ void k_pack_windows(NetFrame f, const uint32_t *__restrict__ tiles, i
 /* ------------------------------------------------------------------ */
 /* Loss: mean squared error between out + bilinear and the target over
  * the cells at least `margin` from the crop's edge, and its gradient with
- * respect to out (bf16). One lane per cell. */
+ * respect to out (bf16). One lane per cell. A crop side on the picture's
+ * border keeps no margin: there the zero padding of the convolution and the
+ * mirrored bilinear base are the whole-frame ones, so the cells along the
+ * picture's sides train as inference sees them. The gradient keeps the
+ * scale of the interior count. */
 
 __global__ __launch_bounds__(256) void k_loss(const NetFrame *__restrict__ frames,
                                               const Crop *__restrict__ crops,
@@ -634,9 +638,12 @@ __global__ __launch_bounds__(256) void k_loss(const NetFrame *__restrict__ frame
 #pragma unroll
         for (int ch = 0; ch < C; ch++)
             d[ch] = 0.0f;
-        if (ci >= margin && ci < h2 - margin && cj >= margin && cj < w2 - margin) {
-            const Crop cr = crops[(size_t)(*step % steps) * n + k];
-            const NetFrame &f = frames[cr.frame];
+        const Crop cr = crops[(size_t)(*step % steps) * n + k];
+        const NetFrame &f = frames[cr.frame];
+        const int m_top = cr.y0 == 0 ? 0 : margin, m_left = cr.x0 == 0 ? 0 : margin;
+        const int m_bottom = cr.y0 + 2u * (uint32_t)h2 >= f.height ? 0 : margin;
+        const int m_right = cr.x0 + 2u * (uint32_t)w2 >= f.width ? 0 : margin;
+        if (ci >= m_top && ci < h2 - m_bottom && cj >= m_left && cj < w2 - m_right) {
             const size_t count = (size_t)n * 3 * (size_t)(2 * (h2 - 2 * margin)) *
                                  (size_t)(2 * (w2 - 2 * margin));
             const float scale = 2.0f / (float)count;

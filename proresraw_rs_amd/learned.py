@@ -224,7 +224,13 @@ class Net:
         return bool(used.value), float(psnr.value)
 
 
-def random_crops(rng, frames: List[TrainFrame], steps: int, batch: int, cells: int) -> np.ndarray:
+def random_crops(rng, frames: List[TrainFrame], steps: int, batch: int, cells: int,
+                 hard=None, hard_frac: float = 0.0) -> np.ndarray:
+    """(steps, batch, 3) crops of 2*cells sensels: frame, top, left (even).
+    With `hard` (per frame, the squared error of each 64x64 tile of the
+    current network's output), a fraction hard_frac of the crops are drawn
+    around tiles picked in proportion to their error (frames likewise), so
+    training dwells where the network is furthest from plain develop."""
     crops = np.zeros((steps, batch, 3), np.uint32)
     idx = rng.integers(0, len(frames), size=(steps, batch))
     for k, f in enumerate(frames):
@@ -235,19 +241,63 @@ def random_crops(rng, frames: List[TrainFrame], steps: int, batch: int, cells: i
         crops[..., 0][sel] = k
         crops[..., 1][sel] = 2 * rng.integers(0, (f.header.height - 2 * cells) // 2 + 1, size=n)
         crops[..., 2][sel] = 2 * rng.integers(0, (f.header.width - 2 * cells) // 2 + 1, size=n)
+    if hard is None or hard_frac <= 0:
+        return crops
+    flat = crops.reshape(-1, 3)
+    pick = np.nonzero(rng.random(flat.shape[0]) < hard_frac)[0]
+    sizes = [t.size for t in hard]
+    weights = np.concatenate([t.reshape(-1) for t in hard]).astype(np.float64)
+    weights /= weights.sum()
+    starts = np.cumsum([0] + sizes)
+    chosen = rng.choice(weights.size, size=pick.size, p=weights)
+    k = np.searchsorted(starts, chosen, side="right") - 1
+    for i, c, fk in zip(pick, chosen, k):
+        f = frames[fk]
+        t = c - starts[fk]
+        tw = hard[fk].shape[1]
+        ty, tx = divmod(int(t), tw)
+        side = 2 * cells
+        # The tile inside the crop's loss region, at a random place.
+        top = 64 * ty + 32 - side // 2 + int(rng.integers(-cells // 2, cells // 2 + 1))
+        left = 64 * tx + 32 - side // 2 + int(rng.integers(-cells // 2, cells // 2 + 1))
+        top = min(max(top, 0), f.header.height - side) & ~1
+        left = min(max(left, 0), f.header.width - side) & ~1
+        flat[i] = (fk, top, left)
     return crops
+
+
+def tile_errors(net: "Net", frames: List[TrainFrame], stream=None) -> List[np.ndarray]:
+    """Per frame, the squared error of the network's develop(Demosaiced)
+    against the frame's target, summed over each 64x64 tile."""
+    torch = _torch()
+    s = stream if stream is not None else torch.cuda.Stream()
+    out = []
+    for f in frames:
+        h, w = f.header.height, f.header.width
+        planes = [torch.empty(h * w, dtype=torch.float32, device="cuda") for _ in range(3)]
+        net.develop(f.header, f.samples, f.params, Stage.Demosaiced, planes, check_tiles=0,
+                    stream=s)
+        s.synchronize()
+        d = (torch.stack(planes).reshape(3, h, w) - f.target.reshape(3, h, w)).double() ** 2
+        th, tw = h // 64, w // 64
+        e = d[:, :th * 64, :tw * 64].reshape(3, th, 64, tw, 64).sum(dim=(0, 2, 4))
+        out.append(e.cpu().numpy())
+    return out
 
 
 def train(net: Net, frames: List[TrainFrame], steps: int, batch: int = 16, cells: int = 144,
           margin: int = 8, lr: float = 1e-3, seed: int = 0, log_every: int = 1000,
-          log=print, use_graph: bool = True, lr_min: Optional[float] = None):
+          log=print, use_graph: bool = True, lr_min: Optional[float] = None, hard=None,
+          hard_frac: float = 0.0):
     """Adam over random crops of 2*cells sensels (loss on the cells at least
     `margin` from the crop edge); one step captured in a graph and replayed.
     With lr_min, the rate decays from lr to lr_min along a cosine over the
-    run. Returns the list of (step, loss)."""
+    run; with `hard` (tile_errors) a fraction hard_frac of the crops sit
+    around the hardest tiles. Returns the list of (step, loss)."""
     torch = _torch()
     rng = np.random.default_rng(seed)
-    net.train_setup(frames, random_crops(rng, frames, steps, batch, cells), cells, margin)
+    net.train_setup(frames, random_crops(rng, frames, steps, batch, cells, hard, hard_frac),
+                    cells, margin)
     net.set_schedule(lr_min if lr_min is not None else 0.0, steps if lr_min is not None else 0)
     stream = torch.cuda.Stream()
     history = []

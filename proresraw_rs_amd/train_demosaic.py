@@ -50,9 +50,20 @@ def develop_plain(torch, header, samples, params, stage, planes, stream):
                                       ctypes.byref(err)), err)
 
 
+def tile_psnr(a, b, h, w):
+    """PSNR (peak 1) of every whole 64x64 tile between two 3 x h x w
+    tensors."""
+    d = (a - b).double() ** 2
+    th, tw = h // 64, w // 64
+    d = d[:, :th * 64, :tw * 64].reshape(3, th, 64, tw, 64).sum(dim=(0, 2, 4)) / (3 * 4096)
+    return (-10.0 * np.log10(np.maximum(d.cpu().numpy(), 1e-20))).reshape(-1)
+
+
 def evaluate(torch, net, zero, packets, stages, log=log):
     """Per frame and stage: PSNR / max abs of learned and bilinear against
-    plain develop, and develop times (ms, events on one stream)."""
+    plain develop, the PSNR of every 64x64 tile (stages without the output
+    matrix: what the sampled-tile check compares), and develop times (ms,
+    events on one stream)."""
     stream = torch.cuda.Stream()
     rows = []
     for k, (packet, q) in enumerate(packets):
@@ -87,6 +98,11 @@ def evaluate(torch, net, zero, packets, stages, log=log):
                 mse = float((d * d).mean())
                 row[name] = {"psnr": (-10.0 * math.log10(mse)) if mse > 0 else float("inf"),
                              "max_abs": float(d.abs().max())}
+            if stage != Stage.Rec2020:
+                tp = tile_psnr(torch.stack(out["learned"]).reshape(3, h.height, h.width),
+                               ref.float().reshape(3, h.height, h.width), h.height, h.width)
+                row["tiles"] = {"n": int(tp.size), "below_45": float((tp < 45.0).mean()),
+                                "p1": float(np.percentile(tp, 1)), "min": float(tp.min())}
             checked = [torch.empty(n, dtype=torch.float32, device="cuda") for _ in range(3)]
             net.develop(h, samples, params, stage, checked, check_tiles=8, min_db=45.0,
                         stream=stream)
@@ -103,10 +119,14 @@ def evaluate(torch, net, zero, packets, stages, log=log):
 def summarize(rows):
     out = {}
     for r in rows:
-        key = f"{r['stage']}/q{r['qscale']}"
+        key = f"{r['stage']}/{r['width']}x{r['height']}/q{r['qscale']}"
         s = out.setdefault(key, {"frames": 0, "learned_psnr": [], "bilinear_psnr": [],
-                                 "learned_max_abs": 0.0, "fallbacks": 0})
+                                 "learned_max_abs": 0.0, "fallbacks": 0, "tiles_below_45": [],
+                                 "tile_min": 200.0})
         s["frames"] += 1
+        if "tiles" in r:
+            s["tiles_below_45"].append(r["tiles"]["below_45"])
+            s["tile_min"] = min(s["tile_min"], r["tiles"]["min"])
         s["fallbacks"] += int(r["check"]["fallback"])
         s["learned_psnr"].append(r["learned"]["psnr"])
         s["bilinear_psnr"].append(r["bilinear"]["psnr"])
@@ -116,6 +136,25 @@ def summarize(rows):
             v = s.pop(k)
             s[k + "_mean"] = float(np.mean(v))
             s[k + "_min"] = float(np.min(v))
+        t = s.pop("tiles_below_45")
+        s["tiles_below_45_mean"] = float(np.mean(t)) if t else None
+    return out
+
+
+def stream_packets(n, seed):
+    """n frames of each kind of the end-to-end stream, seeds of their own:
+    8192x4320 v1 (seeded curve) alignment 4 qscale 15 and 5760x3240 v0
+    alignment 3 qscale 45, RGGB."""
+    from . import synth as S
+    from .workloads import seeded_curve
+    out = []
+    for i in range(n):
+        sc = S.scene(8192, 4320, 0, seed=seed + 2 * i, version=1, alignment=4, qscale=(15, 15),
+                     curve=seeded_curve(seed + 2 * i), white_level=4095)
+        out.append((S.encode_scene(sc), 15))
+        sc = S.scene(5760, 3240, 0, seed=seed + 2 * i + 1, version=0, alignment=3,
+                     qscale=(45, 45), white_level=4095)
+        out.append((S.encode_scene(sc), 45))
     return out
 
 
@@ -133,6 +172,13 @@ def main(argv=None):
     ap.add_argument("--eval-size", default="5760x3240")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--workers", type=int, default=16)
+    ap.add_argument("--init", default=None,
+                    help="start from these weights (an npz of save_weights) instead of new ones")
+    ap.add_argument("--hard-frac", type=float, default=0.0,
+                    help="fraction of crops drawn around the 64x64 tiles where the starting "
+                         "network is furthest from the target")
+    ap.add_argument("--eval-stream", type=int, default=0,
+                    help="also evaluate on this many frames of each end-to-end stream kind")
     args = ap.parse_args(argv)
     import torch
 
@@ -147,17 +193,27 @@ def main(argv=None):
     del packets
     log(f"decoded and developed in {time.time() - t0:.1f} s")
 
-    net = L.Net(L.init_params(args.seed))
     lo, hi = L.noise_feature(15), L.noise_feature(45)
+    net = L.Net(L.load_weights(args.init)[0] if args.init else L.init_params(args.seed))
+    net.set_range(lo, hi)
+    hard = None
+    if args.hard_frac > 0:
+        t0 = time.time()
+        hard = L.tile_errors(net, frames)
+        log(f"tile errors of the starting network in {time.time() - t0:.1f} s")
     t0 = time.time()
-    hist = L.train(net, frames, args.steps, batch=args.batch, cells=144, margin=8, lr=args.lr,
-                   seed=args.seed, log_every=1000, log=log,
-                   lr_min=args.lr_min if args.lr_min > 0 else None)
+    hist = []
+    if args.steps > 0:
+        hist = L.train(net, frames, args.steps, batch=args.batch, cells=144, margin=8,
+                       lr=args.lr, seed=args.seed, log_every=1000, log=log,
+                       lr_min=args.lr_min if args.lr_min > 0 else None, hard=hard,
+                       hard_frac=args.hard_frac)
     train_s = time.time() - t0
     params = net.params()
     L.save_weights(os.path.join(args.out, "demosaic_net.npz"), params, lo, hi,
                    steps=args.steps, batch=args.batch, lr=args.lr, lr_min=args.lr_min,
-                   curves=args.curves, scenes=args.scenes,
+                   curves=args.curves, scenes=args.scenes, init=str(args.init),
+                   hard_frac=args.hard_frac,
                    loss_history=np.array(hist, np.float64))
     log(f"trained {args.steps} steps in {train_s:.1f} s")
     del frames
@@ -169,10 +225,12 @@ def main(argv=None):
     t0 = time.time()
     held = L.scene_packets(args.eval, (ew, eh), args.seed + 7919, workers=args.workers,
                            curves=args.curves)
-    log(f"{args.eval} held-out frames encoded in {time.time() - t0:.1f} s")
+    if args.eval_stream:
+        held = held + stream_packets(args.eval_stream, args.seed * 1000 + 50000)
+    log(f"{len(held)} held-out frames encoded in {time.time() - t0:.1f} s")
     rows = evaluate(torch, net, zero, held,
                     [(Stage.Demosaiced, None), (Stage.WhiteBalanced, (2.0, 1.6)),
-                     (Stage.Rec2020, None)])
+                     (Stage.WhiteBalanced, None), (Stage.Rec2020, None)])
     report = {"train_seconds": train_s, "steps": args.steps, "batch": args.batch,
               "scenes": args.scenes, "scene_size": args.scene_size, "eval_size": args.eval_size,
               "final_loss": hist[-1][1] if hist else None, "summary": summarize(rows),

@@ -105,6 +105,30 @@ def test_forward_layers_loss_and_gradients_match_oracle(env):
         assert np.abs(g - gg).max() <= 0.02 * scale, (l, np.abs(g - gg).max(), scale)
 
 
+def test_loss_keeps_the_picture_border_cells(env):
+    """Crops on the picture's corners: the sides on the border keep no
+    margin in the loss (the convolution's zero padding and the mirrored
+    bilinear base there are the whole-frame ones), the inner sides keep it;
+    the loss gradient at out stays exact against the oracle."""
+    torch, L, frames = env
+    net = L.Net()
+    params = L.init_params(6)
+    for l in range(1, 8):
+        params[l * NO.LAYER_PARAMS:l * NO.LAYER_PARAMS + 9216] *= 3.0
+    h = frames[1].header
+    crops = np.array([[[0, 0, 0], [1, h.height - 2 * CELLS, h.width - 2 * CELLS]]], np.uint32)
+    t = run_step(L, net, frames, params, crops)
+    wins, targets, bases = zip(*[crop_host(frames[c[0]], c[1], c[2], CELLS) for c in crops[0]])
+    borders = [(True, False, True, False), (False, True, False, True)]
+    loss, dout = NO.loss_and_dout(t[L.OUT], np.stack(bases), np.stack(targets), MARGIN,
+                                  borders=borders)
+    assert np.array_equal(t[L.DOUT], dout)
+    assert abs(float(t[L.LOSS][0]) - loss) <= 1e-5 * loss
+    # The border cells do carry gradient (and the far inner cells none).
+    assert np.abs(t[L.DOUT][0, 0, 0]).max() > 0 and np.abs(t[L.DOUT][1, -1, -1]).max() > 0
+    assert np.abs(t[L.DOUT][0, -1, -1]).max() == 0 and np.abs(t[L.DOUT][1, 0, 0]).max() == 0
+
+
 def test_adam_step_is_exact(env):
     torch, L, frames = env
     net = L.Net()
