@@ -43,7 +43,7 @@ def build_hip(verbose=True):
     out = os.path.join(LIB, "libproresraw_amd.so")
     srcs = [os.path.join(CSRC, n) for n in
             ("prr_kernels.hip", "prr_net.hip", "prr_dng.hip", "prr_orient.hip", "prr_host.cpp",
-             "prr_net_host.cpp", "prr_mov.cpp")]
+             "prr_parse.cpp", "prr_net_host.cpp", "prr_mov.cpp")]
     tmp = out + ".tmp"
     _run(["/opt/rocm/bin/hipcc", *HIPCC_FLAGS, *srcs, "-o", tmp], verbose)
     os.replace(tmp, out)
