@@ -13,7 +13,10 @@ mosaic of `Conversion::new` (demosaic.rs:275-304) and the output of
 Network, on the mosaic packed at half resolution (NHWC, 32 channels):
   x0      channels 0-3: R, G on the red row, G on the blue row, B of each
           2x2 cell of the RGGB mosaic (other patterns are flipped to RGGB);
-          channel 4: the noise feature 128*sqrt(noise[0]); 5-31: zero
+          channel 4: the noise feature 128*sqrt(noise[0]); 5-8: the
+          noise-level map 128*sigma(v) of the cell's four sensels (sigma
+          the square root of develop's variance, demosaic.rs:420-425);
+          9-31: zero
   h0      = relu(conv0(x0) + b0)
   h_l     = h_{l-1} + relu(conv_l(h_{l-1}) + b_l),   l = 1..6
   out     = conv7(h6) + b7; channel 3*(2*dy+dx)+c is colour c of the
@@ -100,8 +103,18 @@ def preclip(v, clip_r, clip_b):
     return v
 
 
-def pack(v, feature):
-    """RGGB plane (H, W) -> x0 (H/2, W/2, 32), bf16."""
+def sigma128(v, noise, off):
+    """128 * sqrt(c + max(b, a * (off + v)^2)) in float32, in the order of
+    develop's variance (demosaic.rs:420-425)."""
+    f = np.float32
+    a, b, c = (f(x) for x in noise)
+    t = f(off) + np.asarray(v, np.float32)
+    return (f(128.0) * np.sqrt(c + np.maximum(b, a * (t * t)))).astype(np.float32)
+
+
+def pack(v, feature, noise=None, off=0.0):
+    """RGGB plane (H, W) -> x0 (H/2, W/2, 32), bf16; with `noise` (a, b, c)
+    and `off` (black / range), the noise-level map in channels 5-8."""
     h2, w2 = v.shape[0] // 2, v.shape[1] // 2
     x = np.zeros((h2, w2, C), np.float32)
     x[..., 0] = v[0:2 * h2:2, 0:2 * w2:2]
@@ -109,6 +122,9 @@ def pack(v, feature):
     x[..., 2] = v[1:2 * h2:2, 0:2 * w2:2]
     x[..., 3] = v[1:2 * h2:2, 1:2 * w2:2]
     x[..., 4] = np.float32(feature)
+    if noise is not None:
+        for k in range(4):
+            x[..., 5 + k] = sigma128(x[..., k], noise, off)
     return bf16(x)
 
 

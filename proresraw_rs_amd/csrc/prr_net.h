@@ -42,6 +42,11 @@ struct NetFrame {
     float one_over_range;
     float clip_r, clip_b;   /* red and blue sensels clipped at these */
     float feature;          /* 128 * sqrt(noise[0]) */
+    /* The noise model of DemosaicParams (demosaic.rs:57-74): the variance
+     * of a normalized value v is noise_c + max(noise_b, noise_a * (off + v)^2),
+     * off = black / range; noise_a comes from the quantizer on the device
+     * when qstatus is set. */
+    float noise_a, noise_b, noise_c, noise_off;
     /* When set, the feature comes from the quantizer qmat0 * max_qscale
      * read on the device (develop(&frame, stage) after a device decode). */
     const prr_device_status *qstatus;

@@ -5,7 +5,10 @@
  *   x0      the RGGB mosaic packed 4 channels per 2x2 cell (R, G on the
  *           red row, G on the blue row, B) in the normalized units develop
  *           works on, red and blue clipped at the stage's clip values; the
- *           noise feature 128*sqrt(noise[0]) in channel 4; zeros after
+ *           noise feature 128*sqrt(noise[0]) in channel 4; the noise-level
+ *           map 128*sigma(v) of the cell's four sensels in channels 5-8,
+ *           sigma(v) the noise deviation develop's tests compare v with
+ *           (demosaic.rs:420-425); zeros after
  *   layer 0 h0 = relu(conv(x0) + b)
  *   1..6    h = h + relu(conv(h) + b)
  *   7       out = conv(h) + b; channels 3*(2*dy+dx)+c are the colours of
@@ -527,25 +530,45 @@ __device__ inline void bilinear(const NetFrame &f, int oy, int ox, int hh, int w
     }
 }
 
-/* The noise feature 128*sqrt(noise[0]), noise[0] from the quantizer as
- * DemosaicParams::with_white_balance computes it (demosaic.rs:86-101). */
+/* noise[0] of DemosaicParams::with_white_balance (demosaic.rs:86-101): from
+ * the quantizer read on the device, or as given. */
+__device__ inline float noise_a_of(const NetFrame &f)
+{
+    if (!f.qstatus)
+        return f.noise_a;
+    const double q = (double)(f.qmat0 * f.qstatus->max_qscale);
+    return (float)((q * q * 5.0 + 64.0 / 3.0) / 150994944.0);
+}
+
+/* The noise feature 128*sqrt(noise[0]). */
 __device__ inline float feature_of(const NetFrame &f)
 {
     if (!f.qstatus)
         return f.feature;
-    const double q = (double)(f.qmat0 * f.qstatus->max_qscale);
-    const float n0 = (float)((q * q * 5.0 + 64.0 / 3.0) / 150994944.0);
-    return 128.0f * sqrtf(n0);
+    return 128.0f * sqrtf(noise_a_of(f));
+}
+
+/* 128 * the noise deviation of a normalized value: sqrt of develop's
+ * variance (demosaic.rs:420-425), in its operation order. */
+__device__ inline float sigma128(float a, const NetFrame &f, float v)
+{
+    const float t = f.noise_off + v;
+    return 128.0f * sqrtf(f.noise_c + fmaxf(f.noise_b, a * (t * t)));
 }
 
 /* One packed cell: 32 bf16 channels. */
 __device__ inline void pack_cell(const NetFrame &f, int y, int x, uint16_t *dst)
 {
-    const uint16_t r = f2bf(norm_at(f, y, x)), g1 = f2bf(norm_at(f, y, x + 1));
-    const uint16_t g2 = f2bf(norm_at(f, y + 1, x)), b = f2bf(norm_at(f, y + 1, x + 1));
+    const float vr = norm_at(f, y, x), vg1 = norm_at(f, y, x + 1);
+    const float vg2 = norm_at(f, y + 1, x), vb = norm_at(f, y + 1, x + 1);
+    const uint16_t r = f2bf(vr), g1 = f2bf(vg1), g2 = f2bf(vg2), b = f2bf(vb);
+    const float a = noise_a_of(f);
+    const uint16_t sr = f2bf(sigma128(a, f, vr)), sg1 = f2bf(sigma128(a, f, vg1));
+    const uint16_t sg2 = f2bf(sigma128(a, f, vg2)), sb = f2bf(sigma128(a, f, vb));
     uint4 *d = (uint4 *)dst;
-    d[0] = make_uint4(pack2(r, g1), pack2(g2, b), pack2(f2bf(feature_of(f)), 0), 0);
-    d[1] = make_uint4(0, 0, 0, 0);
+    d[0] = make_uint4(pack2(r, g1), pack2(g2, b), pack2(f2bf(feature_of(f)), sr),
+                      pack2(sg1, sg2));
+    d[1] = make_uint4(pack2(sb, 0), 0, 0, 0);
     d[2] = make_uint4(0, 0, 0, 0);
     d[3] = make_uint4(0, 0, 0, 0);
 }
@@ -847,6 +870,7 @@ hipError_t launch_conv_infer(int mode, const uint16_t *in, int h, int w, const u
 }
 
 const void *conv_infer_kernel() { return (const void *)k_conv_infer<IM_RES>; }
+
 
 hipError_t launch_weights(const float *params, uint16_t *wf, uint16_t *wt, hipStream_t s)
 {

@@ -197,8 +197,9 @@ extern "C" int prr_net_set_range(prr_net *net, float lo, float hi, prr_error *er
 
 /* The NetFrame of a mosaic developed at `d` (flips bring the pattern to
  * RGGB: the red sensel of BGGR sits at (1, 1), and so on). */
-static NetFrame net_frame(const prr_header *hdr, const DevelopDesc &d, float noise0,
-                          const prr_device_status *qstatus, const float *target)
+static NetFrame net_frame(const prr_header *hdr, const DevelopDesc &d,
+                          const prr_demosaic_params &p, const prr_device_status *qstatus,
+                          const float *target)
 {
     NetFrame f;
     std::memset(&f, 0, sizeof f);
@@ -213,7 +214,11 @@ static NetFrame net_frame(const prr_header *hdr, const DevelopDesc &d, float noi
     f.one_over_range = d.one_over_range;
     f.clip_r = d.clip[0];
     f.clip_b = d.clip[1];
-    f.feature = 128.0f * std::sqrt(noise0);
+    f.feature = 128.0f * std::sqrt(p.noise[0]);
+    f.noise_a = p.noise[0];
+    f.noise_b = p.noise[1];
+    f.noise_c = p.noise[2];
+    f.noise_off = p.black_over_range;
     f.qstatus = qstatus;
     f.qmat0 = hdr->quant_matrix[0];
     f.flip_y = (uint32_t)(d.cfa_phase >> 1) & 1u;
@@ -237,7 +242,7 @@ extern "C" int prr_net_train_setup(prr_net *net, const prr_net_frame *frames, ui
             return net_fail(err, PRR_E_INVALID_ARGUMENT, "frame samples or target", i);
         DevelopDesc d = develop_desc(&fr.header, fr.d_samples, &fr.params, PRR_STAGE_DEMOSAICED);
         d.stride = fr.stride;
-        nf[i] = net_frame(&fr.header, d, fr.params.noise[0], nullptr, fr.d_target);
+        nf[i] = net_frame(&fr.header, d, fr.params, nullptr, fr.d_target);
         if (nf[i].flip_x || nf[i].flip_y)
             return net_fail(err, PRR_E_INVALID_ARGUMENT, "training frames must be RGGB", i);
     }
@@ -430,7 +435,7 @@ extern "C" int prr_net_develop_device(prr_net *net, const prr_header *hdr,
         d.qmat0 = hdr->quant_matrix[0];
         d.range = (uint16_t)(hdr->white_level - 256u);
     }
-    const NetFrame f = net_frame(hdr, d, params->noise[0], d_status, nullptr);
+    const NetFrame f = net_frame(hdr, d, *params, d_status, nullptr);
     Finish fin;
     std::memset(&fin, 0, sizeof fin);
     fin.wb[0] = d.wb[0];

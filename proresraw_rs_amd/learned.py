@@ -48,12 +48,13 @@ def noise_feature(quantizer: int) -> float:
 def init_params(seed: int = 0) -> np.ndarray:
     """He-normal weights scaled down on the residual and output layers (the
     network starts close to the bilinear base); zero biases. Input channels
-    past the 5 used ones and output channels past the 12 used ones stay
-    zero (their gradients are zero, so training keeps them zero)."""
+    past the 9 used ones (mosaic, noise feature, noise-level map) and output
+    channels past the 12 used ones stay zero (their gradients are zero, so
+    training keeps them zero)."""
     rng = np.random.default_rng(seed)
     p = np.zeros(PARAMS, np.float32)
     for l in range(LAYERS):
-        cin = 5 if l == 0 else C
+        cin = 9 if l == 0 else C
         cout = 12 if l == LAYERS - 1 else C
         scale = 1.0 if l == 0 else (0.25 if l < LAYERS - 1 else 0.1)
         w = np.zeros((9, C, C), np.float32)

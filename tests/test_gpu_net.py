@@ -71,7 +71,9 @@ def test_forward_layers_loss_and_gradients_match_oracle(env):
     wins, targets, bases = zip(*[crop_host(frames[c[0]], c[1], c[2], CELLS) for c in crops[0]])
 
     # Packing is exact.
-    x0 = np.stack([NO.pack(w, L.noise_feature(frames[c[0]].quantizer)) for w, c in zip(wins, crops[0])])
+    x0 = np.stack([NO.pack(w, L.noise_feature(frames[c[0]].quantizer), frames[c[0]].params.noise,
+                           frames[c[0]].params.black_over_range) for w, c in zip(wins, crops[0])])
+    assert np.abs(x0[..., 5:9]).min() > 0  # the noise-level map is there
     assert np.array_equal(t[L.X0], x0)
 
     # Each layer against the oracle on the GPU's own input.
@@ -284,18 +286,19 @@ def test_trained_network_passes_the_floor_and_beats_bilinear(env):
         assert abs(worst - want) < 1e-3, (worst, want)
 
 
-def test_inference_convolution_equals_the_training_forward(env):
+@pytest.mark.parametrize("cells", [72, 100, 131])
+def test_inference_convolution_equals_the_training_forward(env, cells):
     """The inference convolution (persistent, weights held in registers,
     12 output channels) does the training forward's arithmetic: on a frame
     that is exactly one training crop, the learned develop at
     Stage::Demosaiced equals, bit for bit, min(out + bilinear, 1) built
-    from the training forward's output of the same crop."""
+    from the training forward's output of the same crop; frame sizes that
+    are and are not multiples of its 4 x 64-cell tile."""
     torch, L, _ = env
     from proresraw_rs_amd import synth as S
     from proresraw_rs_amd.api import Stage
-    cells = 72
-    sc = S.scene(2 * cells, 2 * cells, 0, seed=4242, version=0, alignment=2, qscale=(30, 30),
-                 white_level=4095)
+    sc = S.scene(2 * cells, 2 * cells, 0, seed=4242 + cells, version=0, alignment=2,
+                 qscale=(30, 30), white_level=4095)
     f = L.frame_pair(S.encode_scene(sc))
     net = L.Net(L.init_params(11))
     net.set_range(0.0, 100.0)
