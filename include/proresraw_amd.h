@@ -203,9 +203,12 @@ int prr_develop_into(prr_context *ctx, const prr_header *hdr, const uint16_t *sa
 /* ---- device-resident twins (enqueue only) ------------------------------ */
 /* Decode a packet already in device memory. `hdr` and `info` come from
  * prr_parse_frame_info on the packet's first bytes. `d_packet` must be 4-byte
- * aligned and readable for ((len + 3) & ~3) + 8 bytes. `d_samples` holds
- * coded_width*coded_height values; `d_linear`, when not NULL, receives the
- * linearized visible picture (width*height) in the same pass;
+ * aligned and readable for ((len + 3) & ~3) + 8 bytes. `d_samples` (16-byte
+ * aligned) holds coded_width*coded_height values; the decode also uses it
+ * to stage each tile's coefficients before they are transformed, so its
+ * contents are undefined until the decode completes. `d_linear` (4-byte
+ * aligned), when not NULL, receives the linearized visible picture
+ * (width*height) in the same pass;
  * `d_tile_flags`, when not NULL, gets one byte per tile (1 = skipped);
  * `d_status` receives the frame's prr_device_status. */
 int prr_decode_frame_device(prr_context *ctx, const prr_header *hdr, const prr_frame_info *info,
@@ -389,8 +392,8 @@ typedef struct {
     int32_t explicit_wb;      /* DEVELOP: 0 header gains, 1 wb_red / wb_blue */
     float wb_red;
     float wb_blue;
-    uint16_t *d_samples;      /* DECODE / DEVELOP: coded_width * coded_height */
-    uint16_t *d_linear;       /* DECODE / DEVELOP, optional: width * height */
+    uint16_t *d_samples;      /* DECODE / DEVELOP: coded_width * coded_height, 16-byte aligned */
+    uint16_t *d_linear;       /* DECODE / DEVELOP, optional: width * height, 4-byte aligned */
     uint16_t *d_rgb;          /* REDUCED */
     float *d_r, *d_g, *d_b;   /* DEVELOP: planes (or interleaved in d_r) */
     uint8_t *d_tile_flags;    /* optional, one byte per tile */

@@ -152,17 +152,20 @@ extern "C" int prr_set_decode_config(prr_context *ctx, uint32_t streams_per_wg, 
 
 /* The decode launch of a frame of nb_tiles tiles. The automatic width is
  * the narrowest of 8, 16, 32 and 64 streams per workgroup that keeps the
- * decode's waves at 2.5 a SIMD or fewer, counting the streams of every
- * frame decoding at once: a wave's loop runs as long as its longest
- * stream, so wider groups issue fewer wave instructions for the same
- * symbols (the streams of a frame are of similar length), while the waves
- * left must still hide the loop's latency from one another. */
+ * decode's waves at 1.25 a SIMD or fewer, counting the streams of every
+ * frame decoding at once: a wave's
+ * loop runs as long as its longest stream, so wider groups issue fewer
+ * wave instructions for the same symbols (the streams of a frame are of
+ * similar length), while the waves left must still hide the loop's
+ * latency from one another. On 5760x3240 frames (36,540 streams) this is
+ * 32 alone and 64 with two frames decoding, the fastest widths measured
+ * for each. */
 static DecodeLaunch frame_launch(const prr_context *ctx, uint32_t nb_tiles)
 {
     DecodeLaunch l = ctx->launch;
     if (ctx->spw_auto) {
         const uint64_t streams = 4ull * nb_tiles * ctx->in_flight;
-        const uint64_t max_waves = (uint64_t)ctx->simds * 5 / 2;
+        const uint64_t max_waves = (uint64_t)ctx->simds * 5 / 4;
         uint32_t spw = 8;
         while (spw < 64 && streams / spw > max_waves)
             spw <<= 1;
@@ -220,6 +223,8 @@ extern "C" int prr_decode_frame_device(prr_context *ctx, const prr_header *hdr,
         return rc;
     if (!d_samples || (idct != PRR_IDCT_APPLE && idct != PRR_IDCT_FFMPEG))
         return fail(err, PRR_E_INVALID_ARGUMENT, "samples or idct");
+    if (((uintptr_t)d_samples & 15) || ((uintptr_t)d_linear & 3))
+        return fail(err, PRR_E_INVALID_ARGUMENT, "samples or linear alignment");
     hipStream_t s = (hipStream_t)stream;
     HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
     rc = ensure(ctx, ctx->offsets, sizeof(uint64_t) * ((size_t)info->nb_tiles + 1), err);
@@ -247,6 +252,8 @@ extern "C" int prr_decode_frame_device_noalloc(prr_context *ctx, const prr_heade
         return rc;
     if (!d_samples || (idct != PRR_IDCT_APPLE && idct != PRR_IDCT_FFMPEG))
         return fail(err, PRR_E_INVALID_ARGUMENT, "samples or idct");
+    if (((uintptr_t)d_samples & 15) || ((uintptr_t)d_linear & 3))
+        return fail(err, PRR_E_INVALID_ARGUMENT, "samples or linear alignment");
     if (info->nb_tiles > PRR_NOALLOC_MAX_TILES)
         return fail(err, PRR_E_INVALID_ARGUMENT, "tiles above static scratch", info->nb_tiles,
                     PRR_NOALLOC_MAX_TILES);
@@ -649,7 +656,7 @@ extern "C" int prr_develop_into(prr_context *ctx, const prr_header *hdr, const u
 /* What a frame's tile-group layout depends on, kept so the layout can be
  * redone when the streams per workgroup change. */
 struct BatchShape {
-    uint32_t per_block, alignment, nb_tiles;
+    uint32_t per_block, alignment, nb_tiles, full;
 };
 
 struct prr_batch {
@@ -1205,9 +1212,13 @@ extern "C" void prr_batch_destroy(prr_batch *b)
     delete b;
 }
 
-static uint32_t frame_spw(uint32_t want, uint32_t per_block, uint32_t amax, uint32_t lds_limit,
-                          size_t &lds)
+static uint32_t frame_spw(uint32_t want, uint32_t per_block, uint32_t amax, uint32_t full,
+                          uint32_t lds_limit, size_t &lds)
 {
+    if (full) {
+        lds = staged_decode_lds_bytes(amax);
+        return want;
+    }
     uint32_t spw = want;
     for (;;) {
         lds = decode_lds_bytes(spw, per_block, amax);
@@ -1218,7 +1229,8 @@ static uint32_t frame_spw(uint32_t want, uint32_t per_block, uint32_t amax, uint
 }
 
 /* Tile groups of every frame for the batch's streams per workgroup: each
- * frame's spw (shrunk until its LDS fits), first queue item and the
+ * frame's spw (a full decode keeps it, its LDS does not grow with it;
+ * a reduced one shrinks until its LDS fits), first queue item and the
  * batch's LDS size. */
 static void relayout(prr_batch *b)
 {
@@ -1228,7 +1240,7 @@ static void relayout(prr_batch *b)
         BatchFrame &bf = b->frames[i];
         const BatchShape &sh = b->shapes[i];
         size_t lds;
-        bf.spw = frame_spw(b->spw, sh.per_block, sh.alignment,
+        bf.spw = frame_spw(b->spw, sh.per_block, sh.alignment, sh.full,
                            std::min(b->ctx->launch.lds_limit, b->lds_budget), lds);
         const uint32_t T = bf.spw / 4;
         b->n_items += (sh.nb_tiles + T - 1) / T;
@@ -1261,6 +1273,8 @@ extern "C" int prr_batch_add(prr_batch *b, const prr_header *hdr, const prr_fram
     } else if (fr->kind == PRR_BATCH_DECODE || fr->kind == PRR_BATCH_DEVELOP) {
         if (!fr->d_samples || (fr->idct != PRR_IDCT_APPLE && fr->idct != PRR_IDCT_FFMPEG))
             return fail(err, PRR_E_INVALID_ARGUMENT, "samples or idct");
+        if (((uintptr_t)fr->d_samples & 15) || ((uintptr_t)fr->d_linear & 3))
+            return fail(err, PRR_E_INVALID_ARGUMENT, "samples or linear alignment");
         bf.mode = fr->idct == PRR_IDCT_FFMPEG ? BATCH_FULL_FFMPEG : BATCH_FULL_APPLE;
         bf.samples = fr->d_samples;
         bf.linear = fr->d_linear;
@@ -1285,7 +1299,8 @@ extern "C" int prr_batch_add(prr_batch *b, const prr_header *hdr, const prr_fram
     b->offset_base.push_back(b->offsets_total);
     b->offsets_total += (size_t)info->nb_tiles + 1;
     b->frames.push_back(bf);
-    b->shapes.push_back({per_block, hdr->alignment, info->nb_tiles});
+    b->shapes.push_back({per_block, hdr->alignment, info->nb_tiles,
+                         fr->kind == PRR_BATCH_REDUCED ? 0u : 1u});
     relayout(b);
     if (fr->kind == PRR_BATCH_DEVELOP) {
         BatchDevelop j;

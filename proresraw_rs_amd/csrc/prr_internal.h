@@ -79,7 +79,7 @@ struct BatchFrame {
     uint32_t mode;
     uint32_t average;
     uint32_t spw;
-    uint16_t *samples;       /* full decode */
+    uint16_t *samples;       /* full decode: also the stage of its coefficients */
     uint16_t *linear;        /* full decode, optional */
     uint16_t *rgb;           /* reduced decode */
     uint8_t *flags;
@@ -148,6 +148,8 @@ hipError_t launch_batch(const BatchFrame *d_frames, uint32_t n_frames, uint32_t 
                         const BatchDevelop *d_jobs, uint32_t n_jobs, uint32_t max_w, uint32_t max_h, uint32_t grid,
                         uint32_t chunk, size_t lds, hipStream_t s);
 size_t decode_lds_bytes(uint32_t spw, uint32_t per_block, uint32_t amax);
+/* LDS of a full decode workgroup (any spw). */
+size_t staged_decode_lds_bytes(uint32_t amax);
 
 /* Layout of the allocation-free decode's static device scratch: tile
  * offsets (PRR_NOALLOC_MAX_TILES + 1 uint64, padded to 256 bytes), then the

@@ -5,8 +5,10 @@
  *                 decoder.rs:104-116), tile groups ordered longest first
  *  k_decode<F>    one wave per group of tiles: one lane per component
  *                 stream runs the entropy decoder (bits.rs,
- *                 coefficients.rs) over the packet into LDS coefficient
- *                 blocks, then the wave runs the 8x8 inverse transform
+ *                 coefficients.rs) over the packet, staging coefficients
+ *                 in the tile's own region of the code-value buffer; a
+ *                 chunk of blocks at a time goes to LDS and the wave runs
+ *                 the 8x8 inverse transform
  *                 (idct.rs Apple or FFmpeg), places each stream on its
  *                 Bayer phase (decoder.rs:272-307) through an LDS tile and
  *                 writes code values and, fused, linear values (curve.rs)
@@ -281,23 +283,55 @@ __device__ inline void codeword(uint32_t p, uint32_t cb, uint32_t &value, uint32
 
 enum { S_RUN = 0, S_ZEROS = 1, S_LEVEL = 2, S_AFTER_ZEROS = 3 };
 
+/* Where a lane's decoded coefficients go. Coefficient n of a stream of
+ * 2^lb blocks is scan position n >> lb of block n & (2^lb - 1); both sinks
+ * keep that frequency-major order, which the transforms read without bank
+ * conflicts once the blocks are in LDS. */
+
+/* In LDS, at coefs[((n >> lb) << kshift) + (n & mask)] (kshift = the
+ * frame's alignment, so every stream of a group has the same stride);
+ * symbols that store nothing write coefs[dummy], a padding slot. */
+struct LdsSink {
+    PRR_LDS int16_t *coefs;
+    uint32_t lb, kshift, mask, dummy;
+    __device__ void dc(uint32_t blk, uint32_t v) const { coefs[blk] = (int16_t)v; }
+    __device__ void ac(bool level, uint32_t n, uint32_t v) const
+    {
+        coefs[level ? ((n >> lb) << kshift) + (n & mask) : dummy] = (int16_t)v;
+    }
+};
+
+/* In the frame's code-value buffer, over the stream's own share of its
+ * tile: a tile of 2^lb blocks a stream is 16 rows of 16 << lb sensels, the
+ * same number of values as its four streams' coefficients, so stream s
+ * takes rows 4s .. 4s + 3 of the tile and coefficient n sits at row
+ * n >> (4 + lb), column n & (16 << lb) - 1 of them. The region is zeroed
+ * before the decode; only levels are stored. `row0` is the u16 offset of
+ * the stream's first sensel, gap = coded width - tile width. */
+struct GlobalSink {
+    uint16_t *samples;
+    uint32_t row0, sh, gap;
+    __device__ void dc(uint32_t blk, uint32_t v) const { samples[row0 + blk] = (uint16_t)v; }
+    __device__ void ac(bool level, uint32_t n, uint32_t v) const
+    {
+        if (level)
+            samples[row0 + n + (n >> sh) * gap] = (uint16_t)v;
+    }
+};
+
 /* decode_component_reduced (coefficients.rs:37-146) of one stream by one
  * lane. The DC values are a short loop of their own; the AC part runs one
  * symbol per iteration through four states (run length, zero run, level,
  * level after zeros) with selects rather than branches, so lanes decoding
- * different streams stay in step. Coefficient n (scan position n >> lb of
- * block n & mask) goes to coefs[((n >> lb) << kshift) + (n & mask)]: the
- * frequency-major layout the transform reads without bank conflicts.
- * Symbols that store nothing write coefs[dummy], a padding slot. Returns
- * true if the symbol budget ran out. */
-template <typename P>
+ * different streams stay in step. Coefficients go to `sink`. Returns true
+ * if the symbol budget ran out. */
+template <typename P, class Sink>
 __device__ inline bool decode_stream(Bits<P> &b, uint32_t lb, uint32_t log2_coefs,
-                                     uint32_t kshift, const PRR_LDS uint32_t *tbl,
-                                     PRR_LDS int16_t *coefs, uint32_t dummy, uint32_t budget)
+                                     const PRR_LDS uint32_t *tbl, const Sink &sink,
+                                     uint32_t budget)
 {
     const uint32_t blocks = 1u << lb;
     const uint32_t codes = 1u << (lb + log2_coefs);
-    const uint32_t mask = blocks - 1;
     uint32_t sym, len;
     if (b.hi == 0)
         return false;
@@ -305,7 +339,7 @@ __device__ inline bool decode_stream(Bits<P> &b, uint32_t lb, uint32_t log2_coef
     bits_consume(b, len);
     const uint32_t odd = sym & 1u;
     int32_t dc = (int32_t)((((odd + sym) >> 1) ^ (0u - odd)) + odd);
-    coefs[0] = (int16_t)dc;
+    sink.dc(0, (uint32_t)dc);
     uint32_t dc_cb = 0x064u, sign = 0;
     for (uint32_t blk = 1; blk < blocks; blk++) {
         if (b.hi == 0)
@@ -318,7 +352,7 @@ __device__ inline bool decode_stream(Bits<P> &b, uint32_t lb, uint32_t log2_coef
         sign = delta < 0 ? 1u : 0u;
         bits_consume(b, len);
         dc = (int32_t)((uint32_t)dc + (uint32_t)delta);
-        coefs[blk] = (int16_t)dc;
+        sink.dc(blk, (uint32_t)dc);
     }
 
     uint32_t n = blocks, state = S_RUN, left = 0;
@@ -335,7 +369,7 @@ __device__ inline bool decode_stream(Bits<P> &b, uint32_t lb, uint32_t log2_coef
         const bool negative = (int32_t)(top << (len & 31u)) < 0;
         const uint32_t magnitude = (sym + 1u) & 0xFFFFu;
         const uint32_t value = negative ? 0u - magnitude : magnitude;
-        coefs[level ? ((n >> lb) << kshift) + (n & mask) : dummy] = (int16_t)value;
+        sink.ac(level, n, value);
         const uint32_t n2 = zeros ? n + 1u + sym : level ? n + 1u : n;
         const uint32_t left2 = state == S_RUN ? sym : left - 1u;
         const uint32_t next = zeros                    ? (uint32_t)S_AFTER_ZEROS
@@ -595,13 +629,41 @@ __host__ inline size_t group_lds_bytes(uint32_t spw, uint32_t stride)
     return LDS_HEAD + 16 * sizeof(TileInfo) + coef_bytes;
 }
 
+/* LDS of a full decode (coefficients staged in the code-value buffer):
+ * the chunk copy holds, for each of the chunk's tiles and streams, four
+ * rows of 8 << alignment dwords, padded to `staged_stream_dwords` so that
+ * the lanes reading one scan position of every block hit distinct banks
+ * (pad = half the blocks, at least 4 so each copy is a 16-byte LDS store).
+ * The transform stage (16 rows of the chunk's tiles, 8 KiB) is placed over
+ * the same bytes. */
+__device__ __host__ inline uint32_t staged_stream_dwords(uint32_t amax)
+{
+    const uint32_t half = (1u << amax) >> 1;
+    return (32u << amax) + (half > 4 ? half : 4);
+}
+
+__device__ __host__ inline uint32_t chunk_tiles(uint32_t amax)
+{
+    return 64u / (4u << amax) ? 64u / (4u << amax) : 1u;
+}
+
+__host__ inline size_t staged_lds_bytes(uint32_t amax)
+{
+    return LDS_HEAD + 16 * sizeof(TileInfo) +
+           (size_t)16 * chunk_tiles(amax) * staged_stream_dwords(amax);
+}
+
 /* Everything up to and including the entropy decode. Each lane `l` below
  * 4T decodes stream l & 3 of the group's tile l >> 2 whose streams are
- * listed in `want` (bit s set = decode stream s). Returns the lane's
- * overrun flag. */
+ * listed in `want` (bit s set = decode stream s): into LDS (reduced
+ * decodes) or, STAGED (full decodes), into the tile's region of `samples`
+ * (GlobalSink; zeroed here first).
+ * Returns the lane's overrun flag. */
+template <bool STAGED>
 __device__ inline bool front_end(const FrameDesc &f, Group &g, const TileSet &ts, uint32_t spw,
                                  uint32_t stride, uint32_t log2_coefs,
-                                 uint32_t want, uint8_t *flags, prr_device_status *status)
+                                 uint32_t want, uint8_t *flags, prr_device_status *status,
+                                 uint16_t *samples)
 {
     const uint32_t lane = threadIdx.x;
     g.T = spw >> 2;
@@ -635,9 +697,11 @@ __device__ inline bool front_end(const FrameDesc &f, Group &g, const TileSet &ts
         if (lane < 16)
             ((uint32_t *)g.qmat)[lane] = qv;
     }
-    uint32_t zero_words = (spw * stride) / 2;
-    for (uint32_t i = lane; i < zero_words; i += 64)
-        ((uint32_t *)g.coefs)[i] = 0;
+    if constexpr (!STAGED) {
+        uint32_t zero_words = (spw * stride) / 2;
+        for (uint32_t i = lane; i < zero_words; i += 64)
+            ((uint32_t *)g.coefs)[i] = 0;
+    }
 
     /* Tile::parse (tile.rs:126-154), by the four lanes of each tile. */
     const uint32_t tl = lane >> 2, s = lane & 3;
@@ -674,6 +738,22 @@ __device__ inline bool front_end(const FrameDesc &f, Group &g, const TileSet &ts
             atomicAdd(&status->n_skipped, 1u);
     }
     __syncthreads();
+    if constexpr (STAGED) {
+        /* Zero the regions of the group's decodable tiles, 8 sensels a
+         * store; the barrier's release orders these stores before the
+         * coefficient stores of every lane. */
+        const uint32_t sh_row = 1 + f.alignment; /* log2 of 16-byte pieces a tile row */
+        const uint32_t total = g.n << (4 + sh_row);
+        for (uint32_t i = lane; i < total; i += 64) {
+            const uint32_t tli = i >> (4 + sh_row), row = (i >> sh_row) & 15u;
+            const uint32_t q = i & ((1u << sh_row) - 1);
+            const TileInfo ti = g.tiles[tli];
+            if (ti.valid == 1 && q < (2u << ti.log2))
+                *(uint4 *)(samples + (size_t)(ti.y + row) * f.coded_width + ti.x + 8 * q) =
+                    make_uint4(0, 0, 0, 0);
+        }
+        __syncthreads();
+    }
 
     bool overrun = false;
     if (valid && ((want >> s) & 1u)) {
@@ -688,14 +768,21 @@ __device__ inline bool front_end(const FrameDesc &f, Group &g, const TileSet &ts
          * bytes + codes + 1250, from the reader's refill and shift rules),
          * so it ends only streams the crate would not leave. */
         uint32_t budget = 16u * (8u * (uint32_t)(stop - start) + codes) + 2048u;
-        PRR_LDS int16_t *mine = (PRR_LDS int16_t *)(g.coefs + lane * stride);
         const PRR_LDS uint32_t *tbl = (const PRR_LDS uint32_t *)g.tbl;
-        const uint32_t dummy = (1u << log2_coefs) << f.alignment; /* first padding slot */
         const uint64_t base = start & ~(uint64_t)3;
         Bits<global_words> rd;
         bits_init(rd, (global_words)(f.packet + base), (uint32_t)(start - base),
                   (uint32_t)(stop - base), align);
-        overrun = decode_stream(rd, tlog2, log2_coefs, f.alignment, tbl, mine, dummy, budget);
+        if constexpr (STAGED) {
+            const GlobalSink sink{samples, (ty + 4 * s) * f.coded_width + tx, 4 + tlog2,
+                                  f.coded_width - (16u << tlog2)};
+            overrun = decode_stream(rd, tlog2, log2_coefs, tbl, sink, budget);
+        } else {
+            const LdsSink sink{(PRR_LDS int16_t *)(g.coefs + lane * stride), tlog2, f.alignment,
+                               (1u << tlog2) - 1,
+                               (1u << log2_coefs) << f.alignment /* first padding slot */};
+            overrun = decode_stream(rd, tlog2, log2_coefs, tbl, sink, budget);
+        }
     }
     if (overrun)
         atomicOr(&status->overrun, 1u);
@@ -899,7 +986,11 @@ __global__ __launch_bounds__(1024) void k_prepare(FrameDesc f, prr_device_status
 }
 
 /* Full decode (decoder.rs:226-307) of tile group `group` (spw/4
- * consecutive tiles) by one 64-lane wave. */
+ * consecutive tiles) by one 64-lane wave. The coefficients are decoded
+ * into the tiles' own regions of `samples` and copied into LDS a chunk at
+ * a time before the transforms, so a wave holds a few KiB of LDS whatever
+ * spw is: 64-stream waves at four a SIMD. (With every stream's
+ * coefficients in LDS, a 64-stream wave took 68 KiB.) */
 template <int FFMPEG>
 __device__ inline void decode_group_full(const FrameDesc &f, const TileSet &ts, uint16_t *samples,
                                          uint16_t *linear, uint8_t *flags,
@@ -907,14 +998,14 @@ __device__ inline void decode_group_full(const FrameDesc &f, const TileSet &ts, 
                                          uint32_t *smem)
 {
     const uint32_t amax = f.alignment;
-    const uint32_t stride = stream_stride(64, amax);
-    Group g = carve(smem, spw, stride);
-    front_end(f, g, ts, spw, stride, 6, 0xFu, flags, status);
+    Group g = carve(smem, spw, 0);
+    front_end<true>(f, g, ts, spw, 0, 6, 0xFu, flags, status, samples);
 
     const uint32_t lane = threadIdx.x;
     const uint32_t blocks_max = 1u << amax;
     const uint32_t W = 16u << amax;                 /* stage tile width */
-    const uint32_t tiles_per_chunk = 64u / (4u << amax) ? 64u / (4u << amax) : 1u;
+    const uint32_t tiles_per_chunk = chunk_tiles(amax);
+    const uint32_t SS = staged_stream_dwords(amax);
     /* Bayer phases of the streams (header.rs:79-82). */
     const uint32_t rr = f.bayer >> 1, rc = f.bayer & 1;
     const uint32_t slots = (4u * g.T) << amax;
@@ -923,16 +1014,46 @@ __device__ inline void decode_group_full(const FrameDesc &f, const TileSet &ts, 
         const uint32_t ls = slot >> amax, b = slot & (blocks_max - 1);
         const uint32_t tl = ls >> 2, s = ls & 3;
         const uint32_t tc = tl - (c0 >> amax) / 4;
-        uint16_t *const stage = chunk_stage(g, c0, amax, stride);
+        /* The transform stage (16 rows of the chunk's tiles) over the
+         * chunk's coefficients: each lane has read its block's coefficients
+         * into registers before the wave writes any stage value (a wave's
+         * LDS operations run in order). */
+        uint16_t *const stage = (uint16_t *)g.coefs;
+        {
+            /* The chunk's coefficients into LDS: each decodable tile's
+             * region, 16 bytes a lane and load, tile t's stream s rows at
+             * (t * 4 + s) * SS dwords. */
+            uint32_t *area = (uint32_t *)g.coefs;
+            const uint32_t first = (c0 >> amax) / 4;
+            const uint32_t sh_row = 1 + amax; /* log2 of 16-byte pieces a tile row */
+            const uint32_t total = tiles_per_chunk << (4 + sh_row);
+            for (uint32_t i = lane; i < total; i += 64) {
+                const uint32_t tcc = i >> (4 + sh_row), row = (i >> sh_row) & 15u;
+                const uint32_t q = i & ((1u << sh_row) - 1);
+                if (first + tcc >= g.n)
+                    continue;
+                const TileInfo ti = g.tiles[first + tcc];
+                if (ti.valid != 1 || q >= (2u << ti.log2))
+                    continue;
+                *(uint4 *)(area + (tcc * 4 + (row >> 2)) * SS + (row & 3) * (8u << amax) + 4 * q) =
+                    *(const uint4 *)(samples + (size_t)(ti.y + row) * f.coded_width + ti.x + 8 * q);
+            }
+            __syncthreads();
+        }
         if (tl < g.n) {
             TileInfo ti = g.tiles[tl];
             if (ti.valid == 1 && b < (1u << ti.log2)) {
-                const int16_t *src = g.coefs + ls * stride + b;
+                /* Scan position p of block b: row p >> 4 of the stream's
+                 * four, column ((p & 15) << log2) + b. */
+                const uint16_t *src =
+                    (const uint16_t *)((uint32_t *)g.coefs + (tc * 4 + s) * SS) + b;
                 int32_t d[64];
 #pragma unroll
-                for (int k = 0; k < 64; k++)
-                    d[k] = dequantize(src[kInv8.v[k] << amax],
+                for (int k = 0; k < 64; k++) {
+                    const uint32_t p = kInv8.v[k];
+                    d[k] = dequantize((int16_t)src[(p >> 4) * W + ((p & 15u) << ti.log2)],
                                       quant_step(g.qmat[k], ti.qscale));
+                }
                 uint32_t v[64];
                 if (FFMPEG)
                     ffmpeg_8x8(d, v);
@@ -994,7 +1115,7 @@ __device__ inline void decode_group_reduced(const FrameDesc &f, const TileSet &t
     const uint32_t stride = stream_stride(per_block, amax);
     Group g = carve(smem, spw, stride);
     const uint32_t want = average ? 0xFu : 0xDu;
-    front_end(f, g, ts, spw, stride, log2_coefs, want, flags, status);
+    front_end<false>(f, g, ts, spw, stride, log2_coefs, want, flags, status, nullptr);
 
     const uint32_t lane = threadIdx.x;
     const uint32_t blocks_max = 1u << amax;
@@ -2107,8 +2228,8 @@ hipError_t launch_decode(const FrameDesc &f, int ffmpeg, uint16_t *samples, uint
                          uint8_t *flags, prr_device_status *status, uint32_t *order,
                          const DecodeLaunch &cfg, hipStream_t s)
 {
-    size_t lds;
-    const uint32_t spw = pick_spw(cfg, 64, f.alignment, lds);
+    const uint32_t spw = cfg.streams_per_wg;
+    const size_t lds = staged_lds_bytes(f.alignment);
     const uint32_t n = f.nb_tiles, solo = min(n, cfg.solo_tiles);
     const uint32_t items = single_frame_items(n, spw, solo);
     hipLaunchKernelGGL(k_prepare, dim3(1), dim3(1024), 0, s, f, status, order, 1u);
@@ -2219,6 +2340,8 @@ size_t decode_lds_bytes(uint32_t spw, uint32_t per_block, uint32_t amax)
 {
     return group_lds_bytes(spw, stream_stride(per_block, amax));
 }
+
+size_t staged_decode_lds_bytes(uint32_t amax) { return staged_lds_bytes(amax); }
 
 /* ------------------------------------------------------------------ */
 /* Device scratch of the allocation-free decode

@@ -270,6 +270,15 @@ def test_new_entry_points_reject_bad_arguments_before_device_work():
     # The allocation-free decode checks its arguments before its context.
     assert lib.prr_decode_frame_device_noalloc(None, None, None, None, 0, 0, None, None, None,
                                                None, None, ctypes.byref(err)) == 101
+    # A device decode stages each tile's coefficients in its sample buffer,
+    # which must be 16-byte aligned (the linear picture 4-byte aligned):
+    # refused before the context is used (the handle is never dereferenced).
+    for samples, linear in ((ctypes.c_void_p(72), None), (p8, ctypes.c_void_p(66))):
+        for fn in (lib.prr_decode_frame_device, lib.prr_decode_frame_device_noalloc):
+            err = N.PrrError()
+            assert fn(ctypes.c_void_p(8), ctypes.byref(hdr), ctypes.byref(info), p8, 16, 0,
+                      samples, linear, None, p8, None, ctypes.byref(err)) == 101
+            assert err.what == b"samples or linear alignment"
     # DNG tiles: a picture narrower than two sensels, a stride below the width.
     buf = (ctypes.c_uint16 * 16)()
     out = (ctypes.c_uint8 * 16)()
