@@ -129,9 +129,9 @@ const char *prr_version(void);
 /* Decode launch shape (no counterpart in the crate, which has no launch to
  * shape): lanes decoding streams per workgroup, 8, 16, 32 or 64, or 0 for
  * the automatic width (the default): the narrowest that keeps the decode at
- * 2.5 waves a SIMD or fewer, counting every frame decoding at once (a
- * 4096x2160 frame decodes 8 wide, a 5760x3240 one 16 wide, two 5760x3240
- * frames of a pipeline 32 wide). */
+ * 1.25 waves a SIMD or fewer, counting every frame decoding at once (a
+ * 4096x2160 frame decodes 16 wide, a 5760x3240 one 32 wide, two 5760x3240
+ * frames of a pipeline 64 wide). */
 int prr_set_decode_config(prr_context *ctx, uint32_t streams_per_wg, prr_error *err);
 
 /* ---- host-side frame structure (no device work) ----------------------- */
