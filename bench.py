@@ -46,8 +46,9 @@ def parse_args():
     ap.add_argument("--batch-frames", type=int, default=64)
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--learned-frames", type=int, default=32)
-    ap.add_argument("--learned-distinct", type=int, default=8,
-                    help="distinct packets (half of each frame kind) cycled in the learned leg")
+    ap.add_argument("--learned-distinct", type=int, default=32,
+                    help="distinct packets (half of each frame kind) cycled in the learned leg "
+                         "(default: every frame its own scene)")
     ap.add_argument("--net-weights", default=None,
                     help="weights of the learned demosaic (default: the shipped ones)")
     ap.add_argument("--no-clip", action="store_true")
@@ -547,8 +548,8 @@ def learned_bench(args, plain_value):
     balance + Rec.2020 matrix with the on-device fallback to plain develop
     (noise outside the trained range, or any of 8 sampled 64x64 tiles below
     45 dB against plain develop). The same batch with plain develop is timed
-    beside it. --learned-distinct packets (half of each kind) are encoded and
-    cycled."""
+    beside it. --learned-distinct packets (half of each kind; by default one a
+    frame) are encoded and cycled."""
     import torch
 
     import proresraw_rs_amd as P

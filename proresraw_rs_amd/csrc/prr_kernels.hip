@@ -314,8 +314,11 @@ struct GlobalSink {
     __device__ void dc(uint32_t blk, uint32_t v) const { samples[row0 + blk] = (uint16_t)v; }
     __device__ void ac(bool level, uint32_t n, uint32_t v) const
     {
+        /* n >> sh is a row (0..3) and gap below 2^16: a 24-bit product,
+         * full rate where a 32-bit multiply issues at quarter rate. */
+        const uint32_t at = row0 + n + __mul24(n >> sh, gap);
         if (level)
-            samples[row0 + n + (n >> sh) * gap] = (uint16_t)v;
+            samples[at] = (uint16_t)v;
     }
 };
 
