@@ -64,6 +64,12 @@ int prr_net_train_setup(prr_net *net, const prr_net_frame *frames, uint32_t n_fr
  * steps (0, the default, keeps it constant); computed on the device from
  * the step count. */
 int prr_net_set_schedule(prr_net *net, float lr_min, uint32_t decay_steps, prr_error *err);
+/* Loss of the following steps: the mean of d^2 + d^4 / tau^2 over the loss
+ * cells' colours (d = network + bilinear - target), which weighs the few
+ * large errors on edges more than the squared error does; tau = 0 (the
+ * default) is the squared error. Takes effect in steps enqueued (or
+ * captured) afterwards. */
+int prr_net_set_loss(prr_net *net, float tau, prr_error *err);
 /* One step: pack, forward, loss, backward, Adam (lr) unless lr is 0, step
  * count + 1. Enqueued on `stream`; allocates nothing. */
 int prr_net_train_step(prr_net *net, float lr, void *stream, prr_error *err);

@@ -237,13 +237,14 @@ def unshuffle(dpred, channels=C):
     return d
 
 
-def loss_and_dout(out, base, target, margin, emulate=True, borders=None):
+def loss_and_dout(out, base, target, margin, emulate=True, borders=None, tau=0.0):
     """Mean squared error over the cells [margin, H2 - margin) x [margin,
     W2 - margin) (all 12 sub-pixel colours) and its gradient with respect to
     out; the per-element derivative 2 * diff / count is rounded to bf16.
     borders[k] = (top, bottom, left, right): the sides of crop k that lie on
     the picture's border, where the margin is 0; count stays the interior
-    one."""
+    one. tau > 0: the mean of diff^2 + diff^4 / tau^2, derivative
+    2 * diff * (1 + 2 * diff^2 / tau^2) / count."""
     N, H2, W2, _ = out.shape
     ft = np.float32 if emulate else np.float64
     pred = shuffle(np.asarray(out, ft)[..., :OUT_CHANNELS]) + np.asarray(base, ft)
@@ -258,12 +259,16 @@ def loss_and_dout(out, base, target, margin, emulate=True, borders=None):
         m2, m3 = (0 if l else margin), (0 if r else margin)
         mask[k, 0, 2 * m0:2 * (H2 - m1), 2 * m2:2 * (W2 - m3)] = True
     d = np.where(mask, diff, 0)
-    loss = float(np.sum(np.asarray(d, np.float64) ** 2) / count)
+    d64 = np.asarray(d, np.float64)
+    inv = 1.0 / (tau * tau) if tau > 0 else 0.0
+    loss = float(np.sum(d64 ** 2 * (1.0 + d64 ** 2 * inv)) / count)
     if emulate:
         scale = np.float32(2.0) / np.float32(count)
-        dpred = (d.astype(np.float32) * scale).astype(np.float32)
+        d32 = d.astype(np.float32)
+        inv32 = np.float32(inv)
+        dpred = ((d32 * scale) * (np.float32(1.0) + np.float32(2.0) * (d32 * d32) * inv32)).astype(np.float32)
     else:
-        dpred = d * (2.0 / count)
+        dpred = d * (2.0 / count) * (1.0 + 2.0 * d * d * inv)
     dout = unshuffle(dpred)
     return loss, (bf16(dout) if emulate else dout)
 

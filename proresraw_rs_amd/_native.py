@@ -210,6 +210,7 @@ SIGNATURES = {
     "prr_net_train_setup": (i32, [c_vp, ctypes.POINTER(PrrNetFrame), u32, c_vp, u32, u32, u32,
                                   u32, E]),
     "prr_net_set_schedule": (i32, [c_vp, ctypes.c_float, u32, E]),
+    "prr_net_set_loss": (i32, [c_vp, ctypes.c_float, E]),
     "prr_net_train_step": (i32, [c_vp, ctypes.c_float, c_vp, E]),
     "prr_net_read": (i32, [c_vp, i32, u32, c_vp, sz, E]),
     "prr_net_develop_device": (i32, [c_vp, H, c_vp, DP, c_vp, i32, u32, ctypes.c_float, c_vp,

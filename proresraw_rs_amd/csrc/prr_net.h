@@ -92,9 +92,12 @@ hipError_t launch_adam(float *params, const float *grads, float *m, float *v, ui
  * into x0. */
 hipError_t launch_pack_crops(const NetFrame *frames, const Crop *crops, const uint32_t *step,
                              uint32_t steps, int n, int h2, int w2, uint16_t *x0, hipStream_t s);
+/* Mean over the loss cells of d^2 + d^4 * inv_tau2 (inv_tau2 = 0: the
+ * squared error) and its gradient at out, rounded to bf16. */
 hipError_t launch_loss(const NetFrame *frames, const Crop *crops, const uint32_t *step,
-                       uint32_t steps, int n, int h2, int w2, int margin, const float *out,
-                       uint16_t *dout, float *loss_part, float *loss, hipStream_t s);
+                       uint32_t steps, int n, int h2, int w2, int margin, float inv_tau2,
+                       const float *out, uint16_t *dout, float *loss_part, float *loss,
+                       hipStream_t s);
 /* Inference over a whole frame (even width and height); nothing when
  * *skip is set. */
 hipError_t launch_pack_frame(const NetFrame &f, uint16_t *x0, const uint32_t *skip, hipStream_t s);

@@ -645,7 +645,7 @@ __global__ void k_pack_windows(NetFrame f, const uint32_t *__restrict__ tiles, i
 __global__ __launch_bounds__(256) void k_loss(const NetFrame *__restrict__ frames,
                                               const Crop *__restrict__ crops,
                                               const uint32_t *__restrict__ step, uint32_t steps,
-                                              int n, int h2, int w2, int margin,
+                                              int n, int h2, int w2, int margin, float inv_tau2,
                                               const float *__restrict__ out,
                                               uint16_t *__restrict__ dout,
                                               float *__restrict__ loss_part)
@@ -684,8 +684,11 @@ __global__ __launch_bounds__(256) void k_loss(const NetFrame *__restrict__ frame
                         const int ch = 3 * (2 * dy + dx) + col;
                         const float pred = out[i * C + ch] + base[col];
                         const float diff = pred - f.target[col * plane + at];
-                        sq += diff * diff;
-                        d[ch] = diff * scale;
+                        /* d^2 + d^4 / tau^2 (inv_tau2 = 0: the squared
+                         * error) and its derivative 2d (1 + 2 d^2 / tau^2). */
+                        const float d2 = diff * diff;
+                        sq += d2 * (1.0f + d2 * inv_tau2);
+                        d[ch] = diff * scale * (1.0f + 2.0f * d2 * inv_tau2);
                     }
                 }
         }
@@ -738,17 +741,53 @@ __global__ __launch_bounds__(256) void k_compose(NetFrame f, Finish fin,
     if (i >= (size_t)h2 * w2)
         return;
     const int ci = (int)(i / w2), cj = (int)(i - (size_t)ci * w2);
+    const int hh = (int)f.height, ww = (int)f.width;
+    /* The cell's 4x4 neighbourhood of normalized values (rows 2ci - 1 ..
+     * 2ci + 2, mirrored at the picture's edges): every value the four
+     * sensels' bilinear estimates read, each read once. */
+    float nb[4][4];
 #pragma unroll
-    for (int dy = 0; dy < 2; dy++)
+    for (int r = 0; r < 4; r++) {
+        const int y = mirror(2 * ci - 1 + r, hh);
+#pragma unroll
+        for (int c = 0; c < 4; c++)
+            nb[r][c] = norm_at(f, y, mirror(2 * cj - 1 + c, ww));
+    }
+    float o[OUT_CH];
+#pragma unroll
+    for (int q = 0; q < OUT_CH / 4; q++) {
+        const float4 v = ((const float4 *)(out + i * OUT_CH))[q];
+        o[4 * q] = v.x;
+        o[4 * q + 1] = v.y;
+        o[4 * q + 2] = v.z;
+        o[4 * q + 3] = v.w;
+    }
+#pragma unroll
+    for (int dy = 0; dy < 2; dy++) {
+        float res[2][3];
 #pragma unroll
         for (int dx = 0; dx < 2; dx++) {
-            const int y = 2 * ci + dy, x = 2 * cj + dx;
+            /* bilinear() at (2ci + dy, 2cj + dx), from the neighbourhood. */
+            auto v = [&](int ky, int kx) { return nb[dy + 1 + ky][dx + 1 + kx]; };
             float base[3];
-            bilinear(f, 0, 0, (int)f.height, (int)f.width, y, x, base);
+            const float c = v(0, 0);
+            if (dy == dx) {
+                const float cross = ((v(-1, 0) + v(1, 0)) + (v(0, -1) + v(0, 1))) * 0.25f;
+                const float diag = ((v(-1, -1) + v(-1, 1)) + (v(1, -1) + v(1, 1))) * 0.25f;
+                base[0] = dy ? diag : c;
+                base[1] = cross;
+                base[2] = dy ? c : diag;
+            } else {
+                const float horiz = (v(0, -1) + v(0, 1)) * 0.5f;
+                const float vert = (v(-1, 0) + v(1, 0)) * 0.5f;
+                base[0] = dy ? vert : horiz;
+                base[1] = c;
+                base[2] = dy ? horiz : vert;
+            }
             const int ch = 3 * (2 * dy + dx);
-            float r = fminf((out[i * OUT_CH + ch + 0] + base[0]) * fin.wb[0], 1.0f);
-            float g = fminf(out[i * OUT_CH + ch + 1] + base[1], 1.0f);
-            float b = fminf((out[i * OUT_CH + ch + 2] + base[2]) * fin.wb[1], 1.0f);
+            float r = fminf((o[ch + 0] + base[0]) * fin.wb[0], 1.0f);
+            float g = fminf(o[ch + 1] + base[1], 1.0f);
+            float b = fminf((o[ch + 2] + base[2]) * fin.wb[1], 1.0f);
             if (fin.has_matrix) {
                 const float r2 = fmaf(b, fin.m[2], fmaf(r, fin.m[0], g * fin.m[1]));
                 const float g2 = fmaf(b, fin.m[5], fmaf(r, fin.m[3], g * fin.m[4]));
@@ -757,13 +796,21 @@ __global__ __launch_bounds__(256) void k_compose(NetFrame f, Finish fin,
                 g = g2;
                 b = b2;
             }
-            const uint32_t sy = f.flip_y ? f.height - 1 - (uint32_t)y : (uint32_t)y;
-            const uint32_t sx = f.flip_x ? f.width - 1 - (uint32_t)x : (uint32_t)x;
-            const size_t at = (size_t)sy * f.width + sx;
-            pr[at] = r;
-            pg[at] = g;
-            pb[at] = b;
+            res[dx][0] = r;
+            res[dx][1] = g;
+            res[dx][2] = b;
         }
+        /* The pair's two sensels are neighbours in sensor order too (the
+         * flip only swaps them): one 8-byte store a plane. */
+        const int y = 2 * ci + dy;
+        const uint32_t sy = f.flip_y ? f.height - 1 - (uint32_t)y : (uint32_t)y;
+        const uint32_t sx0 = f.flip_x ? f.width - 2 - (uint32_t)(2 * cj) : (uint32_t)(2 * cj);
+        const int lo = f.flip_x ? 1 : 0;
+        const size_t at = (size_t)sy * f.width + sx0;
+        *(float2 *)(pr + at) = make_float2(res[lo][0], res[1 - lo][0]);
+        *(float2 *)(pg + at) = make_float2(res[lo][1], res[1 - lo][1]);
+        *(float2 *)(pb + at) = make_float2(res[lo][2], res[1 - lo][2]);
+    }
 }
 
 /* Sampled-tile check: per tile, the network's output at each sensel
@@ -974,13 +1021,14 @@ hipError_t launch_pack_crops(const NetFrame *frames, const Crop *crops, const ui
 }
 
 hipError_t launch_loss(const NetFrame *frames, const Crop *crops, const uint32_t *step,
-                       uint32_t steps, int n, int h2, int w2, int margin, const float *out,
-                       uint16_t *dout, float *loss_part, float *loss, hipStream_t s)
+                       uint32_t steps, int n, int h2, int w2, int margin, float inv_tau2,
+                       const float *out, uint16_t *dout, float *loss_part, float *loss,
+                       hipStream_t s)
 {
     const size_t cells = (size_t)n * h2 * w2;
     const uint32_t blocks = (uint32_t)((cells + 255) / 256);
     hipLaunchKernelGGL(k_loss, dim3(blocks), dim3(256), 0, s, frames, crops, step, steps, n, h2,
-                       w2, margin, out, dout, loss_part);
+                       w2, margin, inv_tau2, out, dout, loss_part);
     const double count = (double)n * 3.0 * (2.0 * (h2 - 2 * margin)) * (2.0 * (w2 - 2 * margin));
     hipLaunchKernelGGL(k_loss_total, dim3(1), dim3(64), 0, s, loss_part, blocks,
                        (float)(1.0 / count), loss);

@@ -58,6 +58,7 @@ struct prr_net {
     uint32_t n_frames = 0, steps = 0, batch = 0, cells = 0, margin = 0;
     float lr_min = 0.0f;
     uint32_t decay = 0;
+    float inv_tau2 = 0.0f; /* loss d^2 + d^4 / tau^2; 0: squared error */
     /* inference */
     Buf ix0, ia, ib, iout, scratch, psnr, flag;
     Buf wx0, wa, wb, wout; /* the sampled-tile windows */
@@ -318,8 +319,9 @@ extern "C" int prr_net_train_step(prr_net *net, float lr, void *stream, prr_erro
                         net->out.as<float>(), s),
             "conv kernel");
     NET_TRY(launch_loss(net->frames.as<NetFrame>(), net->crops.as<Crop>(), step, net->steps, n,
-                        hw, hw, (int)net->margin, net->out.as<float>(), net->dout.as<uint16_t>(),
-                        net->loss_part.as<float>(), net->loss.as<float>(), s),
+                        hw, hw, (int)net->margin, net->inv_tau2, net->out.as<float>(),
+                        net->dout.as<uint16_t>(), net->loss_part.as<float>(),
+                        net->loss.as<float>(), s),
             "loss kernel");
 
     NET_TRY(launch_wgrad(H(6), net->dout.as<uint16_t>(), n, hw, hw, part, G(7), s), "wgrad");
@@ -357,6 +359,14 @@ extern "C" int prr_net_set_schedule(prr_net *net, float lr_min, uint32_t decay_s
         return net_fail(err, PRR_E_INVALID_ARGUMENT, "net or lr_min");
     net->lr_min = lr_min;
     net->decay = decay_steps;
+    return PRR_OK;
+}
+
+extern "C" int prr_net_set_loss(prr_net *net, float tau, prr_error *err)
+{
+    if (!net || !(tau >= 0.0f) || tau > 1e18f)
+        return net_fail(err, PRR_E_INVALID_ARGUMENT, "net or tau");
+    net->inv_tau2 = tau > 0.0f ? 1.0f / (tau * tau) : 0.0f;
     return PRR_OK;
 }
 

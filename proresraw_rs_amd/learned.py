@@ -182,6 +182,10 @@ class Net:
     def set_schedule(self, lr_min: float, decay_steps: int):
         self._call("prr_net_set_schedule", ctypes.c_float(lr_min), decay_steps)
 
+    def set_loss(self, tau: float):
+        """Loss of later steps: mean of d^2 + d^4 / tau^2 (0: squared error)."""
+        self._call("prr_net_set_loss", ctypes.c_float(tau))
+
     def train_step(self, lr: float, stream=None):
         torch = _torch()
         s = stream if stream is not None else torch.cuda.current_stream()
@@ -289,17 +293,19 @@ def tile_errors(net: "Net", frames: List[TrainFrame], stream=None) -> List[np.nd
 def train(net: Net, frames: List[TrainFrame], steps: int, batch: int = 16, cells: int = 144,
           margin: int = 8, lr: float = 1e-3, seed: int = 0, log_every: int = 1000,
           log=print, use_graph: bool = True, lr_min: Optional[float] = None, hard=None,
-          hard_frac: float = 0.0):
+          hard_frac: float = 0.0, tau: float = 0.0):
     """Adam over random crops of 2*cells sensels (loss on the cells at least
     `margin` from the crop edge); one step captured in a graph and replayed.
     With lr_min, the rate decays from lr to lr_min along a cosine over the
     run; with `hard` (tile_errors) a fraction hard_frac of the crops sit
-    around the hardest tiles. Returns the list of (step, loss)."""
+    around the hardest tiles; tau > 0 trains on d^2 + d^4 / tau^2 instead of
+    the squared error. Returns the list of (step, loss)."""
     torch = _torch()
     rng = np.random.default_rng(seed)
     net.train_setup(frames, random_crops(rng, frames, steps, batch, cells, hard, hard_frac),
                     cells, margin)
     net.set_schedule(lr_min if lr_min is not None else 0.0, steps if lr_min is not None else 0)
+    net.set_loss(tau)
     stream = torch.cuda.Stream()
     history = []
     t0 = time.time()

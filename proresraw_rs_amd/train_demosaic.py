@@ -177,6 +177,9 @@ def main(argv=None):
     ap.add_argument("--hard-frac", type=float, default=0.0,
                     help="fraction of crops drawn around the 64x64 tiles where the starting "
                          "network is furthest from the target")
+    ap.add_argument("--tail-tau", type=float, default=0.0,
+                    help="train on d^2 + d^4 / tau^2 (weighs the few large edge errors the "
+                         "sampled-tile check sees); 0: the squared error")
     ap.add_argument("--eval-stream", type=int, default=0,
                     help="also evaluate on this many frames of each end-to-end stream kind")
     args = ap.parse_args(argv)
@@ -207,13 +210,13 @@ def main(argv=None):
         hist = L.train(net, frames, args.steps, batch=args.batch, cells=144, margin=8,
                        lr=args.lr, seed=args.seed, log_every=1000, log=log,
                        lr_min=args.lr_min if args.lr_min > 0 else None, hard=hard,
-                       hard_frac=args.hard_frac)
+                       hard_frac=args.hard_frac, tau=args.tail_tau)
     train_s = time.time() - t0
     params = net.params()
     L.save_weights(os.path.join(args.out, "demosaic_net.npz"), params, lo, hi,
                    steps=args.steps, batch=args.batch, lr=args.lr, lr_min=args.lr_min,
                    curves=args.curves, scenes=args.scenes, init=str(args.init),
-                   hard_frac=args.hard_frac,
+                   hard_frac=args.hard_frac, tail_tau=args.tail_tau,
                    loss_history=np.array(hist, np.float64))
     log(f"trained {args.steps} steps in {train_s:.1f} s")
     del frames

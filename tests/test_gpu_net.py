@@ -131,6 +131,37 @@ def test_loss_keeps_the_picture_border_cells(env):
     assert np.abs(t[L.DOUT][0, -1, -1]).max() == 0 and np.abs(t[L.DOUT][1, 0, 0]).max() == 0
 
 
+def test_tail_weighted_loss_matches_oracle(env):
+    """prr_net_set_loss(tau): the loss d^2 + d^4 / tau^2 and its bf16
+    gradient at out, exact against the oracle; tau = 0 gives back the
+    squared error bit for bit, and the tail term weighs large errors more
+    (a planted tau in the oracle, 0.02 against 0.03, fails)."""
+    torch, L, frames = env
+    net = L.Net()
+    params = L.init_params(8)
+    for l in range(1, 8):
+        params[l * NO.LAYER_PARAMS:l * NO.LAYER_PARAMS + 9216] *= 3.0
+    crops = np.array([[[0, 10, 20], [1, 40, 36]]], np.uint32)
+    wins, targets, bases = zip(*[crop_host(frames[c[0]], c[1], c[2], CELLS) for c in crops[0]])
+    net.set_loss(0.0)
+    t0 = run_step(L, net, frames, params, crops)
+    net.set_loss(0.03)
+    t = run_step(L, net, frames, params, crops)
+    net.set_loss(0.0)
+    assert np.array_equal(t[L.OUT], t0[L.OUT])
+    loss, dout = NO.loss_and_dout(t[L.OUT], np.stack(bases), np.stack(targets), MARGIN, tau=0.03)
+    assert np.array_equal(t[L.DOUT], dout)
+    assert abs(float(t[L.LOSS][0]) - loss) <= 1e-5 * loss
+    mse, dmse = NO.loss_and_dout(t[L.OUT], np.stack(bases), np.stack(targets), MARGIN)
+    assert np.array_equal(t0[L.DOUT], dmse)
+    assert float(t[L.LOSS][0]) > float(t0[L.LOSS][0])
+    big = np.abs(dmse.astype(np.float32)) > 0
+    ratio = np.abs(t[L.DOUT].astype(np.float32)[big]) / np.abs(dmse.astype(np.float32)[big])
+    assert ratio.min() >= 1.0 - 2.0 ** -7 and ratio.max() > 1.5
+    _, wrong = NO.loss_and_dout(t[L.OUT], np.stack(bases), np.stack(targets), MARGIN, tau=0.02)
+    assert not np.array_equal(t[L.DOUT], wrong)
+
+
 def test_adam_step_is_exact(env):
     torch, L, frames = env
     net = L.Net()
@@ -247,6 +278,34 @@ def test_zero_network_is_the_bilinear_base(env):
     assert np.array_equal(got, want)
     # The sampled-tile PSNR the kernel reports is the PSNR of those tiles.
     assert 20.0 < worst < 80.0
+
+
+@pytest.mark.parametrize("pattern", [1, 2, 3])
+def test_zero_network_is_the_bilinear_base_on_every_pattern(env, pattern):
+    """Patterns other than RGGB go through the network flipped to RGGB
+    order and come back in sensor order: with a zero network the output is
+    the flipped picture's bilinear base, flipped back, exactly (the
+    composition writes a cell's two sensels of a row as one pair, swapped
+    by a horizontal flip)."""
+    torch, L, _ = env
+    from proresraw_rs_amd import synth as S
+    from proresraw_rs_amd.api import Stage
+    sc = S.scene(128, 96, pattern, seed=600 + pattern, version=0, alignment=2, qscale=(30, 30),
+                 white_level=4095)
+    f = L.frame_pair(S.encode_scene(sc))
+    h = f.header
+    net = L.Net(np.zeros(NO.PARAMS, np.float32))
+    net.set_range(0.0, 10.0)
+    got, _ = develop_pair(torch, L, f, Stage.Demosaiced, net, check_tiles=0)
+    samples = f.samples.cpu().numpy().view(np.uint16).reshape(h.coded_height(), h.coded_width())
+    norm = NO.normalized(samples[:h.height, :h.width], list(h.curve), f.params.black,
+                         f.params.one_over_range)
+    red_row, red_col = int(h.bayer_pattern) >> 1, int(h.bayer_pattern) & 1
+    assert (red_row, red_col) != (0, 0)
+    flip = (slice(None, None, -1 if red_row else 1), slice(None, None, -1 if red_col else 1))
+    base = NO.bilinear(NO.preclip(np.ascontiguousarray(norm[flip]), 1.0, 1.0))
+    want = np.minimum(base, np.float32(1.0))[(slice(None),) + flip]
+    assert np.array_equal(got, np.ascontiguousarray(want).reshape(3, -1))
 
 
 def test_trained_network_passes_the_floor_and_beats_bilinear(env):
