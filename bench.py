@@ -1,6 +1,7 @@
 """Benchmark: plain GPU decode + develop of ProRes RAW frames on one MI355X.
 
-Workload (one step = one frame): the 5760x3240 frame kind of the
+Workload (one step = a batch of --frames-per-step frames, 16 by default):
+the 5760x3240 frame kind of the
 end-to-end stream (RGGB, bitstream v0 with the default curve, tile
 alignment 3, qscale 45, seeded scene of 8 gradients + 200 discs with
 Poisson-Gaussian noise) is decoded with the Apple transform, linearized in
@@ -34,8 +35,11 @@ sys.path.insert(0, ROOT)
 def parse_args():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=500)
-    ap.add_argument("--warmup", type=int, default=30)
+    ap.add_argument("--steps", type=int, default=32)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--frames-per-step", type=int, default=16,
+                    help="frames a step submits to the pipelined stream (a frame takes about "
+                         "0.3 ms, so a step of one frame would time the pipeline's fill)")
     ap.add_argument("--width", type=int, default=5760)
     ap.add_argument("--height", type=int, default=3240)
     ap.add_argument("--no-graph", action="store_true")
@@ -162,10 +166,12 @@ def device_packet(packet):
     return d
 
 
-def pipelined(packet, d_packet, hdr, info, planes, stage, idct, steps, warmup, time_kernels=True):
-    """The pipelined stream of frames (prr_pipeline, four buffer sets): one
-    step = one frame submitted; the timed region is bracketed by device
-    synchronization, so it holds exactly `steps` whole frames. Returns
+def pipelined(packet, d_packet, hdr, info, planes, stage, idct, frames, warmup,
+              time_kernels=True):
+    """The pipelined stream of frames (prr_pipeline, four buffer sets):
+    `warmup` untimed frames, then `frames` frames submitted back to back; the
+    timed region is bracketed by device synchronization, so it holds exactly
+    `frames` whole frames. Returns
     (seconds, mean decode ms, mean develop ms inside the stream)."""
     import torch
 
@@ -182,7 +188,7 @@ def pipelined(packet, d_packet, hdr, info, planes, stage, idct, steps, warmup, t
     pipe.sync()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(steps):
+    for _ in range(frames):
         submit()
     pipe.sync()
     torch.cuda.synchronize()
@@ -195,7 +201,7 @@ def pipelined(packet, d_packet, hdr, info, planes, stage, idct, steps, warmup, t
         # Kernel durations inside the stream, with HIP events on the streams
         # the kernels run on (each shares the chip with the other there).
         pipe.timing(True)
-        for _ in range(min(steps, 60)):
+        for _ in range(min(frames, 60)):
             submit()
         dec_ms, dev_ms, _ = pipe.timing(False)
     pipe.close()
@@ -284,9 +290,10 @@ def plain_leg(args, packet, workload):
         else:
             step()
 
-    serial_steps = max(args.steps // 2, 10)
+    frames, warm = args.steps * args.frames_per_step, args.warmup * args.frames_per_step
+    serial_steps = max(frames // 2, 10)
     with torch.cuda.stream(stream):
-        for _ in range(args.warmup):
+        for _ in range(warm):
             run_step()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -296,14 +303,14 @@ def plain_leg(args, packet, workload):
         serial_s = time.perf_counter() - t0
 
     elapsed, pipe_dec_ms, pipe_dev_ms = pipelined(packet, d_packet, hdr, info, planes,
-                                                  P.Stage.Rec2020, P.Idct.Apple, args.steps,
-                                                  args.warmup)
+                                                  P.Stage.Rec2020, P.Idct.Apple, frames, warm)
     table_bytes = 2 * info.nb_tiles
     coded_bits = 8 * (len(packet) - info.header_end - table_bytes)
     return {
         "npix": npix, "width": w, "height": h, "packet_bytes": len(packet),
         "nb_tiles": info.nb_tiles, "coded_width": cw, "coded_height": ch,
-        "value": npix * args.steps / elapsed / 1e9, "ms_per_step": 1e3 * elapsed / args.steps,
+        "value": npix * frames / elapsed / 1e9, "ms_per_step": 1e3 * elapsed / args.steps,
+        "frames": frames, "ms_per_frame": 1e3 * elapsed / frames,
         "serial_value": npix * serial_steps / serial_s / 1e9,
         "serial_ms": 1e3 * serial_s / serial_steps, "serial_graph": graph is not None,
         "decode_ms_alone": t_decode_ms, "develop_ms_alone": t_develop_ms,
@@ -370,10 +377,10 @@ def c1_leg(args):
     planes = [torch.empty(npix, dtype=torch.float32, device="cuda") for _ in range(3)]
     out = {"workload": "4096x2160 BGGR v1 a3 q45, seeded qmat + curve, 2% corrupt tiles; "
                        "pipelined decode+linearize+develop(Rec2020)", "packet_bytes": len(packet)}
-    steps = max(args.steps // 2, 10)
+    steps = max(args.steps * args.frames_per_step // 2, 10)
     for idct in (P.Idct.Apple, P.Idct.Ffmpeg):
         el, dec, dv = pipelined(packet, d_packet, hdr, info, planes, P.Stage.Rec2020, idct,
-                                steps, args.warmup)
+                                steps, args.warmup * args.frames_per_step)
         out[idct.name.lower()] = {"value": round(npix * steps / el / 1e9, 3),
                                   "ms_per_frame": round(1e3 * el / steps, 4),
                                   "decode_ms": round(dec, 4), "develop_ms": round(dv, 4)}
@@ -436,9 +443,11 @@ def main():
                                "frame i+1 beside develop of frame i",
                    "packet_bytes": r["packet_bytes"], "tiles": r["nb_tiles"],
                    "coded_bits_per_sensel": round(r["coded_bits"] / r["npix"], 3),
+                   "frames_per_step": args.frames_per_step, "frames_timed": r["frames"],
+                   "ms_per_frame": round(r["ms_per_frame"], 4),
                    "pipeline_depth": 4,
                    "serial_value": round(r["serial_value"], 3),
-                   "serial_ms_per_step": round(r["serial_ms"], 4),
+                   "serial_ms_per_frame": round(r["serial_ms"], 4),
                    "serial_graph": r["serial_graph"]},
         "roofline": roofline,
         "cpu_baseline": cpu,
