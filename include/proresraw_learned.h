@@ -77,9 +77,11 @@ int prr_net_train_step(prr_net *net, float lr, void *stream, prr_error *err);
 /* Tensors of the last step, for checking (synchronous copies to host):
  * PRR_NET_X0 / H (layer 0..6) / R (layer 1..6) / DOUT: bf16, batch*cells*
  * cells*32 values; OUT: fp32, same count; GRAD: fp32, PRR_NET_PARAMS;
- * LOSS: one fp32; STEP: one uint32. */
+ * LOSS: one fp32; STEP: one uint32. PRR_NET_FRAME_X0: the whole-frame input
+ * packed by the last prr_net_develop_device that ran the network, bf16,
+ * (height/2)*(width/2)*32 values. */
 enum { PRR_NET_X0 = 0, PRR_NET_H = 1, PRR_NET_R = 2, PRR_NET_OUT = 3, PRR_NET_DOUT = 4,
-       PRR_NET_GRAD = 5, PRR_NET_LOSS = 6, PRR_NET_STEP = 7 };
+       PRR_NET_GRAD = 5, PRR_NET_LOSS = 6, PRR_NET_STEP = 7, PRR_NET_FRAME_X0 = 8 };
 int prr_net_read(prr_net *net, int what, uint32_t layer, void *host, size_t bytes,
                  prr_error *err);
 

@@ -16,7 +16,11 @@ Network, on the mosaic packed at half resolution (NHWC, 32 channels):
           channel 4: the noise feature 128*sqrt(noise[0]); 5-8: the
           noise-level map 128*sigma(v) of the cell's four sensels (sigma
           the square root of develop's variance, demosaic.rs:420-425);
-          9-31: zero
+          9-12: the green pass's curvature ratios (horizontal, vertical)
+          at the red sensel, then at the blue one, 0 where its noise tests
+          leave them out (demosaic.rs:495-532); 13-16: 1 where each of the
+          four sensels lies above its noise deviation at the chroma offset,
+          else 0 (the test of demosaic.rs:576-587); 17-31: zero
   h0      = relu(conv0(x0) + b0)
   h_l     = h_{l-1} + relu(conv_l(h_{l-1}) + b_l),   l = 1..6
   out     = conv7(h6) + b7; channel 3*(2*dy+dx)+c is colour c of the
@@ -103,18 +107,74 @@ def preclip(v, clip_r, clip_b):
     return v
 
 
-def sigma128(v, noise, off):
-    """128 * sqrt(c + max(b, a * (off + v)^2)) in float32, in the order of
-    develop's variance (demosaic.rs:420-425)."""
+def deviation(v, noise, off):
+    """sqrt(c + max(b, a * (off + v)^2)) in float32: the noise deviation
+    develop compares values with (demosaic.rs:420-425, 512, 521, 579-581)."""
     f = np.float32
     a, b, c = (f(x) for x in noise)
     t = f(off) + np.asarray(v, np.float32)
-    return (f(128.0) * np.sqrt(c + np.maximum(b, a * (t * t)))).astype(np.float32)
+    return np.sqrt(c + np.maximum(b, a * (t * t))).astype(np.float32)
 
 
-def pack(v, feature, noise=None, off=0.0):
-    """RGGB plane (H, W) -> x0 (H/2, W/2, 32), bf16; with `noise` (a, b, c)
-    and `off` (black / range), the noise-level map in channels 5-8."""
+def mirror(d, pos, n, c):
+    """The green pass's edge rule (mirror, demosaic.rs:547-557) on the
+    neighbour arrays d[0..3] of sensels at positions `pos` of lines of n,
+    centres c: the crate's if / else-if chain as masks."""
+    m0 = pos == 0
+    m1 = ~m0 & (pos == 1)
+    ml = ~m0 & ~m1 & (pos == n - 1)
+    ml2 = ~m0 & ~m1 & ~ml & (pos == n - 2)
+    d0, d1, d2, d3 = d
+    return [np.where(m0, d1, d0), np.where(ml, d0, d1),
+            np.where(m0, d3, np.where(m1, c, d2)),
+            np.where(m0, d3, np.where(ml, d2, np.where(ml2, c, d3)))]
+
+
+def curvature_ratios(v, noise, gp_off, sy, sx, clip):
+    """The green pass's curvature ratios (green_row_impl,
+    demosaic.rs:495-532) at the sensels (sy + 2i, sx + 2j) of the clipped
+    RGGB plane v: (horizontal, vertical) arrays, t[2+dir] / (t[dir] * 2)
+    where the crate's noise tests apply it, else 0. Rows and columns are
+    clamped (demosaic.rs:469-472, 499), greens clipped at 1 and same-colour
+    sensels at `clip` (demosaic.rs:496, 502-505), then mirrored at the
+    picture's edges (demosaic.rs:506-507)."""
+    f = np.float32
+    hgt, wid = v.shape
+    ys = np.arange(sy, hgt, 2)[:, None]
+    xs = np.arange(sx, wid, 2)[None, :]
+
+    def at(dy, dx):
+        return v[np.clip(ys + dy, 0, hgt - 1), np.clip(xs + dx, 0, wid - 1)]
+
+    one, cl = f(1.0), f(clip)
+    c = np.minimum(at(0, 0), cl)
+    hh = mirror([np.minimum(at(0, -1), one), np.minimum(at(0, 1), one),
+                 np.minimum(at(0, -2), cl), np.minimum(at(0, 2), cl)], xs, wid, c)
+    vv = mirror([np.minimum(at(-1, 0), one), np.minimum(at(1, 0), one),
+                 np.minimum(at(-2, 0), cl), np.minimum(at(2, 0), cl)], ys, hgt, c)
+    gate = c > deviation(c, noise, gp_off)
+    sums2, sums3 = hh[2] + hh[3], vv[2] + vv[3]
+    t = [c * f(2.0) + sums2, c * f(2.0) + sums3, c * f(-2.0) + sums2, c * f(-2.0) + sums3]
+    out = []
+    for k, d in enumerate((hh, vv)):
+        sd = [deviation(x, noise, gp_off) for x in d]
+        p, q = d[0] - d[1], d[2] - d[3]
+        any_ = ((p * q > 0) | ((d[2] - c) * (d[3] - c) > 0) | (sd[0] + sd[1] > np.abs(p)) |
+                (sd[2] + sd[3] > np.abs(q)))
+        all_ = (d[0] > sd[0]) & (d[1] > sd[1]) & (d[2] > sd[2]) & (d[3] > sd[3])
+        with np.errstate(all="ignore"):
+            r = (t[2 + k] / (t[k] * f(2.0))).astype(np.float32)
+        out.append(np.where(gate & any_ & all_, r, f(0.0)).astype(np.float32))
+    return out
+
+
+def pack(v, feature, noise=None, off=0.0, gp_off=None, clip=(1.0, 1.0)):
+    """RGGB plane (H, W), red and blue already clipped at `clip` -> x0
+    (H/2, W/2, 32), bf16; with `noise` (a, b, c) and `off` (black / range),
+    the noise-level map in channels 5-8; with `gp_off` (black *
+    one_over_range, the green pass's offset) as well, the curvature ratios
+    and above-noise tests in channels 9-16. v is the whole picture: the
+    ratios read neighbours two sensels away."""
     h2, w2 = v.shape[0] // 2, v.shape[1] // 2
     x = np.zeros((h2, w2, C), np.float32)
     x[..., 0] = v[0:2 * h2:2, 0:2 * w2:2]
@@ -123,8 +183,16 @@ def pack(v, feature, noise=None, off=0.0):
     x[..., 3] = v[1:2 * h2:2, 1:2 * w2:2]
     x[..., 4] = np.float32(feature)
     if noise is not None:
+        dev = [deviation(x[..., k], noise, off) for k in range(4)]
         for k in range(4):
-            x[..., 5 + k] = sigma128(x[..., k], noise, off)
+            x[..., 5 + k] = np.float32(128.0) * dev[k]
+        if gp_off is not None:
+            rr = curvature_ratios(v, noise, gp_off, 0, 0, clip[0])
+            rb = curvature_ratios(v, noise, gp_off, 1, 1, clip[1])
+            for k, r in enumerate(rr + rb):
+                x[..., 9 + k] = r[:h2, :w2]
+            for k in range(4):
+                x[..., 13 + k] = (dev[k] < x[..., k]).astype(np.float32)
     return bf16(x)
 
 

@@ -47,6 +47,10 @@ struct NetFrame {
      * off = black / range; noise_a comes from the quantizer on the device
      * when qstatus is set. */
     float noise_a, noise_b, noise_c, noise_off;
+    /* The green pass's offset in the variance: black * one_over_range
+     * (green_row_impl, demosaic.rs:484), where the chroma pass uses
+     * black / range (noise_off). */
+    float gp_off;
     /* When set, the feature comes from the quantizer qmat0 * max_qscale
      * read on the device (develop(&frame, stage) after a device decode). */
     const prr_device_status *qstatus;

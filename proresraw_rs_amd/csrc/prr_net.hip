@@ -8,7 +8,12 @@
  *           noise feature 128*sqrt(noise[0]) in channel 4; the noise-level
  *           map 128*sigma(v) of the cell's four sensels in channels 5-8,
  *           sigma(v) the noise deviation develop's tests compare v with
- *           (demosaic.rs:420-425); zeros after
+ *           (demosaic.rs:420-425); in channels 9-12 the green pass's
+ *           curvature ratios (horizontal, vertical) at the red sensel, then
+ *           at the blue one (demosaic.rs:495-532: 0 where its noise tests
+ *           leave them out); in channels 13-16 whether each of the four
+ *           sensels lies above its noise deviation at the chroma offset (1
+ *           or 0, the test of demosaic.rs:576-587); zeros after
  *   layer 0 h0 = relu(conv(x0) + b)
  *   1..6    h = h + relu(conv(h) + b)
  *   7       out = conv(h) + b; channels 3*(2*dy+dx)+c are the colours of
@@ -548,29 +553,109 @@ __device__ inline float feature_of(const NetFrame &f)
     return 128.0f * sqrtf(noise_a_of(f));
 }
 
-/* 128 * the noise deviation of a normalized value: sqrt of develop's
- * variance (demosaic.rs:420-425), in its operation order. */
-__device__ inline float sigma128(float a, const NetFrame &f, float v)
+/* The noise deviation sqrt(variance(v)) at offset `off` (demosaic.rs:420-425,
+ * in its operation order). */
+__device__ inline float deviation(float a, const NetFrame &f, float off, float v)
 {
-    const float t = f.noise_off + v;
-    return 128.0f * sqrtf(f.noise_c + fmaxf(f.noise_b, a * (t * t)));
+    const float t = off + v;
+    return sqrtf(f.noise_c + fmaxf(f.noise_b, a * (t * t)));
 }
 
-/* One packed cell: 32 bf16 channels. */
+/* The green pass's edge rule (mirror, demosaic.rs:547-557) on the four
+ * neighbours d of a sensel at `pos` of a line of `len`, centre c. */
+__device__ inline void mirror_line(float *d, int pos, int len, float c)
+{
+    if (pos == 0) {
+        d[0] = d[1];
+        d[2] = d[3];
+    } else if (pos == 1) {
+        d[2] = c;
+    } else if (pos == len - 1) {
+        d[1] = d[0];
+        d[3] = d[2];
+    } else if (pos == len - 2) {
+        d[3] = c;
+    }
+}
+
+/* The green pass's curvature ratios at the red or blue sensel (y, x) of the
+ * RGGB-order mosaic (green_row_impl, demosaic.rs:495-532): horizontal and
+ * vertical, t[2 + dir] / (t[dir] * 2) where the crate's noise tests apply
+ * it, else 0. Neighbours are read at clamped rows and columns, greens
+ * clipped at 1 and same-colour sensels at `clip` (the row's), and mirrored
+ * at the picture's edges, as the crate reads them; flips to RGGB order
+ * only swap the two neighbours of each pair, which leaves every test and
+ * ratio unchanged. */
+__device__ inline float2 curvature_ratios(const NetFrame &f, float a, int y, int x, float clip)
+{
+    const int w = (int)f.width, h = (int)f.height;
+    auto at = [&](int yy, int xx) {
+        return norm_at(f, min(max(yy, 0), h - 1), min(max(xx, 0), w - 1));
+    };
+    const float off = f.gp_off;
+    const float c = fminf(at(y, x), clip);
+    float hh[4] = {fminf(at(y, x - 1), 1.0f), fminf(at(y, x + 1), 1.0f),
+                   fminf(at(y, x - 2), clip), fminf(at(y, x + 2), clip)};
+    float vv[4] = {fminf(at(y - 1, x), 1.0f), fminf(at(y + 1, x), 1.0f),
+                   fminf(at(y - 2, x), clip), fminf(at(y + 2, x), clip)};
+    mirror_line(hh, x, w, c);
+    mirror_line(vv, y, h, c);
+    float ratio[2] = {0.0f, 0.0f};
+    if (c > deviation(a, f, off, c)) {
+        const float sums2 = hh[2] + hh[3], sums3 = vv[2] + vv[3];
+        const float t[4] = {c * 2.0f + sums2, c * 2.0f + sums3, c * -2.0f + sums2,
+                            c * -2.0f + sums3};
+#pragma unroll
+        for (int dir = 0; dir < 2; dir++) {
+            const float *d = dir ? vv : hh;
+            float sd[4];
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+                sd[i] = deviation(a, f, off, d[i]);
+            const float p = d[0] - d[1], q = d[2] - d[3];
+            const bool any = p * q > 0.0f || (d[2] - c) * (d[3] - c) > 0.0f ||
+                             sd[0] + sd[1] > fabsf(p) || sd[2] + sd[3] > fabsf(q);
+            const bool all = d[0] > sd[0] && d[1] > sd[1] && d[2] > sd[2] && d[3] > sd[3];
+            if (any && all)
+                ratio[dir] = t[2 + dir] / (t[dir] * 2.0f);
+        }
+    }
+    return make_float2(ratio[0], ratio[1]);
+}
+
+/* One packed cell, 32 bf16 channels (the layout in this file's header),
+ * from the normalized values of its four sensels and the curvature ratios
+ * at its red and blue sensels. */
+__device__ inline void write_cell(const NetFrame &f, float a, float vr, float vg1, float vg2,
+                                  float vb, float2 rr, float2 rb, uint16_t *dst)
+{
+    const uint16_t r = f2bf(vr), g1 = f2bf(vg1), g2 = f2bf(vg2), b = f2bf(vb);
+    const float dr = deviation(a, f, f.noise_off, vr), dg1 = deviation(a, f, f.noise_off, vg1);
+    const float dg2 = deviation(a, f, f.noise_off, vg2), db = deviation(a, f, f.noise_off, vb);
+    const uint16_t sr = f2bf(128.0f * dr), sg1 = f2bf(128.0f * dg1);
+    const uint16_t sg2 = f2bf(128.0f * dg2), sb = f2bf(128.0f * db);
+    /* 1 (bf16 0x3F80) where a sensel lies above its noise deviation at the
+     * chroma offset, the test the chroma estimates make (demosaic.rs:576-587). */
+    auto above = [](float v, float dev) { return dev < v ? (uint16_t)0x3F80u : (uint16_t)0u; };
+    uint4 *d = (uint4 *)dst;
+    d[0] = make_uint4(pack2(r, g1), pack2(g2, b), pack2(f2bf(feature_of(f)), sr),
+                      pack2(sg1, sg2));
+    d[1] = make_uint4(pack2(sb, f2bf(rr.x)), pack2(f2bf(rr.y), f2bf(rb.x)),
+                      pack2(f2bf(rb.y), above(vr, dr)), pack2(above(vg1, dg1), above(vg2, dg2)));
+    d[2] = make_uint4(above(vb, db), 0, 0, 0);
+    d[3] = make_uint4(0, 0, 0, 0);
+}
+
+/* One packed cell of the RGGB-order mosaic at (y, x), every value read from
+ * the frame (the crops and check windows). */
 __device__ inline void pack_cell(const NetFrame &f, int y, int x, uint16_t *dst)
 {
     const float vr = norm_at(f, y, x), vg1 = norm_at(f, y, x + 1);
     const float vg2 = norm_at(f, y + 1, x), vb = norm_at(f, y + 1, x + 1);
-    const uint16_t r = f2bf(vr), g1 = f2bf(vg1), g2 = f2bf(vg2), b = f2bf(vb);
     const float a = noise_a_of(f);
-    const uint16_t sr = f2bf(sigma128(a, f, vr)), sg1 = f2bf(sigma128(a, f, vg1));
-    const uint16_t sg2 = f2bf(sigma128(a, f, vg2)), sb = f2bf(sigma128(a, f, vb));
-    uint4 *d = (uint4 *)dst;
-    d[0] = make_uint4(pack2(r, g1), pack2(g2, b), pack2(f2bf(feature_of(f)), sr),
-                      pack2(sg1, sg2));
-    d[1] = make_uint4(pack2(sb, 0), 0, 0, 0);
-    d[2] = make_uint4(0, 0, 0, 0);
-    d[3] = make_uint4(0, 0, 0, 0);
+    const float2 rr = curvature_ratios(f, a, y, x, f.clip_r);
+    const float2 rb = curvature_ratios(f, a, y + 1, x + 1, f.clip_b);
+    write_cell(f, a, vr, vg1, vg2, vb, rr, rb, dst);
 }
 
 __global__ void k_pack_crops(const NetFrame *__restrict__ frames, const Crop *__restrict__ crops,
@@ -588,17 +673,81 @@ __global__ void k_pack_crops(const NetFrame *__restrict__ frames, const Crop *__
     pack_cell(f, (int)cr.y0 + 2 * ci, (int)cr.x0 + 2 * cj, x0 + i * C);
 }
 
-__global__ void k_pack_frame(NetFrame f, uint16_t *__restrict__ x0,
-                             const uint32_t *__restrict__ skip)
+/* The whole frame, one 256-lane workgroup per tile of PT_W x PT_H cells:
+ * each sensel of the tile and of a 2-sensel halo is normalized once into
+ * LDS with its value as a green-pass neighbour (greens capped at 1) and that
+ * value's deviation, and each lane assembles one cell from them. The
+ * operations on each value are pack_cell's, so the channels are the same
+ * bits (pack_cell packs the crops and check windows). Halo positions
+ * outside the picture hold the clamped sensel; the edge rule replaces every
+ * such neighbour before a test reads it. */
+enum { PT_W = 32, PT_H = 8, PT_SW = 2 * PT_W + 4, PT_SH = 2 * PT_H + 4 };
+
+__global__ __launch_bounds__(256) void k_pack_frame(NetFrame f, uint16_t *__restrict__ x0,
+                                                    const uint32_t *__restrict__ skip)
 {
+    __shared__ float s_raw[PT_SH][PT_SW], s_lim[PT_SH][PT_SW], s_dev[PT_SH][PT_SW];
     if (skip && *skip)
         return;
-    const int w2 = (int)f.width / 2, h2 = (int)f.height / 2;
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (size_t)h2 * w2)
+    const int w = (int)f.width, h = (int)f.height, w2 = w / 2, h2 = h / 2;
+    const int cx0 = (int)blockIdx.x * PT_W, cy0 = (int)blockIdx.y * PT_H;
+    const int sx0 = 2 * cx0 - 2, sy0 = 2 * cy0 - 2;
+    const float a = noise_a_of(f);
+    for (int i = (int)threadIdx.x; i < PT_SH * PT_SW; i += 256) {
+        const int ly = i / PT_SW, lx = i - ly * PT_SW;
+        const int y = min(max(sy0 + ly, 0), h - 1), x = min(max(sx0 + lx, 0), w - 1);
+        const float v = norm_at(f, y, x);
+        /* Red and blue values are clipped by norm_at (at their row's clip,
+         * the green pass's limit for same-colour neighbours); greens are
+         * capped at 1 as neighbours. */
+        const float lim = ((y ^ x) & 1) ? fminf(v, 1.0f) : v;
+        s_raw[ly][lx] = v;
+        s_lim[ly][lx] = lim;
+        s_dev[ly][lx] = deviation(a, f, f.gp_off, lim);
+    }
+    __syncthreads();
+    const int lj = (int)threadIdx.x % PT_W, li = (int)threadIdx.x / PT_W;
+    const int ci = cy0 + li, cj = cx0 + lj;
+    if (ci >= h2 || cj >= w2)
         return;
-    const int ci = (int)(i / w2), cj = (int)(i - (size_t)ci * w2);
-    pack_cell(f, 2 * ci, 2 * cj, x0 + i * C);
+    const int ry = 2 + 2 * li, rx = 2 + 2 * lj; /* the red sensel in LDS */
+    /* curvature_ratios from the LDS values: neighbours and their
+     * deviations mirrored alike (a replacement's deviation is that of the
+     * value it copies; the centre's is sc). */
+    auto ratios = [&](int ly, int lx, int y, int x) {
+        const float c = s_lim[ly][lx], sc = s_dev[ly][lx];
+        float hh[4] = {s_lim[ly][lx - 1], s_lim[ly][lx + 1], s_lim[ly][lx - 2], s_lim[ly][lx + 2]};
+        float vv[4] = {s_lim[ly - 1][lx], s_lim[ly + 1][lx], s_lim[ly - 2][lx], s_lim[ly + 2][lx]};
+        float sh[4] = {s_dev[ly][lx - 1], s_dev[ly][lx + 1], s_dev[ly][lx - 2], s_dev[ly][lx + 2]};
+        float sv[4] = {s_dev[ly - 1][lx], s_dev[ly + 1][lx], s_dev[ly - 2][lx], s_dev[ly + 2][lx]};
+        mirror_line(hh, x, w, c);
+        mirror_line(sh, x, w, sc);
+        mirror_line(vv, y, h, c);
+        mirror_line(sv, y, h, sc);
+        float ratio[2] = {0.0f, 0.0f};
+        if (c > sc) {
+            const float sums2 = hh[2] + hh[3], sums3 = vv[2] + vv[3];
+            const float t[4] = {c * 2.0f + sums2, c * 2.0f + sums3, c * -2.0f + sums2,
+                                c * -2.0f + sums3};
+#pragma unroll
+            for (int dir = 0; dir < 2; dir++) {
+                const float *d = dir ? vv : hh;
+                const float *sd = dir ? sv : sh;
+                const float p = d[0] - d[1], q = d[2] - d[3];
+                const bool any = p * q > 0.0f || (d[2] - c) * (d[3] - c) > 0.0f ||
+                                 sd[0] + sd[1] > fabsf(p) || sd[2] + sd[3] > fabsf(q);
+                const bool all = d[0] > sd[0] && d[1] > sd[1] && d[2] > sd[2] && d[3] > sd[3];
+                if (any && all)
+                    ratio[dir] = t[2 + dir] / (t[dir] * 2.0f);
+            }
+        }
+        return make_float2(ratio[0], ratio[1]);
+    };
+    const float vr = s_raw[ry][rx], vg1 = s_raw[ry][rx + 1];
+    const float vg2 = s_raw[ry + 1][rx], vb = s_raw[ry + 1][rx + 1];
+    const float2 rr = ratios(ry, rx, 2 * ci, 2 * cj);
+    const float2 rb = ratios(ry + 1, rx + 1, 2 * ci + 1, 2 * cj + 1);
+    write_cell(f, a, vr, vg1, vg2, vb, rr, rb, x0 + ((size_t)ci * w2 + cj) * C);
 }
 
 /* The window of S_y x S_x cells (at most WIN x WIN) around sampled 64x64
@@ -1037,9 +1186,8 @@ hipError_t launch_loss(const NetFrame *frames, const Crop *crops, const uint32_t
 
 hipError_t launch_pack_frame(const NetFrame &f, uint16_t *x0, const uint32_t *skip, hipStream_t s)
 {
-    const size_t cells = (size_t)(f.width / 2) * (f.height / 2);
-    hipLaunchKernelGGL(k_pack_frame, dim3((uint32_t)((cells + 255) / 256)), dim3(256), 0, s, f,
-                       x0, skip);
+    const dim3 grid((f.width / 2 + PT_W - 1) / PT_W, (f.height / 2 + PT_H - 1) / PT_H);
+    hipLaunchKernelGGL(k_pack_frame, grid, dim3(256), 0, s, f, x0, skip);
     return hipGetLastError();
 }
 

@@ -66,6 +66,7 @@ struct prr_net {
      * so that later calls (and graph capture) copy nothing. */
     std::map<std::tuple<uint32_t, uint32_t, uint32_t>, Buf> tiles;
     uint32_t last_k = 0;
+    size_t last_cells = 0;  /* cells of the last whole-frame packing */
     uint32_t conv_grid = 0; /* persistent inference convolution: workgroups */
 };
 
@@ -220,6 +221,7 @@ static NetFrame net_frame(const prr_header *hdr, const DevelopDesc &d,
     f.noise_b = p.noise[1];
     f.noise_c = p.noise[2];
     f.noise_off = p.black_over_range;
+    f.gp_off = (float)d.black * d.one_over_range;
     f.qstatus = qstatus;
     f.qmat0 = hdr->quant_matrix[0];
     f.flip_y = (uint32_t)(d.cfa_phase >> 1) & 1u;
@@ -397,6 +399,9 @@ extern "C" int prr_net_read(prr_net *net, int what, uint32_t layer, void *host, 
     case PRR_NET_GRAD: src = net->grads.p; need = sizeof(float) * PARAMS; break;
     case PRR_NET_LOSS: src = net->loss.p; need = 4; break;
     case PRR_NET_STEP: src = net->step.p; need = 4; break;
+    case PRR_NET_FRAME_X0: src = net->last_cells ? net->ix0.p : nullptr;
+        need = 2 * net->last_cells * C;
+        break;
     default: return net_fail(err, PRR_E_INVALID_ARGUMENT, "what", (uint64_t)what);
     }
     if (!src)
@@ -541,6 +546,7 @@ extern "C" int prr_net_develop_device(prr_net *net, const prr_header *hdr,
     d.gate = flag;
     NET_TRY(launch_develop(d, d_r, d_g, d_b, s), "develop kernel");
     net->last_k = k;
+    net->last_cells = cells;
     return PRR_OK;
 }
 

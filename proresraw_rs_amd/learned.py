@@ -30,7 +30,11 @@ C = 32
 HERE = os.path.dirname(os.path.abspath(__file__))
 WEIGHTS = os.path.join(HERE, "weights", "demosaic_net.npz")
 
-X0, H, R, OUT, DOUT, GRAD, LOSS, STEP = range(8)
+X0, H, R, OUT, DOUT, GRAD, LOSS, STEP, FRAME_X0 = range(9)
+# Input channels the packing fills (prr_net.hip's layout): the mosaic (4),
+# the noise feature (1), the noise-level map (4), the green pass's
+# curvature ratios (4) and the above-noise tests (4).
+INPUT_CHANNELS = 17
 
 
 def _torch():
@@ -48,13 +52,13 @@ def noise_feature(quantizer: int) -> float:
 def init_params(seed: int = 0) -> np.ndarray:
     """He-normal weights scaled down on the residual and output layers (the
     network starts close to the bilinear base); zero biases. Input channels
-    past the 9 used ones (mosaic, noise feature, noise-level map) and output
-    channels past the 12 used ones stay zero (their gradients are zero, so
-    training keeps them zero)."""
+    past the 17 used ones (mosaic, noise feature, noise-level map, curvature
+    ratios, above-noise tests) and output channels past the 12 used ones
+    stay zero (their gradients are zero, so training keeps them zero)."""
     rng = np.random.default_rng(seed)
     p = np.zeros(PARAMS, np.float32)
     for l in range(LAYERS):
-        cin = 9 if l == 0 else C
+        cin = INPUT_CHANNELS if l == 0 else C
         cout = 12 if l == LAYERS - 1 else C
         scale = 1.0 if l == 0 else (0.25 if l < LAYERS - 1 else 0.1)
         w = np.zeros((9, C, C), np.float32)
@@ -64,13 +68,21 @@ def init_params(seed: int = 0) -> np.ndarray:
 
 
 def load_weights(path: str = WEIGHTS):
-    """(params, feature_lo, feature_hi) saved by save_weights."""
+    """(params, feature_lo, feature_hi) saved by save_weights. A file records
+    the input channels its network reads (9 when it predates the curvature
+    ratios and above-noise tests: its weights for those are zero); one that
+    reads channels the packing does not fill is refused."""
     with np.load(path, allow_pickle=False) as z:
+        n_in = int(z["input_channels"]) if "input_channels" in z.files else 9
+        if n_in > INPUT_CHANNELS:
+            raise ValueError(f"{path}: the network reads {n_in} input channels, the packing "
+                             f"fills {INPUT_CHANNELS}")
         return z["params"].astype(np.float32), float(z["feature_lo"]), float(z["feature_hi"])
 
 
 def save_weights(path: str, params, lo: float, hi: float, **info):
     os.makedirs(os.path.dirname(path), exist_ok=True)
+    info.setdefault("input_channels", INPUT_CHANNELS)
     np.savez(path, params=np.asarray(params, np.float32), feature_lo=np.float32(lo),
              feature_hi=np.float32(hi), **{k: np.asarray(v) for k, v in info.items()})
 
@@ -227,6 +239,14 @@ class Net:
         used, psnr = ctypes.c_uint32(), ctypes.c_float()
         self._call("prr_net_last_check", ctypes.byref(used), ctypes.byref(psnr))
         return bool(used.value), float(psnr.value)
+
+    def frame_input(self, header: FrameHeader) -> np.ndarray:
+        """The whole-frame input the last develop packed (it must have run
+        the network): (height/2, width/2, 32) float32 of the bf16 values."""
+        h2, w2 = header.height // 2, header.width // 2
+        out = np.zeros(h2 * w2 * C, np.uint16)
+        self._call("prr_net_read", FRAME_X0, 0, out.ctypes.data, out.nbytes)
+        return (out.astype(np.uint32) << 16).view(np.float32).reshape(h2, w2, C)
 
 
 def random_crops(rng, frames: List[TrainFrame], steps: int, batch: int, cells: int,

@@ -28,16 +28,31 @@ def env(gpu):
     return torch, L, frames
 
 
-def crop_host(frame, y0, x0, cells):
-    """Normalized mosaic, target and bilinear base of one crop (numpy)."""
+def frame_host(frame):
+    """The whole normalized mosaic of a training frame, red and blue clipped
+    at the training stage's clip (1)."""
     h = frame.header
     samples = frame.samples.cpu().numpy().view(np.uint16).reshape(h.coded_height(), h.coded_width())
     p = frame.params
     norm = NO.normalized(samples[:h.height, :h.width], list(h.curve), p.black, p.one_over_range)
-    win = norm[y0:y0 + 2 * cells, x0:x0 + 2 * cells]
-    win = NO.preclip(win, 1.0, 1.0)
+    return NO.preclip(norm, 1.0, 1.0)
+
+
+def crop_host(frame, y0, x0, cells):
+    """Normalized mosaic, target and bilinear base of one crop (numpy)."""
+    win = frame_host(frame)[y0:y0 + 2 * cells, x0:x0 + 2 * cells]
     target = frame.target.cpu().numpy()[:, y0:y0 + 2 * cells, x0:x0 + 2 * cells]
     return win, target, NO.bilinear(win)
+
+
+def pack_host(L, frame, y0, x0, cells):
+    """The oracle's packing of one crop: the whole frame packed (the
+    curvature ratios read neighbours outside the crop), then cut."""
+    p = frame.params
+    gp_off = np.float32(p.black) * np.float32(p.one_over_range)
+    x = NO.pack(frame_host(frame), L.noise_feature(frame.quantizer), p.noise,
+                p.black_over_range, gp_off=gp_off)
+    return x[y0 // 2:y0 // 2 + cells, x0 // 2:x0 // 2 + cells]
 
 
 def run_step(L, net, frames, params, crops, lr=0.0):
@@ -71,9 +86,10 @@ def test_forward_layers_loss_and_gradients_match_oracle(env):
     wins, targets, bases = zip(*[crop_host(frames[c[0]], c[1], c[2], CELLS) for c in crops[0]])
 
     # Packing is exact.
-    x0 = np.stack([NO.pack(w, L.noise_feature(frames[c[0]].quantizer), frames[c[0]].params.noise,
-                           frames[c[0]].params.black_over_range) for w, c in zip(wins, crops[0])])
+    x0 = np.stack([pack_host(L, frames[c[0]], c[1], c[2], CELLS) for c in crops[0]])
     assert np.abs(x0[..., 5:9]).min() > 0  # the noise-level map is there
+    assert (x0[..., 9:13] > 0).any() and (x0[..., 9:13] < 0).any()  # curvature ratios
+    assert (x0[..., 13:17] == 1).any() and (x0[..., 13:17] == 0).any()  # above-noise tests
     assert np.array_equal(t[L.X0], x0)
 
     # Each layer against the oracle on the GPU's own input.
@@ -306,6 +322,42 @@ def test_zero_network_is_the_bilinear_base_on_every_pattern(env, pattern):
     base = NO.bilinear(NO.preclip(np.ascontiguousarray(norm[flip]), 1.0, 1.0))
     want = np.minimum(base, np.float32(1.0))[(slice(None),) + flip]
     assert np.array_equal(got, np.ascontiguousarray(want).reshape(3, -1))
+
+
+@pytest.mark.parametrize("pattern", [0, 1, 2, 3])
+def test_frame_packing_matches_the_oracle_on_every_pattern(env, pattern):
+    """The whole-frame packing (the tiled kernel that shares each sensel's
+    work through LDS) equals the oracle's packing of the frame flipped to
+    RGGB order, every channel bit for bit: a size that leaves partial tiles,
+    a white-balanced stage (red and blue clipped below 1) and the noise
+    model of the frame's quantizer."""
+    torch, L, _ = env
+    from proresraw_rs_amd import synth as S
+    from proresraw_rs_amd.api import DemosaicParams, Stage
+    sc = S.scene(150, 98, pattern, seed=700 + pattern, version=pattern & 1, alignment=2,
+                 qscale=(20 + 8 * pattern, 20 + 8 * pattern), white_level=4095)
+    f = L.frame_pair(S.encode_scene(sc))
+    h = f.header
+    prm = DemosaicParams.with_white_balance(h, f.quantizer, 2.0, 1.6)
+    net = L.Net(np.zeros(NO.PARAMS, np.float32))
+    net.set_range(0.0, 10.0)
+    n = h.width * h.height
+    planes = [torch.zeros(n, dtype=torch.float32, device="cuda") for _ in range(3)]
+    net.develop(h, f.samples, prm, Stage.WhiteBalanced, planes, check_tiles=0)
+    torch.cuda.synchronize()
+    got = net.frame_input(h)
+    samples = f.samples.cpu().numpy().view(np.uint16).reshape(h.coded_height(), h.coded_width())
+    norm = NO.normalized(samples[:h.height, :h.width], list(h.curve), prm.black,
+                         prm.one_over_range)
+    red_row, red_col = int(h.bayer_pattern) >> 1, int(h.bayer_pattern) & 1
+    v = NO.preclip(NO.canonical(norm, red_row, red_col), *prm.clip)
+    gp_off = np.float32(prm.black) * np.float32(prm.one_over_range)
+    want = NO.pack(v, L.noise_feature(f.quantizer), prm.noise, prm.black_over_range,
+                   gp_off=gp_off, clip=prm.clip)
+    assert min(prm.clip) < 1.0
+    assert (want[..., 9:13] != 0).any() and (want[..., 13:17] == 1).any()
+    bad = np.argwhere(got.view(np.uint32) != want.view(np.uint32))
+    assert bad.size == 0, f"{len(bad)} values differ, first at {bad[0]}"
 
 
 def test_trained_network_passes_the_floor_and_beats_bilinear(env):

@@ -198,3 +198,75 @@ def test_pack_and_normalize_follow_develop_units():
     assert float(x[1, 0, 0]) == float(NO.bf16(mosaic[2:3, 0])[0])
     assert float(x[0, 1, 3]) == float(NO.bf16(mosaic[1:2, 3])[0])
     assert float(x[0, 0, 4]) == 0.5 and not x[..., 5:].any()
+
+
+def _wgsl_params(q, clip, black=256, rng_=3839):
+    """The develop uniforms of a 12-bit RGGB frame at quantizer q
+    (DemosaicParams::with_white_balance's noise model, demosaic.rs:86-101)."""
+    from tests import wgsl_develop as W
+    F = np.float32
+    base = q * q * 5.0 + 64.0 / 3.0
+    noise = (F(base / 150994944.0), F(base / (rng_ * rng_ * 256.0)), F(64.0 / 3.0 / rng_ ** 2))
+    return W.Params(0, black, F(1.0 / rng_), noise, clip, F(black / rng_), (1.0, 1.0))
+
+
+def test_curvature_ratios_equal_the_wgsl_restatements():
+    """NO.curvature_ratios (the network's input channels 9-12) equals, bit
+    for bit, the curvature ratios of the second restatement of the crate's
+    green pass (tests/wgsl_develop.py, written from develop.wgsl and itself
+    equal bit for bit to the oracle's develop) at every red and blue sensel
+    of random mosaics and constructed fields, with the ratio non-zero in
+    both directions and signs, zeroed by the noise tests and by the centre's
+    gate, and every edge rule taken."""
+    from tests import wgsl_develop as W
+    from tests.test_develop_restatement import smooth_field
+    F = np.float32
+    rng = np.random.default_rng(7)
+    cases = []
+    for i in range(24):
+        w, h = 2 * int(rng.integers(3, 24)), 2 * int(rng.integers(3, 20))
+        lo, hi = [(0, 4096), (200, 3000), (0, 600), (250, 4095)][i % 4]
+        cases.append((rng.integers(lo, hi, size=(h, w)), [15, 45, 135][i % 3],
+                      (1.0, 1.0) if i % 2 else (F(0.8), F(0.6))))
+    for j, kind in enumerate(("bump", "hue_step", "ramp_cross")):
+        for w, h in ((32, 24), (24, 32)):
+            cases.append((smooth_field(w, h, kind, j), 15, (1.0, 1.0)))
+    # Every picture has even sides, so red sensels sit at positions 0 and
+    # len - 2 of their lines and blue ones at 1 and len - 1: each case takes
+    # all four edge rules in both directions.
+    seen = dict.fromkeys(["h+", "h-", "v+", "v-", "h0_noise", "v0_noise", "gate_closed"], 0)
+    for lin, q, clip in cases:
+        p = _wgsl_params(q, clip)
+        lin = np.asarray(lin, np.int64)
+        norm = (F(p.one_over_range) * (lin - p.black).astype(np.float32)).astype(np.float32)
+        cov, want = W.Coverage(), {}
+        W.green(norm, p, cov, ratios=want)
+        v = NO.preclip(norm, *clip)
+        off = F(p.black) * F(p.one_over_range)
+        for sy, sx, cl in ((0, 0, clip[0]), (1, 1, clip[1])):
+            got = NO.curvature_ratios(v, p.noise, off, sy, sx, cl)
+            for k, name in enumerate("hv"):
+                ref = want[name][sy::2, sx::2]
+                assert np.array_equal(got[k].view(np.uint32), ref.view(np.uint32)), (name, sy)
+                seen[name + "+"] += int((got[k] > 0).sum())
+                seen[name + "-"] += int((got[k] < 0).sum())
+        for k in ("h", "v"):
+            seen[k + "0_noise"] += cov.counts.get(f"ratio_{k}_zero_by_noise", 0)
+        seen["gate_closed"] += cov.counts.get("green_gate_closed", 0)
+    print("ratio branches:", seen)
+    assert all(n > 0 for n in seen.values()), seen
+
+
+def test_above_noise_channels_follow_the_chroma_test():
+    """Channels 13-16 are 1 exactly where deviation(v) < v (the chroma
+    estimate's test, demosaic.rs:576-587): with a = 0 the deviation is the
+    constant sqrt(b + c), so the bit turns on strictly above it."""
+    F = np.float32
+    noise = (F(0.0), F(1e-4), F(2.25e-4))   # deviation sqrt(3.25e-4)
+    s = np.sqrt(F(3.25e-4))
+    vals = [np.nextafter(s, F(0)), s, np.nextafter(s, F(1)), F(0.5)]
+    v = np.array([[vals[0], vals[1]], [vals[2], vals[3]]], np.float32)
+    v = np.tile(v, (3, 3))
+    x = NO.pack(v, 0.0, noise, 0.0, gp_off=F(0.0))
+    assert x[0, 0, 13:17].tolist() == [0.0, 0.0, 1.0, 1.0]
+    assert np.all(x[..., 5:9] == NO.bf16(np.full((3, 3, 4), F(128.0) * s)))

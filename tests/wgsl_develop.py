@@ -116,11 +116,13 @@ def mirror(d, pos, length, c):
     return [n0, n1, n2, n3]
 
 
-def green(norm, p, cov, plant=None):
+def green(norm, p, cov, plant=None, ratios=None):
     """Pass 1 (develop.wgsl:145-200) over a height x width picture of
     normalized values: (red or blue, green) per sensel. plant=
     "ratio_denominator" drops the factor 2 of the ratio's denominator (a
-    mistake for tests to catch)."""
+    mistake for tests to catch). A dict passed as `ratios` receives the
+    horizontal and vertical curvature ratios ("h", "v"; meaningful at the
+    red and blue sensels)."""
     h, w = norm.shape
     ys, xs = np.mgrid[0:h, 0:w]
     red_row, red_col = p.cfa >> 1, p.cfa & 1
@@ -164,6 +166,8 @@ def green(norm, p, cov, plant=None):
         cov.add(f"ratio_{name}_neg", site & take & (r < 0))
         cov.add(f"ratio_{name}_zero_by_noise", site & gate & any_ & ~above)
         ratio.append(r)
+        if ratios is not None:
+            ratios[name] = r
     cov.add("green_gate_closed", site & ~gate)
     half, one = F(0.5), F(1.0)
     gh = (sums[0] * half) * (one - ratio[0])
