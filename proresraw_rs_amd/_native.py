@@ -216,6 +216,7 @@ SIGNATURES = {
     "prr_net_develop_device": (i32, [c_vp, H, c_vp, DP, c_vp, i32, u32, ctypes.c_float, c_vp,
                                      c_vp, c_vp, c_vp, E]),
     "prr_net_last_check": (i32, [c_vp, ctypes.POINTER(u32), ctypes.POINTER(ctypes.c_float), E]),
+    "prr_net_set_pairs": (i32, [c_vp, i32, E]),
     "prr_is_prores_raw_fourcc": (i32, [c_vp]),
     "prr_mov_read_video_track": (i32, [i32, MOV_ACCEPT, c_vp, ctypes.POINTER(c_vp),
                                        ctypes.c_char_p, sz]),

@@ -40,14 +40,35 @@ typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4;
 
 __device__ inline float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
 
-/* fp32 -> bf16, round to nearest even, NaN kept quiet. */
-__device__ inline uint16_t f2bf(float f)
+typedef __bf16 bf16x2v __attribute__((ext_vector_type(2)));
+typedef float f32x2v __attribute__((ext_vector_type(2)));
+
+/* fp32 -> bf16, round to nearest even: the compiler's conversion, one
+ * v_cvt_pk_bf16_f32 for two values. On every non-NaN float it equals adding
+ * 0x7FFF plus the lowest kept bit and truncating (checked over all 2^32 bit
+ * patterns on gfx950, denormals and infinities included); NaN stays NaN. */
+__device__ inline uint16_t f2bf(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
+
+/* Two values, the first in the low half. */
+__device__ inline uint32_t f2bf2(float a, float b)
 {
-    uint32_t u = __float_as_uint(f);
-    if ((u & 0x7F800000u) == 0x7F800000u && (u & 0x7FFFFFu))
-        return (uint16_t)((u >> 16) | 0x40u);
-    u += 0x7FFFu + ((u >> 16) & 1u);
-    return (uint16_t)(u >> 16);
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2v){a, b}, bf16x2v));
+}
+
+/* The bf16 in the low / high half of a packed pair, as fp32. */
+__device__ inline float bf_lo(uint32_t p) { return __uint_as_float(p << 16); }
+__device__ inline float bf_hi(uint32_t p) { return __uint_as_float(p & 0xFFFF0000u); }
+
+/* Four channels of a residual layer's output, bf16(resid + bf16(relu(v))),
+ * packed (resid: the input's four bf16 channels); *r gets bf16(relu(v)). */
+__device__ inline uint2 resid_relu4(uint2 resid, const float *v, uint2 *r = nullptr)
+{
+    const uint32_t q0 = f2bf2(fmaxf(v[0], 0.0f), fmaxf(v[1], 0.0f));
+    const uint32_t q1 = f2bf2(fmaxf(v[2], 0.0f), fmaxf(v[3], 0.0f));
+    if (r)
+        *r = make_uint2(q0, q1);
+    return make_uint2(f2bf2(bf_lo(resid.x) + bf_lo(q0), bf_hi(resid.x) + bf_hi(q0)),
+                      f2bf2(bf_lo(resid.y) + bf_lo(q1), bf_hi(resid.y) + bf_hi(q1)));
 }
 
 __device__ inline uint32_t pack2(uint16_t a, uint16_t b) { return (uint32_t)a | ((uint32_t)b << 16); }
@@ -160,34 +181,24 @@ __global__ __launch_bounds__(256) void k_conv(const uint16_t *__restrict__ in, i
                 *(float4 *)(out_f32 + px * C + co) = make_float4(v[0], v[1], v[2], v[3]);
                 continue;
             }
-            uint16_t o[4];
+            uint2 o;
             if (EPI == EPI_RELU) {
-#pragma unroll
-                for (int i = 0; i < 4; i++)
-                    o[i] = f2bf(fmaxf(v[i], 0.0f));
-            } else if (EPI == EPI_RESID_RELU || EPI == EPI_ADD) {
+                o = make_uint2(f2bf2(fmaxf(v[0], 0.0f), fmaxf(v[1], 0.0f)),
+                               f2bf2(fmaxf(v[2], 0.0f), fmaxf(v[3], 0.0f)));
+            } else if (EPI == EPI_RESID_RELU) {
                 const uint2 rv = *(const uint2 *)(resid + px * C + co);
-                const uint16_t rs[4] = {(uint16_t)rv.x, (uint16_t)(rv.x >> 16), (uint16_t)rv.y,
-                                        (uint16_t)(rv.y >> 16)};
-                uint16_t rr[4];
-#pragma unroll
-                for (int i = 0; i < 4; i++) {
-                    if (EPI == EPI_RESID_RELU) {
-                        rr[i] = f2bf(fmaxf(v[i], 0.0f));
-                        o[i] = f2bf(bf2f(rs[i]) + bf2f(rr[i]));
-                    } else {
-                        o[i] = f2bf(bf2f(rs[i]) + v[i]);
-                    }
-                }
-                if (EPI == EPI_RESID_RELU && aux)
-                    *(uint2 *)(aux + px * C + co) =
-                        make_uint2(pack2(rr[0], rr[1]), pack2(rr[2], rr[3]));
+                uint2 rr;
+                o = resid_relu4(rv, v, &rr);
+                if (aux)
+                    *(uint2 *)(aux + px * C + co) = rr;
+            } else if (EPI == EPI_ADD) {
+                const uint2 rv = *(const uint2 *)(resid + px * C + co);
+                o = make_uint2(f2bf2(bf_lo(rv.x) + v[0], bf_hi(rv.x) + v[1]),
+                               f2bf2(bf_lo(rv.y) + v[2], bf_hi(rv.y) + v[3]));
             } else { /* EPI_LINEAR */
-#pragma unroll
-                for (int i = 0; i < 4; i++)
-                    o[i] = f2bf(v[i]);
+                o = make_uint2(f2bf2(v[0], v[1]), f2bf2(v[2], v[3]));
             }
-            *(uint2 *)(out + px * C + co) = make_uint2(pack2(o[0], o[1]), pack2(o[2], o[3]));
+            *(uint2 *)(out + px * C + co) = o;
         }
     }
 }
@@ -319,20 +330,193 @@ __global__ __launch_bounds__(256) void k_conv_infer(const uint16_t *__restrict__
                         *(float4 *)(out12 + px * OUT_CH + co) = make_float4(v[0], v[1], v[2], v[3]);
                     continue;
                 }
-                uint16_t o[4];
-                if (MODE == IM_HEAD) {
+                uint2 o;
+                if (MODE == IM_HEAD)
+                    o = make_uint2(f2bf2(fmaxf(v[0], 0.0f), fmaxf(v[1], 0.0f)),
+                                   f2bf2(fmaxf(v[2], 0.0f), fmaxf(v[3], 0.0f)));
+                else
+                    o = resid_relu4(*(const uint2 *)&s_in[wave + 1][16 * nt + r16 + 1][co], v);
+                *(uint2 *)(out + px * C + co) = o;
+            }
+        }
+    }
+}
+
+/* Two residual layers in one pass (inference): h2 = h1 + relu(conv_B(h1) +
+ * b_B) with h1 = h0 + relu(conv_A(h0) + b_A), h1 kept in LDS. A workgroup
+ * takes a strip of PS_W output columns and PS_SEG rows and walks it row by
+ * row: rings of RING rows hold h0 (columns x0 - 2 .. x0 + 65 of the strip)
+ * and h1 (the same columns); at step t waves 0 and 1 compute h1 row t (64
+ * columns, x0 - 1 .. x0 + 62, two 16-pixel tiles each) from h0 rows t - 1 ..
+ * t + 1, while waves 2 and 3 compute h2 row t - 2 (x0 .. x0 + 63, of which
+ * the first PS_W are kept) from h1 rows t - 3 .. t - 1, so each wave holds
+ * one layer's weights in registers, and h0 row t + 2 is loaded into
+ * registers meanwhile. h1 outside the picture is zero (the next layer's
+ * padding). Every value is computed with k_conv_infer's operations in its
+ * order (the same MFMAs, taps in order, the same epilogue), so h2 equals
+ * two k_conv_infer<IM_RES> launches bit for bit, with half their HBM
+ * traffic and 3 % more matrix work (64 columns computed for 62). */
+enum { PS_W = 62, PS_COLS = 68, PS_SEG = 64, RING = 4, PS_SLOTS = 66 * 4 };
+
+__global__ __launch_bounds__(256) void k_conv_pair(const uint16_t *__restrict__ in, int H, int W,
+                                                   const uint16_t *__restrict__ wA,
+                                                   const float *__restrict__ bA,
+                                                   const uint16_t *__restrict__ wB,
+                                                   const float *__restrict__ bB,
+                                                   uint16_t *__restrict__ out,
+                                                   const uint32_t *__restrict__ skip)
+{
+    __shared__ __attribute__((aligned(16))) uint16_t s_x[RING][PS_COLS][CP];
+    __shared__ __attribute__((aligned(16))) uint16_t s_h[RING][PS_COLS][CP];
+    if (skip && *skip)
+        return;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int r16 = lane & 15, kq = (lane >> 4) * 8;
+    const int strips = (W + PS_W - 1) / PS_W;
+    const int strip = (int)blockIdx.x % strips, seg = (int)blockIdx.x / strips;
+    const int x0 = strip * PS_W, y0 = seg * PS_SEG, y1 = min(y0 + PS_SEG, H);
+    const bool second = wave >= 2;  /* layer B */
+    const int nt0 = 2 * (wave & 1); /* this wave's first 16-pixel tile */
+
+    const uint16_t *wgt = second ? wB : wA;
+    const float *bias = second ? bB : bA;
+    bf16x8 wa[TAPS][2];
+#pragma unroll
+    for (int t = 0; t < TAPS; t++)
+#pragma unroll
+        for (int mt = 0; mt < 2; mt++)
+            wa[t][mt] = *(const bf16x8 *)(wgt + ((size_t)(t * C + 16 * mt + r16) * C + kq));
+    float bv[2][4];
+#pragma unroll
+    for (int mt = 0; mt < 2; mt++)
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+            bv[mt][i] = bias[16 * mt + (lane >> 4) * 4 + i];
+
+    for (int i = tid; i < RING * PS_COLS * CP / 8; i += 256) {
+        ((uint4 *)&s_x[0][0][0])[i] = make_uint4(0, 0, 0, 0);
+        ((uint4 *)&s_h[0][0][0])[i] = make_uint4(0, 0, 0, 0);
+    }
+    /* h0 row y, columns x0 - 2 .. x0 + 63 (zero outside the picture), into
+     * registers: 66 pixels x 4 16-byte quarters, two a lane. Rows are
+     * loaded two steps before they are stored to the ring (`next` holds the
+     * row of the coming step, `later` the one after), so a load has two
+     * steps of matrix work to arrive in. */
+    uint4 next[2], later[2];
+    auto fetch = [&](int y, uint4 (&pre)[2]) {
+#pragma unroll
+        for (int k = 0; k < 2; k++) {
+            const int i = tid + 256 * k, q = i & 3, c = i >> 2, x = x0 - 2 + c;
+            uint4 v = make_uint4(0, 0, 0, 0);
+            if (i < PS_SLOTS && y >= 0 && y < H && x >= 0 && x < W)
+                v = ((const uint4 *)(in + ((size_t)y * W + x) * C))[q];
+            pre[k] = v;
+        }
+    };
+    auto stash = [&](int y, const uint4 (&pre)[2]) {
+#pragma unroll
+        for (int k = 0; k < 2; k++) {
+            const int i = tid + 256 * k;
+            if (i < PS_SLOTS)
+                *(uint4 *)&s_x[y & (RING - 1)][i >> 2][8 * (i & 3)] = pre[k];
+        }
+    };
+    __syncthreads();
+    fetch(y0 - 2, next);
+    stash(y0 - 2, next);
+    fetch(y0 - 1, next);
+    stash(y0 - 1, next);
+    fetch(y0, next);
+    fetch(y0 + 1, later);
+
+    for (int t = y0 - 1; t <= y1 + 1; t++) {
+        __syncthreads(); /* h1 of step t - 1 written; h0 row t - 3's slot free */
+        stash(t + 1, next);
+#pragma unroll
+        for (int k = 0; k < 2; k++)
+            next[k] = later[k];
+        if (t + 3 <= y1 + 1)
+            fetch(t + 3, later);
+        __syncthreads();
+        f32x4 acc[2][2];
+#pragma unroll
+        for (int mt = 0; mt < 2; mt++)
+#pragma unroll
+            for (int n = 0; n < 2; n++)
+                acc[mt][n] = (f32x4){0.f, 0.f, 0.f, 0.f};
+        if (!second) {
+            if (t > y1)
+                continue;
+            /* h1 row t: pixel x0 - 1 + 16 n + r16 sits at ring column
+             * 16 n + r16 + 1; tap (ky, kx) reads h0 column 16 n + r16 + kx. */
+#pragma unroll
+            for (int tap = 0; tap < TAPS; tap++) {
+                const int ky = tap / 3, kx = tap % 3;
+                bf16x8 b[2];
+#pragma unroll
+                for (int n = 0; n < 2; n++)
+                    b[n] = *(const bf16x8 *)&s_x[(t - 1 + ky) & (RING - 1)]
+                                                [16 * (nt0 + n) + r16 + kx][kq];
+#pragma unroll
+                for (int mt = 0; mt < 2; mt++)
+#pragma unroll
+                    for (int n = 0; n < 2; n++)
+                        acc[mt][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa[tap][mt], b[n],
+                                                                             acc[mt][n], 0, 0, 0);
+            }
+            const bool row_in = t >= 0 && t < H;
+#pragma unroll
+            for (int n = 0; n < 2; n++) {
+                const int col = 16 * (nt0 + n) + r16 + 1, x = x0 - 2 + col;
+                const bool in_pic = row_in && x >= 0 && x < W;
+#pragma unroll
+                for (int mt = 0; mt < 2; mt++) {
+                    const int co = 16 * mt + (lane >> 4) * 4;
+                    float v[4];
 #pragma unroll
                     for (int i = 0; i < 4; i++)
-                        o[i] = f2bf(fmaxf(v[i], 0.0f));
-                } else {
-                    const uint2 rv = *(const uint2 *)&s_in[wave + 1][16 * nt + r16 + 1][co];
-                    const uint16_t rs[4] = {(uint16_t)rv.x, (uint16_t)(rv.x >> 16), (uint16_t)rv.y,
-                                            (uint16_t)(rv.y >> 16)};
-#pragma unroll
-                    for (int i = 0; i < 4; i++)
-                        o[i] = f2bf(bf2f(rs[i]) + bf2f(f2bf(fmaxf(v[i], 0.0f))));
+                        v[i] = acc[mt][n][i] + bv[mt][i];
+                    const uint2 o =
+                        resid_relu4(*(const uint2 *)&s_x[t & (RING - 1)][col][co], v);
+                    *(uint2 *)&s_h[t & (RING - 1)][col][co] = in_pic ? o : make_uint2(0, 0);
                 }
-                *(uint2 *)(out + px * C + co) = make_uint2(pack2(o[0], o[1]), pack2(o[2], o[3]));
+            }
+        } else {
+            const int r = t - 2;
+            if (r < y0 || r >= y1)
+                continue;
+            /* h2 row r: pixel x0 + 16 n + r16 sits at ring column
+             * 16 n + r16 + 2; tap (ky, kx) reads h1 column 16 n + r16 + kx + 1. */
+#pragma unroll
+            for (int tap = 0; tap < TAPS; tap++) {
+                const int ky = tap / 3, kx = tap % 3;
+                bf16x8 b[2];
+#pragma unroll
+                for (int n = 0; n < 2; n++)
+                    b[n] = *(const bf16x8 *)&s_h[(r - 1 + ky) & (RING - 1)]
+                                                [16 * (nt0 + n) + r16 + kx + 1][kq];
+#pragma unroll
+                for (int mt = 0; mt < 2; mt++)
+#pragma unroll
+                    for (int n = 0; n < 2; n++)
+                        acc[mt][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa[tap][mt], b[n],
+                                                                             acc[mt][n], 0, 0, 0);
+            }
+#pragma unroll
+            for (int n = 0; n < 2; n++) {
+                const int px = 16 * (nt0 + n) + r16, x = x0 + px;
+                if (px >= PS_W || x >= W)
+                    continue;
+#pragma unroll
+                for (int mt = 0; mt < 2; mt++) {
+                    const int co = 16 * mt + (lane >> 4) * 4;
+                    float v[4];
+#pragma unroll
+                    for (int i = 0; i < 4; i++)
+                        v[i] = acc[mt][n][i] + bv[mt][i];
+                    *(uint2 *)(out + ((size_t)r * W + x) * C + co) =
+                        resid_relu4(*(const uint2 *)&s_h[r & (RING - 1)][px + 2][co], v);
+                }
             }
         }
     }
@@ -629,18 +813,15 @@ __device__ inline float2 curvature_ratios(const NetFrame &f, float a, int y, int
 __device__ inline void write_cell(const NetFrame &f, float a, float vr, float vg1, float vg2,
                                   float vb, float2 rr, float2 rb, uint16_t *dst)
 {
-    const uint16_t r = f2bf(vr), g1 = f2bf(vg1), g2 = f2bf(vg2), b = f2bf(vb);
     const float dr = deviation(a, f, f.noise_off, vr), dg1 = deviation(a, f, f.noise_off, vg1);
     const float dg2 = deviation(a, f, f.noise_off, vg2), db = deviation(a, f, f.noise_off, vb);
-    const uint16_t sr = f2bf(128.0f * dr), sg1 = f2bf(128.0f * dg1);
-    const uint16_t sg2 = f2bf(128.0f * dg2), sb = f2bf(128.0f * db);
     /* 1 (bf16 0x3F80) where a sensel lies above its noise deviation at the
      * chroma offset, the test the chroma estimates make (demosaic.rs:576-587). */
     auto above = [](float v, float dev) { return dev < v ? (uint16_t)0x3F80u : (uint16_t)0u; };
     uint4 *d = (uint4 *)dst;
-    d[0] = make_uint4(pack2(r, g1), pack2(g2, b), pack2(f2bf(feature_of(f)), sr),
-                      pack2(sg1, sg2));
-    d[1] = make_uint4(pack2(sb, f2bf(rr.x)), pack2(f2bf(rr.y), f2bf(rb.x)),
+    d[0] = make_uint4(f2bf2(vr, vg1), f2bf2(vg2, vb), f2bf2(feature_of(f), 128.0f * dr),
+                      f2bf2(128.0f * dg1, 128.0f * dg2));
+    d[1] = make_uint4(f2bf2(128.0f * db, rr.x), f2bf2(rr.y, rb.x),
                       pack2(f2bf(rb.y), above(vr, dr)), pack2(above(vg1, dg1), above(vg2, dg2)));
     d[2] = make_uint4(above(vb, db), 0, 0, 0);
     d[3] = make_uint4(0, 0, 0, 0);
@@ -844,10 +1025,8 @@ __global__ __launch_bounds__(256) void k_loss(const NetFrame *__restrict__ frame
         uint4 *dst = (uint4 *)(dout + i * C);
 #pragma unroll
         for (int q = 0; q < 4; q++)
-            dst[q] = make_uint4(pack2(f2bf(d[8 * q + 0]), f2bf(d[8 * q + 1])),
-                                pack2(f2bf(d[8 * q + 2]), f2bf(d[8 * q + 3])),
-                                pack2(f2bf(d[8 * q + 4]), f2bf(d[8 * q + 5])),
-                                pack2(f2bf(d[8 * q + 6]), f2bf(d[8 * q + 7])));
+            dst[q] = make_uint4(f2bf2(d[8 * q + 0], d[8 * q + 1]), f2bf2(d[8 * q + 2], d[8 * q + 3]),
+                                f2bf2(d[8 * q + 4], d[8 * q + 5]), f2bf2(d[8 * q + 6], d[8 * q + 7]));
     }
     s_sum[threadIdx.x] = sq;
     __syncthreads();
@@ -1066,6 +1245,19 @@ hipError_t launch_conv_infer(int mode, const uint16_t *in, int h, int w, const u
 }
 
 const void *conv_infer_kernel() { return (const void *)k_conv_infer<IM_RES>; }
+
+hipError_t launch_conv_pair(const uint16_t *in, int h, int w, const uint16_t *wa, const float *ba,
+                            const uint16_t *wb, const float *bb, uint16_t *out,
+                            const uint32_t *skip, hipStream_t s)
+{
+    if (h <= 0 || w <= 0)
+        return hipSuccess;
+    const uint32_t strips = (uint32_t)((w + PS_W - 1) / PS_W);
+    const uint32_t segs = (uint32_t)((h + PS_SEG - 1) / PS_SEG);
+    hipLaunchKernelGGL(k_conv_pair, dim3(strips * segs), dim3(256), 0, s, in, h, w, wa, ba, wb,
+                       bb, out, skip);
+    return hipGetLastError();
+}
 
 
 hipError_t launch_weights(const float *params, uint16_t *wf, uint16_t *wt, hipStream_t s)

@@ -67,6 +67,7 @@ struct prr_net {
     std::map<std::tuple<uint32_t, uint32_t, uint32_t>, Buf> tiles;
     uint32_t last_k = 0;
     size_t last_cells = 0;  /* cells of the last whole-frame packing */
+    bool pairs = true;      /* residual layers two a launch (k_conv_pair) */
     uint32_t conv_grid = 0; /* persistent inference convolution: workgroups */
 };
 
@@ -533,9 +534,16 @@ extern "C" int prr_net_develop_device(prr_net *net, const prr_header *hdr,
     NET_TRY(launch_conv_infer(IM_HEAD, net->ix0.as<uint16_t>(), hh, ww, W(0), B(0), a, nullptr,
                               flag, g, s),
             "conv kernel");
-    for (int l = 1; l < 7; l++) {
-        NET_TRY(launch_conv_infer(IM_RES, a, hh, ww, W(l), B(l), b, nullptr, flag, g, s),
-                "conv kernel");
+    for (int l = 1; l < 7;) {
+        if (net->pairs) {
+            NET_TRY(launch_conv_pair(a, hh, ww, W(l), B(l), W(l + 1), B(l + 1), b, flag, s),
+                    "conv kernel");
+            l += 2;
+        } else {
+            NET_TRY(launch_conv_infer(IM_RES, a, hh, ww, W(l), B(l), b, nullptr, flag, g, s),
+                    "conv kernel");
+            l += 1;
+        }
         std::swap(a, b);
     }
     NET_TRY(launch_conv_infer(IM_TAIL, a, hh, ww, W(7), B(7), nullptr, net->iout.as<float>(), flag,
@@ -547,6 +555,14 @@ extern "C" int prr_net_develop_device(prr_net *net, const prr_header *hdr,
     NET_TRY(launch_develop(d, d_r, d_g, d_b, s), "develop kernel");
     net->last_k = k;
     net->last_cells = cells;
+    return PRR_OK;
+}
+
+extern "C" int prr_net_set_pairs(prr_net *net, int on, prr_error *err)
+{
+    if (!net)
+        return net_fail(err, PRR_E_INVALID_ARGUMENT, "net");
+    net->pairs = on != 0;
     return PRR_OK;
 }
 

@@ -234,6 +234,11 @@ class Net:
                    int(stage), check_tiles, ctypes.c_float(min_db), planes[0].data_ptr(),
                    planes[1].data_ptr(), planes[2].data_ptr(), ctypes.c_void_p(s.cuda_stream))
 
+    def set_pairs(self, on: bool):
+        """Residual layers two a launch (the default) or one a launch; the
+        same bits either way."""
+        self._call("prr_net_set_pairs", int(bool(on)))
+
     def last_check(self):
         """(used_plain, lowest sampled-tile PSNR) of the last develop."""
         used, psnr = ctypes.c_uint32(), ctypes.c_float()

@@ -397,14 +397,16 @@ def test_trained_network_passes_the_floor_and_beats_bilinear(env):
         assert abs(worst - want) < 1e-3, (worst, want)
 
 
+@pytest.mark.parametrize("pairs", [True, False])
 @pytest.mark.parametrize("cells", [72, 100, 131])
-def test_inference_convolution_equals_the_training_forward(env, cells):
-    """The inference convolution (persistent, weights held in registers,
-    12 output channels) does the training forward's arithmetic: on a frame
-    that is exactly one training crop, the learned develop at
-    Stage::Demosaiced equals, bit for bit, min(out + bilinear, 1) built
-    from the training forward's output of the same crop; frame sizes that
-    are and are not multiples of its 4 x 64-cell tile."""
+def test_inference_convolution_equals_the_training_forward(env, cells, pairs):
+    """The inference convolutions (persistent, weights held in registers,
+    12 output channels; the residual layers two a launch or one a launch)
+    do the training forward's arithmetic: on a frame that is exactly one
+    training crop, the learned develop at Stage::Demosaiced equals, bit for
+    bit, min(out + bilinear, 1) built from the training forward's output of
+    the same crop; frame sizes that are and are not multiples of the 4 x
+    64-cell tile, the paired layers' 62-column strips and 64-row segments."""
     torch, L, _ = env
     from proresraw_rs_amd import synth as S
     from proresraw_rs_amd.api import Stage
@@ -412,6 +414,7 @@ def test_inference_convolution_equals_the_training_forward(env, cells):
                  qscale=(30, 30), white_level=4095)
     f = L.frame_pair(S.encode_scene(sc))
     net = L.Net(L.init_params(11))
+    net.set_pairs(pairs)
     net.set_range(0.0, 100.0)
     net.train_setup([f], np.zeros((1, 1, 3), np.uint32), cells, 0)
     net.train_step(0.0)
@@ -427,3 +430,32 @@ def test_inference_convolution_equals_the_training_forward(env, cells):
                 full[c, dy::2, dx::2] = out[:, :, 3 * (2 * dy + dx) + c]
     want = np.minimum((full + base) * np.float32(1.0), np.float32(1.0)).astype(np.float32)
     np.testing.assert_array_equal(got, want.reshape(3, -1))
+
+
+@pytest.mark.parametrize("pattern", [0, 3])
+def test_paired_residual_layers_equal_single_layers(env, pattern):
+    """The residual layers two a launch (the middle activations kept in LDS
+    along a strip) give the planes of one layer a launch bit for bit, with
+    the shipped weights, on a frame of 9 strips and 6 segments whose last
+    ones are partial, and on a flipped pattern, white balanced to Rec.2020."""
+    torch, L, _ = env
+    from proresraw_rs_amd import synth as S
+    from proresraw_rs_amd.api import DemosaicParams, Stage
+    sc = S.scene(1000, 700, pattern, seed=880 + pattern, version=0, alignment=3,
+                 qscale=(33, 33), white_level=4095)
+    f = L.frame_pair(S.encode_scene(sc))
+    h = f.header
+    prm = DemosaicParams.new(h, f.quantizer)
+    params, lo, hi = L.load_weights()
+    net = L.Net(params)
+    net.set_range(lo, hi)
+    out = {}
+    for pairs in (True, False):
+        net.set_pairs(pairs)
+        planes = [torch.zeros(h.width * h.height, dtype=torch.float32, device="cuda")
+                  for _ in range(3)]
+        net.develop(h, f.samples, prm, Stage.Rec2020, planes, check_tiles=0)
+        torch.cuda.synchronize()
+        assert not net.last_check()[0]
+        out[pairs] = np.stack([p.cpu().numpy() for p in planes])
+    assert np.array_equal(out[True].view(np.uint32), out[False].view(np.uint32))
