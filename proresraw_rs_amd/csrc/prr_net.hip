@@ -669,22 +669,30 @@ __global__ void k_noise_gate(NetFrame f, float lo, float hi, uint32_t *use_plain
 /* ------------------------------------------------------------------ */
 /* Mosaic access in RGGB order. */
 
-__device__ inline uint32_t lin_at(const NetFrame &f, uint32_t code)
+/* Curve::linearize (curve.rs:14-24) through the 9 curve points `curve`
+ * (the frame's, or a copy in LDS). */
+__device__ inline uint32_t lin_at(const uint32_t *curve, uint32_t code)
 {
     const uint32_t seg = code >> 13, frac = code & 0x1FFFu;
-    const uint32_t c0 = f.curve[seg], c1 = f.curve[seg + 1];
+    const uint32_t c0 = curve[seg], c1 = curve[seg + 1];
     const uint32_t v = (c0 * 8192u + ((c1 - c0) & 0xFFFFu) * frac + 4096u) >> 13;
     return v < 65535u ? v : 65535u;
 }
 
-/* Normalized value of the sensel at RGGB-order (y, x), red and blue
- * clipped. */
-__device__ inline float norm_at(const NetFrame &f, int y, int x)
+/* The code value of the sensel at RGGB-order (y, x). */
+__device__ inline uint32_t code_at(const NetFrame &f, int y, int x)
 {
     const uint32_t sy = f.flip_y ? f.height - 1 - (uint32_t)y : (uint32_t)y;
     const uint32_t sx = f.flip_x ? f.width - 1 - (uint32_t)x : (uint32_t)x;
-    const uint32_t code = f.samples[(size_t)sy * f.stride + sx];
-    float v = f.one_over_range * (float)((int32_t)lin_at(f, code) - f.black);
+    return f.samples[(size_t)sy * f.stride + sx];
+}
+
+/* Normalized value of code value `code` at RGGB-order (y, x), red and blue
+ * clipped. */
+__device__ inline float norm_of_code(const NetFrame &f, const uint32_t *curve, uint32_t code, int y,
+                                     int x)
+{
+    float v = f.one_over_range * (float)((int32_t)lin_at(curve, code) - f.black);
     if (((y | x) & 1) == 0)
         v = fminf(v, f.clip_r);
     else if ((y & x & 1) == 1)
@@ -692,15 +700,35 @@ __device__ inline float norm_at(const NetFrame &f, int y, int x)
     return v;
 }
 
+/* Normalized value of the sensel at RGGB-order (y, x), red and blue
+ * clipped. `curve`: the frame's 9 curve points, in LDS (stage_curve) where
+ * the frame is a kernel argument (indexing the argument at a run-time index
+ * would copy it to scratch memory), or the frame's own in global memory. */
+__device__ inline float norm_at(const NetFrame &f, int y, int x, const uint32_t *curve)
+{
+    return norm_of_code(f, curve, code_at(f, y, x), y, x);
+}
+
+/* The frame's curve points into LDS (constant indices; the caller
+ * synchronizes before use). */
+__device__ inline void stage_curve(const NetFrame &f, uint32_t *s_curve)
+{
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int j = 0; j < 9; j++)
+            s_curve[j] = f.curve[j];
+    }
+}
+
 __device__ inline int mirror(int i, int n) { return i < 0 ? -i : i >= n ? 2 * n - 2 - i : i; }
 
 /* Bilinear demosaic at (y, x) of the window of rows/columns [oy, oy+hh) x
  * [ox, ox+ww), mirrored at the window's edges. */
-__device__ inline void bilinear(const NetFrame &f, int oy, int ox, int hh, int ww, int y, int x,
-                                float *rgb)
+__device__ inline void bilinear(const NetFrame &f, const uint32_t *curve, int oy, int ox, int hh,
+                                int ww, int y, int x, float *rgb)
 {
     auto v = [&](int dy, int dx) {
-        return norm_at(f, oy + mirror(y + dy, hh), ox + mirror(x + dx, ww));
+        return norm_at(f, oy + mirror(y + dy, hh), ox + mirror(x + dx, ww), curve);
     };
     const int py = y & 1, px = x & 1;
     const float c = v(0, 0);
@@ -747,19 +775,30 @@ __device__ inline float deviation(float a, const NetFrame &f, float off, float v
 
 /* The green pass's edge rule (mirror, demosaic.rs:547-557) on the four
  * neighbours d of a sensel at `pos` of a line of `len`, centre c. */
-__device__ inline void mirror_line(float *d, int pos, int len, float c)
+__device__ inline void mirror_line(float (&d)[4], int pos, int len, float c)
 {
-    if (pos == 0) {
-        d[0] = d[1];
-        d[2] = d[3];
-    } else if (pos == 1) {
-        d[2] = c;
-    } else if (pos == len - 1) {
-        d[1] = d[0];
-        d[3] = d[2];
-    } else if (pos == len - 2) {
-        d[3] = c;
-    }
+    /* The crate's if / else-if chain as selects, so the values stay in
+     * registers. */
+    const bool p0 = pos == 0, p1 = !p0 && pos == 1;
+    const bool pl = !p0 && !p1 && pos == len - 1, pl2 = !p0 && !p1 && !pl && pos == len - 2;
+    const float d0 = d[0], d1 = d[1], d2 = d[2], d3 = d[3];
+    d[0] = p0 ? d1 : d0;
+    d[1] = pl ? d0 : d1;
+    d[2] = p0 ? d3 : p1 ? c : d2;
+    d[3] = p0 ? d3 : pl ? d2 : pl2 ? c : d3;
+}
+
+/* One direction's ratio of the green pass (demosaic.rs:520-530): its four
+ * neighbours d with their deviations sd, the centre c, and t[2 + dir] /
+ * (t[dir] * 2) as tn / (td * 2) when the noise tests let it apply. */
+__device__ inline float direction_ratio(const float (&d)[4], const float (&sd)[4], float c,
+                                        float tn, float td)
+{
+    const float p = d[0] - d[1], q = d[2] - d[3];
+    const bool any = p * q > 0.0f || (d[2] - c) * (d[3] - c) > 0.0f ||
+                     sd[0] + sd[1] > fabsf(p) || sd[2] + sd[3] > fabsf(q);
+    const bool all = d[0] > sd[0] && d[1] > sd[1] && d[2] > sd[2] && d[3] > sd[3];
+    return any && all ? tn / (td * 2.0f) : 0.0f;
 }
 
 /* The green pass's curvature ratios at the red or blue sensel (y, x) of the
@@ -770,11 +809,12 @@ __device__ inline void mirror_line(float *d, int pos, int len, float c)
  * at the picture's edges, as the crate reads them; flips to RGGB order
  * only swap the two neighbours of each pair, which leaves every test and
  * ratio unchanged. */
-__device__ inline float2 curvature_ratios(const NetFrame &f, float a, int y, int x, float clip)
+__device__ inline float2 curvature_ratios(const NetFrame &f, const uint32_t *curve, float a, int y,
+                                          int x, float clip)
 {
     const int w = (int)f.width, h = (int)f.height;
     auto at = [&](int yy, int xx) {
-        return norm_at(f, min(max(yy, 0), h - 1), min(max(xx, 0), w - 1));
+        return norm_at(f, min(max(yy, 0), h - 1), min(max(xx, 0), w - 1), curve);
     };
     const float off = f.gp_off;
     const float c = fminf(at(y, x), clip);
@@ -784,27 +824,19 @@ __device__ inline float2 curvature_ratios(const NetFrame &f, float a, int y, int
                    fminf(at(y - 2, x), clip), fminf(at(y + 2, x), clip)};
     mirror_line(hh, x, w, c);
     mirror_line(vv, y, h, c);
-    float ratio[2] = {0.0f, 0.0f};
+    float2 ratio = make_float2(0.0f, 0.0f);
     if (c > deviation(a, f, off, c)) {
         const float sums2 = hh[2] + hh[3], sums3 = vv[2] + vv[3];
-        const float t[4] = {c * 2.0f + sums2, c * 2.0f + sums3, c * -2.0f + sums2,
-                            c * -2.0f + sums3};
+        float sh[4], sv[4];
 #pragma unroll
-        for (int dir = 0; dir < 2; dir++) {
-            const float *d = dir ? vv : hh;
-            float sd[4];
-#pragma unroll
-            for (int i = 0; i < 4; i++)
-                sd[i] = deviation(a, f, off, d[i]);
-            const float p = d[0] - d[1], q = d[2] - d[3];
-            const bool any = p * q > 0.0f || (d[2] - c) * (d[3] - c) > 0.0f ||
-                             sd[0] + sd[1] > fabsf(p) || sd[2] + sd[3] > fabsf(q);
-            const bool all = d[0] > sd[0] && d[1] > sd[1] && d[2] > sd[2] && d[3] > sd[3];
-            if (any && all)
-                ratio[dir] = t[2 + dir] / (t[dir] * 2.0f);
+        for (int i = 0; i < 4; i++) {
+            sh[i] = deviation(a, f, off, hh[i]);
+            sv[i] = deviation(a, f, off, vv[i]);
         }
+        ratio.x = direction_ratio(hh, sh, c, c * -2.0f + sums2, c * 2.0f + sums2);
+        ratio.y = direction_ratio(vv, sv, c, c * -2.0f + sums3, c * 2.0f + sums3);
     }
-    return make_float2(ratio[0], ratio[1]);
+    return ratio;
 }
 
 /* One packed cell, 32 bf16 channels (the layout in this file's header),
@@ -829,13 +861,14 @@ __device__ inline void write_cell(const NetFrame &f, float a, float vr, float vg
 
 /* One packed cell of the RGGB-order mosaic at (y, x), every value read from
  * the frame (the crops and check windows). */
-__device__ inline void pack_cell(const NetFrame &f, int y, int x, uint16_t *dst)
+__device__ inline void pack_cell(const NetFrame &f, const uint32_t *curve, int y, int x,
+                                 uint16_t *dst)
 {
-    const float vr = norm_at(f, y, x), vg1 = norm_at(f, y, x + 1);
-    const float vg2 = norm_at(f, y + 1, x), vb = norm_at(f, y + 1, x + 1);
+    const float vr = norm_at(f, y, x, curve), vg1 = norm_at(f, y, x + 1, curve);
+    const float vg2 = norm_at(f, y + 1, x, curve), vb = norm_at(f, y + 1, x + 1, curve);
     const float a = noise_a_of(f);
-    const float2 rr = curvature_ratios(f, a, y, x, f.clip_r);
-    const float2 rb = curvature_ratios(f, a, y + 1, x + 1, f.clip_b);
+    const float2 rr = curvature_ratios(f, curve, a, y, x, f.clip_r);
+    const float2 rb = curvature_ratios(f, curve, a, y + 1, x + 1, f.clip_b);
     write_cell(f, a, vr, vg1, vg2, vb, rr, rb, dst);
 }
 
@@ -851,7 +884,7 @@ __global__ void k_pack_crops(const NetFrame *__restrict__ frames, const Crop *__
     const Crop cr = crops[(size_t)(*step % steps) * n + k];
     const NetFrame &f = frames[cr.frame];
     const int ci = c / w2, cj = c - ci * w2;
-    pack_cell(f, (int)cr.y0 + 2 * ci, (int)cr.x0 + 2 * cj, x0 + i * C);
+    pack_cell(f, f.curve, (int)cr.y0 + 2 * ci, (int)cr.x0 + 2 * cj, x0 + i * C);
 }
 
 /* The whole frame, one 256-lane workgroup per tile of PT_W x PT_H cells:
@@ -868,19 +901,36 @@ __global__ __launch_bounds__(256) void k_pack_frame(NetFrame f, uint16_t *__rest
                                                     const uint32_t *__restrict__ skip)
 {
     __shared__ float s_raw[PT_SH][PT_SW], s_lim[PT_SH][PT_SW], s_dev[PT_SH][PT_SW];
+    __shared__ uint32_t s_curve[9];
     if (skip && *skip)
         return;
     const int w = (int)f.width, h = (int)f.height, w2 = w / 2, h2 = h / 2;
     const int cx0 = (int)blockIdx.x * PT_W, cy0 = (int)blockIdx.y * PT_H;
     const int sx0 = 2 * cx0 - 2, sy0 = 2 * cy0 - 2;
-    const float a = noise_a_of(f);
-    for (int i = (int)threadIdx.x; i < PT_SH * PT_SW; i += 256) {
+    stage_curve(f, s_curve);
+    /* Every code value a lane needs is loaded before any is used. */
+    constexpr int N = PT_SH * PT_SW, PER = (N + 255) / 256;
+    uint32_t code[PER];
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+        const int i = (int)threadIdx.x + 256 * k;
         const int ly = i / PT_SW, lx = i - ly * PT_SW;
         const int y = min(max(sy0 + ly, 0), h - 1), x = min(max(sx0 + lx, 0), w - 1);
-        const float v = norm_at(f, y, x);
-        /* Red and blue values are clipped by norm_at (at their row's clip,
-         * the green pass's limit for same-colour neighbours); greens are
-         * capped at 1 as neighbours. */
+        code[k] = i < N ? code_at(f, y, x) : 0u;
+    }
+    const float a = noise_a_of(f);
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+        const int i = (int)threadIdx.x + 256 * k;
+        if (i >= N)
+            break;
+        const int ly = i / PT_SW, lx = i - ly * PT_SW;
+        const int y = min(max(sy0 + ly, 0), h - 1), x = min(max(sx0 + lx, 0), w - 1);
+        const float v = norm_of_code(f, s_curve, code[k], y, x);
+        /* Red and blue values are clipped (at their row's clip, the green
+         * pass's limit for same-colour neighbours); greens are capped at 1
+         * as neighbours. */
         const float lim = ((y ^ x) & 1) ? fminf(v, 1.0f) : v;
         s_raw[ly][lx] = v;
         s_lim[ly][lx] = lim;
@@ -905,24 +955,13 @@ __global__ __launch_bounds__(256) void k_pack_frame(NetFrame f, uint16_t *__rest
         mirror_line(sh, x, w, sc);
         mirror_line(vv, y, h, c);
         mirror_line(sv, y, h, sc);
-        float ratio[2] = {0.0f, 0.0f};
+        float2 ratio = make_float2(0.0f, 0.0f);
         if (c > sc) {
             const float sums2 = hh[2] + hh[3], sums3 = vv[2] + vv[3];
-            const float t[4] = {c * 2.0f + sums2, c * 2.0f + sums3, c * -2.0f + sums2,
-                                c * -2.0f + sums3};
-#pragma unroll
-            for (int dir = 0; dir < 2; dir++) {
-                const float *d = dir ? vv : hh;
-                const float *sd = dir ? sv : sh;
-                const float p = d[0] - d[1], q = d[2] - d[3];
-                const bool any = p * q > 0.0f || (d[2] - c) * (d[3] - c) > 0.0f ||
-                                 sd[0] + sd[1] > fabsf(p) || sd[2] + sd[3] > fabsf(q);
-                const bool all = d[0] > sd[0] && d[1] > sd[1] && d[2] > sd[2] && d[3] > sd[3];
-                if (any && all)
-                    ratio[dir] = t[2 + dir] / (t[dir] * 2.0f);
-            }
+            ratio.x = direction_ratio(hh, sh, c, c * -2.0f + sums2, c * 2.0f + sums2);
+            ratio.y = direction_ratio(vv, sv, c, c * -2.0f + sums3, c * 2.0f + sums3);
         }
-        return make_float2(ratio[0], ratio[1]);
+        return ratio;
     };
     const float vr = s_raw[ry][rx], vg1 = s_raw[ry][rx + 1];
     const float vg2 = s_raw[ry + 1][rx], vb = s_raw[ry + 1][rx + 1];
@@ -952,6 +991,9 @@ __device__ inline void window_origin(const NetFrame &f, uint32_t bx, uint32_t by
 __global__ void k_pack_windows(NetFrame f, const uint32_t *__restrict__ tiles, int n, int sy,
                                int sx, uint16_t *__restrict__ x0)
 {
+    __shared__ uint32_t s_curve[9];
+    stage_curve(f, s_curve);
+    __syncthreads();
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const size_t cells = (size_t)sy * sx;
     if (i >= cells * n)
@@ -960,7 +1002,7 @@ __global__ void k_pack_windows(NetFrame f, const uint32_t *__restrict__ tiles, i
     int wy, wx;
     window_origin(f, tiles[2 * k], tiles[2 * k + 1], sy, sx, wy, wx);
     const int ci = c / sx, cj = c - ci * sx;
-    pack_cell(f, 2 * (wy + ci), 2 * (wx + cj), x0 + i * C);
+    pack_cell(f, s_curve, 2 * (wy + ci), 2 * (wx + cj), x0 + i * C);
 }
 
 /* ------------------------------------------------------------------ */
@@ -1007,7 +1049,7 @@ __global__ __launch_bounds__(256) void k_loss(const NetFrame *__restrict__ frame
                 for (int dx = 0; dx < 2; dx++) {
                     float base[3];
                     const int y = 2 * ci + dy, x = 2 * cj + dx;
-                    bilinear(f, (int)cr.y0, (int)cr.x0, 2 * h2, 2 * w2, y, x, base);
+                    bilinear(f, f.curve, (int)cr.y0, (int)cr.x0, 2 * h2, 2 * w2, y, x, base);
                     const size_t at = (size_t)((int)cr.y0 + y) * f.width + (size_t)((int)cr.x0 + x);
 #pragma unroll
                     for (int col = 0; col < 3; col++) {
@@ -1062,8 +1104,11 @@ __global__ __launch_bounds__(256) void k_compose(NetFrame f, Finish fin,
                                                  float *__restrict__ pb,
                                                  const uint32_t *__restrict__ skip)
 {
+    __shared__ uint32_t s_curve[9];
     if (skip && *skip)
         return;
+    stage_curve(f, s_curve);
+    __syncthreads();
     const int w2 = (int)f.width / 2, h2 = (int)f.height / 2;
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (size_t)h2 * w2)
@@ -1079,7 +1124,7 @@ __global__ __launch_bounds__(256) void k_compose(NetFrame f, Finish fin,
         const int y = mirror(2 * ci - 1 + r, hh);
 #pragma unroll
         for (int c = 0; c < 4; c++)
-            nb[r][c] = norm_at(f, y, mirror(2 * cj - 1 + c, ww));
+            nb[r][c] = norm_at(f, y, mirror(2 * cj - 1 + c, ww), s_curve);
     }
     float o[OUT_CH];
 #pragma unroll
@@ -1153,6 +1198,9 @@ __global__ __launch_bounds__(256) void k_check_psnr(NetFrame f, Finish fin,
 {
     __shared__ double s_sum[256];
     __shared__ uint32_t s_n[256];
+    __shared__ uint32_t s_curve[9];
+    stage_curve(f, s_curve);
+    __syncthreads();
     const uint32_t k = blockIdx.x;
     const uint32_t x0 = 64 * tiles[2 * k], y0 = 64 * tiles[2 * k + 1];
     const float *plain = scratch + (size_t)k * 3 * 4096;
@@ -1168,7 +1216,7 @@ __global__ __launch_bounds__(256) void k_check_psnr(NetFrame f, Finish fin,
         const int y = (int)(f.flip_y ? f.height - 1 - py : py);
         const int x = (int)(f.flip_x ? f.width - 1 - px : px);
         float base[3];
-        bilinear(f, 0, 0, (int)f.height, (int)f.width, y, x, base);
+        bilinear(f, s_curve, 0, 0, (int)f.height, (int)f.width, y, x, base);
         const size_t cell = (size_t)((y >> 1) - wy) * sx + ((x >> 1) - wx);
         const int ch = 3 * (2 * (y & 1) + (x & 1));
         const float r = fminf((out[cell * C + ch + 0] + base[0]) * fin.wb[0], 1.0f);
