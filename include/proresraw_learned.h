@@ -100,9 +100,9 @@ int prr_net_develop_device(prr_net *net, const prr_header *hdr, const uint16_t *
                            const prr_demosaic_params *params, const prr_device_status *d_status,
                            int stage, uint32_t check_tiles, float min_db, float *d_r, float *d_g,
                            float *d_b, void *stream, prr_error *err);
-/* Whether prr_net_develop_device runs the six residual layers two a launch
- * (on: the default; each pair keeps its middle activations in LDS and reads
- * and writes HBM once) or one a launch. Both give the same bits. */
+/* Whether prr_net_develop_device runs the network's eight layers two a
+ * launch (on: the default; each pair keeps its middle activations in LDS and
+ * reads and writes HBM once) or one a launch. Both give the same bits. */
 int prr_net_set_pairs(prr_net *net, int on, prr_error *err);
 
 /* After the stream has run the last prr_net_develop_device: whether it

@@ -79,11 +79,14 @@ hipError_t launch_conv_infer(int mode, const uint16_t *in, int h, int w, const u
                              const float *bias, uint16_t *out, float *out12, const uint32_t *skip,
                              uint32_t grid, hipStream_t s);
 const void *conv_infer_kernel();
-/* Two residual layers (weights wa, ba then wb, bb) in one launch: the same
- * bits as two launch_conv_infer(IM_RES) calls. */
-hipError_t launch_conv_pair(const uint16_t *in, int h, int w, const uint16_t *wa, const float *ba,
-                            const uint16_t *wb, const float *bb, uint16_t *out,
-                            const uint32_t *skip, hipStream_t s);
+/* Two consecutive layers (weights wa, ba then wb, bb) in one launch, the
+ * first of them the network's first layer when `first` (else a residual
+ * one), the second the last layer, writing out12, when `last` (else a
+ * residual one, writing out): the same bits as two launch_conv_infer calls. */
+hipError_t launch_conv_pair(int first, int last, const uint16_t *in, int h, int w,
+                            const uint16_t *wa, const float *ba, const uint16_t *wb,
+                            const float *bb, uint16_t *out, float *out12, const uint32_t *skip,
+                            hipStream_t s);
 hipError_t launch_weights(const float *params, uint16_t *wf, uint16_t *wt, hipStream_t s);
 hipError_t launch_conv(int epi, const uint16_t *in, int n, int h, int w, const uint16_t *wgt,
                        const float *bias, const uint16_t *resid, uint16_t *out, uint16_t *aux,

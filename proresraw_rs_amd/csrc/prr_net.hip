@@ -358,14 +358,20 @@ __global__ __launch_bounds__(256) void k_conv_infer(const uint16_t *__restrict__
  * traffic and 3 % more matrix work (64 columns computed for 62). */
 enum { PS_W = 62, PS_COLS = 68, PS_SEG = 64, RING = 4, PS_SLOTS = 66 * 4 };
 
+template <int AM, int BM>
 __global__ __launch_bounds__(256) void k_conv_pair(const uint16_t *__restrict__ in, int H, int W,
                                                    const uint16_t *__restrict__ wA,
                                                    const float *__restrict__ bA,
                                                    const uint16_t *__restrict__ wB,
                                                    const float *__restrict__ bB,
                                                    uint16_t *__restrict__ out,
+                                                   float *__restrict__ out12,
                                                    const uint32_t *__restrict__ skip)
 {
+    static_assert(AM == IM_HEAD || AM == IM_RES, "layer A: the first or a residual layer");
+    static_assert(BM == IM_RES || BM == IM_TAIL, "layer B: a residual or the last layer");
+    /* The last layer computes one 16-channel tile (its 12 outputs). */
+    constexpr int MTB = BM == IM_TAIL ? 1 : 2;
     __shared__ __attribute__((aligned(16))) uint16_t s_x[RING][PS_COLS][CP];
     __shared__ __attribute__((aligned(16))) uint16_t s_h[RING][PS_COLS][CP];
     if (skip && *skip)
@@ -476,8 +482,12 @@ __global__ __launch_bounds__(256) void k_conv_pair(const uint16_t *__restrict__ 
 #pragma unroll
                     for (int i = 0; i < 4; i++)
                         v[i] = acc[mt][n][i] + bv[mt][i];
-                    const uint2 o =
-                        resid_relu4(*(const uint2 *)&s_x[t & (RING - 1)][col][co], v);
+                    uint2 o;
+                    if (AM == IM_HEAD)
+                        o = make_uint2(f2bf2(fmaxf(v[0], 0.0f), fmaxf(v[1], 0.0f)),
+                                       f2bf2(fmaxf(v[2], 0.0f), fmaxf(v[3], 0.0f)));
+                    else
+                        o = resid_relu4(*(const uint2 *)&s_x[t & (RING - 1)][col][co], v);
                     *(uint2 *)&s_h[t & (RING - 1)][col][co] = in_pic ? o : make_uint2(0, 0);
                 }
             }
@@ -496,7 +506,7 @@ __global__ __launch_bounds__(256) void k_conv_pair(const uint16_t *__restrict__ 
                     b[n] = *(const bf16x8 *)&s_h[(r - 1 + ky) & (RING - 1)]
                                                 [16 * (nt0 + n) + r16 + kx + 1][kq];
 #pragma unroll
-                for (int mt = 0; mt < 2; mt++)
+                for (int mt = 0; mt < MTB; mt++)
 #pragma unroll
                     for (int n = 0; n < 2; n++)
                         acc[mt][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa[tap][mt], b[n],
@@ -508,14 +518,20 @@ __global__ __launch_bounds__(256) void k_conv_pair(const uint16_t *__restrict__ 
                 if (px >= PS_W || x >= W)
                     continue;
 #pragma unroll
-                for (int mt = 0; mt < 2; mt++) {
+                for (int mt = 0; mt < MTB; mt++) {
                     const int co = 16 * mt + (lane >> 4) * 4;
                     float v[4];
 #pragma unroll
                     for (int i = 0; i < 4; i++)
                         v[i] = acc[mt][n][i] + bv[mt][i];
-                    *(uint2 *)(out + ((size_t)r * W + x) * C + co) =
-                        resid_relu4(*(const uint2 *)&s_h[r & (RING - 1)][px + 2][co], v);
+                    if (BM == IM_TAIL) {
+                        if (co < OUT_CH)
+                            *(float4 *)(out12 + ((size_t)r * W + x) * OUT_CH + co) =
+                                make_float4(v[0], v[1], v[2], v[3]);
+                    } else {
+                        *(uint2 *)(out + ((size_t)r * W + x) * C + co) =
+                            resid_relu4(*(const uint2 *)&s_h[r & (RING - 1)][px + 2][co], v);
+                    }
                 }
             }
         }
@@ -1294,16 +1310,26 @@ hipError_t launch_conv_infer(int mode, const uint16_t *in, int h, int w, const u
 
 const void *conv_infer_kernel() { return (const void *)k_conv_infer<IM_RES>; }
 
-hipError_t launch_conv_pair(const uint16_t *in, int h, int w, const uint16_t *wa, const float *ba,
-                            const uint16_t *wb, const float *bb, uint16_t *out,
-                            const uint32_t *skip, hipStream_t s)
+hipError_t launch_conv_pair(int first, int last, const uint16_t *in, int h, int w,
+                            const uint16_t *wa, const float *ba, const uint16_t *wb,
+                            const float *bb, uint16_t *out, float *out12, const uint32_t *skip,
+                            hipStream_t s)
 {
     if (h <= 0 || w <= 0)
         return hipSuccess;
-    const uint32_t strips = (uint32_t)((w + PS_W - 1) / PS_W);
-    const uint32_t segs = (uint32_t)((h + PS_SEG - 1) / PS_SEG);
-    hipLaunchKernelGGL(k_conv_pair, dim3(strips * segs), dim3(256), 0, s, in, h, w, wa, ba, wb,
-                       bb, out, skip);
+    const dim3 grid((uint32_t)((w + PS_W - 1) / PS_W) * (uint32_t)((h + PS_SEG - 1) / PS_SEG));
+    if (first && last)
+        hipLaunchKernelGGL((k_conv_pair<IM_HEAD, IM_TAIL>), grid, dim3(256), 0, s, in, h, w, wa,
+                           ba, wb, bb, out, out12, skip);
+    else if (first)
+        hipLaunchKernelGGL((k_conv_pair<IM_HEAD, IM_RES>), grid, dim3(256), 0, s, in, h, w, wa, ba,
+                           wb, bb, out, out12, skip);
+    else if (last)
+        hipLaunchKernelGGL((k_conv_pair<IM_RES, IM_TAIL>), grid, dim3(256), 0, s, in, h, w, wa, ba,
+                           wb, bb, out, out12, skip);
+    else
+        hipLaunchKernelGGL((k_conv_pair<IM_RES, IM_RES>), grid, dim3(256), 0, s, in, h, w, wa, ba,
+                           wb, bb, out, out12, skip);
     return hipGetLastError();
 }
 

@@ -531,24 +531,29 @@ extern "C" int prr_net_develop_device(prr_net *net, const prr_header *hdr,
     NET_TRY(launch_pack_frame(f, net->ix0.as<uint16_t>(), flag, s), "pack kernel");
     uint16_t *a = net->ia.as<uint16_t>(), *b = net->ib.as<uint16_t>();
     const uint32_t g = net->conv_grid;
-    NET_TRY(launch_conv_infer(IM_HEAD, net->ix0.as<uint16_t>(), hh, ww, W(0), B(0), a, nullptr,
-                              flag, g, s),
-            "conv kernel");
-    for (int l = 1; l < 7;) {
-        if (net->pairs) {
-            NET_TRY(launch_conv_pair(a, hh, ww, W(l), B(l), W(l + 1), B(l + 1), b, flag, s),
+    if (net->pairs) {
+        /* Layers (0, 1), (2, 3), (4, 5), (6, 7), two a launch. */
+        const uint16_t *in = net->ix0.as<uint16_t>();
+        for (int l = 0; l < 8; l += 2) {
+            NET_TRY(launch_conv_pair(l == 0, l == 6, in, hh, ww, W(l), B(l), W(l + 1), B(l + 1), a,
+                                     net->iout.as<float>(), flag, s),
                     "conv kernel");
-            l += 2;
-        } else {
+            in = a;
+            std::swap(a, b);
+        }
+    } else {
+        NET_TRY(launch_conv_infer(IM_HEAD, net->ix0.as<uint16_t>(), hh, ww, W(0), B(0), a,
+                                  nullptr, flag, g, s),
+                "conv kernel");
+        for (int l = 1; l < 7; l++) {
             NET_TRY(launch_conv_infer(IM_RES, a, hh, ww, W(l), B(l), b, nullptr, flag, g, s),
                     "conv kernel");
-            l += 1;
+            std::swap(a, b);
         }
-        std::swap(a, b);
+        NET_TRY(launch_conv_infer(IM_TAIL, a, hh, ww, W(7), B(7), nullptr, net->iout.as<float>(),
+                                  flag, g, s),
+                "conv kernel");
     }
-    NET_TRY(launch_conv_infer(IM_TAIL, a, hh, ww, W(7), B(7), nullptr, net->iout.as<float>(), flag,
-                              g, s),
-            "conv kernel");
     NET_TRY(launch_compose(f, fin, net->iout.as<float>(), d_r, d_g, d_b, flag, s),
             "compose kernel");
     d.gate = flag;

@@ -235,8 +235,8 @@ class Net:
                    planes[1].data_ptr(), planes[2].data_ptr(), ctypes.c_void_p(s.cuda_stream))
 
     def set_pairs(self, on: bool):
-        """Residual layers two a launch (the default) or one a launch; the
-        same bits either way."""
+        """The network's layers two a launch (the default) or one a launch;
+        the same bits either way."""
         self._call("prr_net_set_pairs", int(bool(on)))
 
     def last_check(self):

@@ -401,7 +401,7 @@ def test_trained_network_passes_the_floor_and_beats_bilinear(env):
 @pytest.mark.parametrize("cells", [72, 100, 131])
 def test_inference_convolution_equals_the_training_forward(env, cells, pairs):
     """The inference convolutions (persistent, weights held in registers,
-    12 output channels; the residual layers two a launch or one a launch)
+    12 output channels; the layers two a launch or one a launch)
     do the training forward's arithmetic: on a frame that is exactly one
     training crop, the learned develop at Stage::Demosaiced equals, bit for
     bit, min(out + bilinear, 1) built from the training forward's output of
@@ -434,10 +434,11 @@ def test_inference_convolution_equals_the_training_forward(env, cells, pairs):
 
 @pytest.mark.parametrize("pattern", [0, 3])
 def test_paired_residual_layers_equal_single_layers(env, pattern):
-    """The residual layers two a launch (the middle activations kept in LDS
-    along a strip) give the planes of one layer a launch bit for bit, with
-    the shipped weights, on a frame of 9 strips and 6 segments whose last
-    ones are partial, and on a flipped pattern, white balanced to Rec.2020."""
+    """The layers two a launch (the first with the second, residual pairs,
+    the sixth with the last; the middle activations kept in LDS along a
+    strip) give the planes of one layer a launch bit for bit, with the
+    shipped weights, on a frame of 9 strips and 6 segments whose last ones
+    are partial, and on a flipped pattern, white balanced to Rec.2020."""
     torch, L, _ = env
     from proresraw_rs_amd import synth as S
     from proresraw_rs_amd.api import DemosaicParams, Stage
