@@ -1204,16 +1204,19 @@ __global__ __launch_bounds__(256) void k_compose(NetFrame f, Finish fin,
 
 /* Sampled-tile check: per tile, the network's output at each sensel
  * (out + bilinear, white balance and clip at 1, no matrix) against the
- * plain develop in scratch; PSNR with peak 1. One workgroup per tile. */
-__global__ __launch_bounds__(256) void k_check_psnr(NetFrame f, Finish fin,
+ * plain develop in scratch; PSNR with peak 1. One 1,024-lane workgroup per
+ * tile, four sensels a lane, summed in a fixed order. */
+enum { CK_LANES = 1024 };
+
+__global__ __launch_bounds__(CK_LANES) void k_check_psnr(NetFrame f, Finish fin,
                                                     const float *__restrict__ wout, int sy, int sx,
                                                     const uint32_t *__restrict__ tiles,
                                                     const float *__restrict__ scratch,
                                                     float min_db, float *__restrict__ psnr,
                                                     uint32_t *__restrict__ use_plain)
 {
-    __shared__ double s_sum[256];
-    __shared__ uint32_t s_n[256];
+    __shared__ double s_sum[CK_LANES];
+    __shared__ uint32_t s_n[CK_LANES];
     __shared__ uint32_t s_curve[9];
     stage_curve(f, s_curve);
     __syncthreads();
@@ -1225,7 +1228,7 @@ __global__ __launch_bounds__(256) void k_check_psnr(NetFrame f, Finish fin,
     const float *out = wout + (size_t)k * sy * sx * C;
     double sum = 0.0;
     uint32_t n = 0;
-    for (uint32_t i = threadIdx.x; i < 4096; i += 256) {
+    for (uint32_t i = threadIdx.x; i < 4096; i += CK_LANES) {
         const uint32_t py = y0 + i / 64, px = x0 + i % 64; /* sensor order */
         if (py >= f.height || px >= f.width)
             continue;
@@ -1246,7 +1249,7 @@ __global__ __launch_bounds__(256) void k_check_psnr(NetFrame f, Finish fin,
     s_sum[threadIdx.x] = sum;
     s_n[threadIdx.x] = n;
     __syncthreads();
-    for (uint32_t s = 128; s > 0; s >>= 1) {
+    for (uint32_t s = CK_LANES / 2; s > 0; s >>= 1) {
         if (threadIdx.x < s) {
             s_sum[threadIdx.x] += s_sum[threadIdx.x + s];
             s_n[threadIdx.x] += s_n[threadIdx.x + s];
@@ -1283,8 +1286,8 @@ hipError_t launch_check(const DevelopDesc &d, const NetFrame &f, const Finish &f
     hipError_t e = launch_develop_sampled(d, tiles, n_tiles, scratch, s);
     if (e != hipSuccess)
         return e;
-    hipLaunchKernelGGL(k_check_psnr, dim3(n_tiles), dim3(256), 0, s, f, fin, wout, sy, sx, tiles,
-                       scratch, min_db, psnr, use_plain);
+    hipLaunchKernelGGL(k_check_psnr, dim3(n_tiles), dim3(CK_LANES), 0, s, f, fin, wout, sy, sx,
+                       tiles, scratch, min_db, psnr, use_plain);
     return hipGetLastError();
 }
 
