@@ -96,14 +96,18 @@ __global__ void k_weights(const float *__restrict__ params, uint16_t *__restrict
  *
  * One 256-lane workgroup computes 4 rows x 64 pixels x 32 output channels,
  * one row per wave. The input rows with a one-pixel halo (zero outside the
- * image) and the 9x32x32 weights are staged in LDS with an 80-byte pixel
+ * image) and the 9x32x32 weights are staged in LDS with a 96-byte pixel
  * stride. For each tap a wave issues 8 MFMAs: 2 output-channel tiles x 4
  * pixel tiles of 16, K = the 32 input channels. Operand layout of
  * v_mfma_f32_16x16x32_bf16: lane l holds A[l%16][8(l/16)..+7] (A = weights,
  * rows = output channels), B[8(l/16)..+7][l%16] (B = input, columns =
  * pixels) and gets D[4(l/16)+i][l%16], i = 0..3. */
 
-enum { CT_ROWS = 4, CT_COLS = 64, CP = 40 };
+/* Pixels in LDS are CP bf16 channels apart (96 bytes): measured on gfx950
+ * with the B-operand read pattern below (16 consecutive pixels, four 16-byte
+ * quarters), a 64- or 80-byte stride costs 4 bank-conflict cycles a read,
+ * 72 bytes none but unaligned 16-byte reads, 96 bytes none and the fastest. */
+enum { CT_ROWS = 4, CT_COLS = 64, CP = 48 };
 
 template <int EPI>
 __global__ __launch_bounds__(256) void k_conv(const uint16_t *__restrict__ in, int H, int W,
@@ -404,11 +408,9 @@ __global__ __launch_bounds__(256) void k_conv_pair(const uint16_t *__restrict__ 
         ((uint4 *)&s_h[0][0][0])[i] = make_uint4(0, 0, 0, 0);
     }
     /* h0 row y, columns x0 - 2 .. x0 + 63 (zero outside the picture), into
-     * registers: 66 pixels x 4 16-byte quarters, two a lane. Rows are
-     * loaded two steps before they are stored to the ring (`next` holds the
-     * row of the coming step, `later` the one after), so a load has two
-     * steps of matrix work to arrive in. */
-    uint4 next[2], later[2];
+     * registers: 66 pixels x 4 16-byte quarters, two a lane; loaded a step
+     * before it is stored to the ring. */
+    uint4 next[2];
     auto fetch = [&](int y, uint4 (&pre)[2]) {
 #pragma unroll
         for (int k = 0; k < 2; k++) {
@@ -433,17 +435,19 @@ __global__ __launch_bounds__(256) void k_conv_pair(const uint16_t *__restrict__ 
     fetch(y0 - 1, next);
     stash(y0 - 1, next);
     fetch(y0, next);
-    fetch(y0 + 1, later);
+    stash(y0, next);
+    fetch(y0 + 1, next);
 
+    /* One barrier a step: at step t, layer A reads h0 rows t - 1 .. t + 1
+     * (stored by step t - 1 or before) while row t + 2 goes to the slot of
+     * row t - 2, which step t - 1 was the last to read; layer B reads h1
+     * rows t - 3 .. t - 1 while A writes row t. The barrier that opens step
+     * t + 1 orders all of these. */
     for (int t = y0 - 1; t <= y1 + 1; t++) {
-        __syncthreads(); /* h1 of step t - 1 written; h0 row t - 3's slot free */
-        stash(t + 1, next);
-#pragma unroll
-        for (int k = 0; k < 2; k++)
-            next[k] = later[k];
-        if (t + 3 <= y1 + 1)
-            fetch(t + 3, later);
         __syncthreads();
+        stash(t + 2, next);
+        if (t + 3 <= y1 + 1)
+            fetch(t + 3, next);
         f32x4 acc[2][2];
 #pragma unroll
         for (int mt = 0; mt < 2; mt++)
@@ -455,20 +459,30 @@ __global__ __launch_bounds__(256) void k_conv_pair(const uint16_t *__restrict__ 
                 continue;
             /* h1 row t: pixel x0 - 1 + 16 n + r16 sits at ring column
              * 16 n + r16 + 1; tap (ky, kx) reads h0 column 16 n + r16 + kx. */
-#pragma unroll
-            for (int tap = 0; tap < TAPS; tap++) {
+            /* Each tap's B fragments are read from LDS while the previous
+             * tap's MFMAs run. */
+            auto frag = [&](int tap, bf16x8 (&b)[2]) {
                 const int ky = tap / 3, kx = tap % 3;
-                bf16x8 b[2];
 #pragma unroll
                 for (int n = 0; n < 2; n++)
                     b[n] = *(const bf16x8 *)&s_x[(t - 1 + ky) & (RING - 1)]
                                                 [16 * (nt0 + n) + r16 + kx][kq];
+            };
+            bf16x8 bc[2], bn[2];
+            frag(0, bc);
+#pragma unroll
+            for (int tap = 0; tap < TAPS; tap++) {
+                if (tap + 1 < TAPS)
+                    frag(tap + 1, bn);
 #pragma unroll
                 for (int mt = 0; mt < 2; mt++)
 #pragma unroll
                     for (int n = 0; n < 2; n++)
-                        acc[mt][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa[tap][mt], b[n],
+                        acc[mt][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa[tap][mt], bc[n],
                                                                              acc[mt][n], 0, 0, 0);
+#pragma unroll
+                for (int n = 0; n < 2; n++)
+                    bc[n] = bn[n];
             }
             const bool row_in = t >= 0 && t < H;
 #pragma unroll
@@ -497,20 +511,28 @@ __global__ __launch_bounds__(256) void k_conv_pair(const uint16_t *__restrict__ 
                 continue;
             /* h2 row r: pixel x0 + 16 n + r16 sits at ring column
              * 16 n + r16 + 2; tap (ky, kx) reads h1 column 16 n + r16 + kx + 1. */
-#pragma unroll
-            for (int tap = 0; tap < TAPS; tap++) {
+            auto frag = [&](int tap, bf16x8 (&b)[2]) {
                 const int ky = tap / 3, kx = tap % 3;
-                bf16x8 b[2];
 #pragma unroll
                 for (int n = 0; n < 2; n++)
                     b[n] = *(const bf16x8 *)&s_h[(r - 1 + ky) & (RING - 1)]
                                                 [16 * (nt0 + n) + r16 + kx + 1][kq];
+            };
+            bf16x8 bc[2], bn[2];
+            frag(0, bc);
+#pragma unroll
+            for (int tap = 0; tap < TAPS; tap++) {
+                if (tap + 1 < TAPS)
+                    frag(tap + 1, bn);
 #pragma unroll
                 for (int mt = 0; mt < MTB; mt++)
 #pragma unroll
                     for (int n = 0; n < 2; n++)
-                        acc[mt][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa[tap][mt], b[n],
+                        acc[mt][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa[tap][mt], bc[n],
                                                                              acc[mt][n], 0, 0, 0);
+#pragma unroll
+                for (int n = 0; n < 2; n++)
+                    bc[n] = bn[n];
             }
 #pragma unroll
             for (int n = 0; n < 2; n++) {
