@@ -376,8 +376,13 @@ __global__ __launch_bounds__(256) void k_conv_pair(const uint16_t *__restrict__ 
     static_assert(BM == IM_RES || BM == IM_TAIL, "layer B: a residual or the last layer");
     /* The last layer computes one 16-channel tile (its 12 outputs). */
     constexpr int MTB = BM == IM_TAIL ? 1 : 2;
-    __shared__ __attribute__((aligned(16))) uint16_t s_x[RING][PS_COLS][CP];
-    __shared__ __attribute__((aligned(16))) uint16_t s_h[RING][PS_COLS][CP];
+    /* Rows in LDS quarter-major: [row][channel quarter][pixel][8 channels],
+     * so the B-operand reads (16 consecutive pixels of one quarter), the
+     * epilogue's 8-byte reads and writes and the row stores are all
+     * consecutive 16- or 8-byte accesses (a pixel-major row with a 96-byte
+     * stride gave those many bank conflicts on gfx950). */
+    __shared__ __attribute__((aligned(16))) uint16_t s_x[RING][4][PS_COLS][8];
+    __shared__ __attribute__((aligned(16))) uint16_t s_h[RING][4][PS_COLS][8];
     if (skip && *skip)
         return;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -403,7 +408,7 @@ __global__ __launch_bounds__(256) void k_conv_pair(const uint16_t *__restrict__ 
         for (int i = 0; i < 4; i++)
             bv[mt][i] = bias[16 * mt + (lane >> 4) * 4 + i];
 
-    for (int i = tid; i < RING * PS_COLS * CP / 8; i += 256) {
+    for (int i = tid; i < RING * 4 * PS_COLS; i += 256) {
         ((uint4 *)&s_x[0][0][0])[i] = make_uint4(0, 0, 0, 0);
         ((uint4 *)&s_h[0][0][0])[i] = make_uint4(0, 0, 0, 0);
     }
@@ -426,7 +431,7 @@ __global__ __launch_bounds__(256) void k_conv_pair(const uint16_t *__restrict__ 
         for (int k = 0; k < 2; k++) {
             const int i = tid + 256 * k;
             if (i < PS_SLOTS)
-                *(uint4 *)&s_x[y & (RING - 1)][i >> 2][8 * (i & 3)] = pre[k];
+                *(uint4 *)&s_x[y & (RING - 1)][i & 3][i >> 2][0] = pre[k];
         }
     };
     __syncthreads();
@@ -465,8 +470,8 @@ __global__ __launch_bounds__(256) void k_conv_pair(const uint16_t *__restrict__ 
                 const int ky = tap / 3, kx = tap % 3;
 #pragma unroll
                 for (int n = 0; n < 2; n++)
-                    b[n] = *(const bf16x8 *)&s_x[(t - 1 + ky) & (RING - 1)]
-                                                [16 * (nt0 + n) + r16 + kx][kq];
+                    b[n] = *(const bf16x8 *)&s_x[(t - 1 + ky) & (RING - 1)][lane >> 4]
+                                                [16 * (nt0 + n) + r16 + kx][0];
             };
             bf16x8 bc[2], bn[2];
             frag(0, bc);
@@ -501,8 +506,8 @@ __global__ __launch_bounds__(256) void k_conv_pair(const uint16_t *__restrict__ 
                         o = make_uint2(f2bf2(fmaxf(v[0], 0.0f), fmaxf(v[1], 0.0f)),
                                        f2bf2(fmaxf(v[2], 0.0f), fmaxf(v[3], 0.0f)));
                     else
-                        o = resid_relu4(*(const uint2 *)&s_x[t & (RING - 1)][col][co], v);
-                    *(uint2 *)&s_h[t & (RING - 1)][col][co] = in_pic ? o : make_uint2(0, 0);
+                        o = resid_relu4(*(const uint2 *)&s_x[t & (RING - 1)][co >> 3][col][co & 7], v);
+                    *(uint2 *)&s_h[t & (RING - 1)][co >> 3][col][co & 7] = in_pic ? o : make_uint2(0, 0);
                 }
             }
         } else {
@@ -515,8 +520,8 @@ __global__ __launch_bounds__(256) void k_conv_pair(const uint16_t *__restrict__ 
                 const int ky = tap / 3, kx = tap % 3;
 #pragma unroll
                 for (int n = 0; n < 2; n++)
-                    b[n] = *(const bf16x8 *)&s_h[(r - 1 + ky) & (RING - 1)]
-                                                [16 * (nt0 + n) + r16 + kx + 1][kq];
+                    b[n] = *(const bf16x8 *)&s_h[(r - 1 + ky) & (RING - 1)][lane >> 4]
+                                                [16 * (nt0 + n) + r16 + kx + 1][0];
             };
             bf16x8 bc[2], bn[2];
             frag(0, bc);
@@ -552,7 +557,7 @@ __global__ __launch_bounds__(256) void k_conv_pair(const uint16_t *__restrict__ 
                                 make_float4(v[0], v[1], v[2], v[3]);
                     } else {
                         *(uint2 *)(out + ((size_t)r * W + x) * C + co) =
-                            resid_relu4(*(const uint2 *)&s_h[r & (RING - 1)][px + 2][co], v);
+                            resid_relu4(*(const uint2 *)&s_h[r & (RING - 1)][co >> 3][px + 2][co & 7], v);
                     }
                 }
             }
