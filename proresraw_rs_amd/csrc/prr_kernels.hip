@@ -1419,6 +1419,13 @@ __device__ inline float estimate_pre(const float *xx, const float *gg, const flo
  * ran 10 % slower; 64 x 8 tiles recompute more halo than occupancy repays. */
 enum { DTW = 64, DTH = 16, NORM_W = DTW + 6, NORM_H = DTH + 6, GRN_W = DTW + 4, GRN_H = DTH + 2 };
 
+/* Lanes of a develop workgroup: four waves, six workgroups a CU. 512 lanes
+ * (eight waves, four workgroups a CU: more waves a SIMD, half the work a
+ * lane) ran 8 % slower. */
+constexpr uint32_t DEV_LANES = 256;
+static_assert(DEV_LANES % 64 == 0 && (DEV_LANES / 64) % 2 == 0 && DTH % (DEV_LANES / 32) == 0,
+              "rows a pass are even (one Bayer phase a lane) and divide the tile");
+
 /* LDS of one develop tile: normalized values, and beside each the noise
  * deviation it has as a green-pass neighbour (sqrt(variance) at the green
  * pass offset of the value clipped at 1 for a green sensel, at its row's
@@ -1723,7 +1730,7 @@ __device__ __forceinline__ void store_pair(float *out_r, float *out_g, float *ou
     }
 }
 
-/* One 64x16 tile of output per 256-lane workgroup: normalized code values
+/* One 64x16 tile of output per DEV_LANES-lane workgroup: normalized code values
  * with a 3-sensel halo and green-pass cells with a 1-sensel halo live in
  * LDS; each lane handles a red/blue + green pair of sensels so every lane
  * does the same work. Output sensel (x, y) goes to (y - oy) * ostride +
@@ -1744,11 +1751,21 @@ __device__ inline void develop_tile_as(const DevelopDesc &d, const DevK &k, floa
      * one's deviation as a green-pass neighbour (outside the picture, by the
      * Bayer phase of the position: those entries are never used unmirrored). */
     const float clip_red = d.clip[0], clip_blue = d.clip[1];
-    auto load = [&](int32_t ly, int32_t lx, float cap) {
+    /* Code value at tile position (ly, lx), clamped to the picture. Rows are
+     * addressed from the tile's first clamped row, so a lane's offsets are
+     * 32-bit: (row - ylo) < NORM_H + 2 and the stride < 2^17 (header widths
+     * are 16-bit). */
+    const int32_t ylo = y0 - 3 < 0 ? 0 : y0 - 3;
+    const uint8_t *rows = reinterpret_cast<const uint8_t *>(d.samples + (size_t)ylo * d.stride);
+    auto code_at = [&](int32_t ly, int32_t lx) -> uint32_t {
         const int32_t py = y0 - 3 + ly, px = x0 - 3 + lx;
         const int32_t y = py < 0 ? 0 : py > (int32_t)h - 1 ? (int32_t)h - 1 : py;
         const int32_t x = px < 0 ? 0 : px > (int32_t)w - 1 ? (int32_t)w - 1 : px;
-        const float v = norm_of(d, lds.curve, d.samples[(size_t)y * d.stride + (uint32_t)x]);
+        const uint32_t off = __umul24((uint32_t)(y - ylo), d.stride) + (uint32_t)x;
+        return *reinterpret_cast<const uint16_t *>(rows + 2u * off);
+    };
+    auto store = [&](int32_t ly, int32_t lx, uint32_t code, float cap) {
+        const float v = norm_of(d, lds.curve, code);
         lds.u.in.norm[ly][lx] = v;
         lds.u.in.dev[ly][lx] = deviation<IN_RANGE>(fminf(v, cap), k.bs, k);
     };
@@ -1763,18 +1780,29 @@ __device__ inline void develop_tile_as(const DevelopDesc &d, const DevK &k, floa
     {
         /* The first 64 columns: lane tid keeps column tid % 64 and takes
          * rows tid / 64 + 4j, so its column clamp and (rows 4 apart) its
-         * Bayer phase are fixed. */
-        static_assert(NORM_W >= 64, "the halo row covers 64 lanes");
+         * Bayer phase are fixed; the remaining columns, one sensel a lane.
+         * Every code value a lane needs is requested before the first is
+         * used (rows past the tile read a clamped row and are not kept), so
+         * a lane waits for memory once. */
+        constexpr int32_t STEP = DEV_LANES / 64, ROWS = (NORM_H + STEP - 1) / STEP;
+        constexpr uint32_t EXTRA = NORM_H * (NORM_W - 64);
+        static_assert(NORM_W >= 64 && EXTRA <= DEV_LANES, "the halo fits one pass of the lanes");
         const int32_t lx = (int32_t)(tid & 63), ly0 = (int32_t)(tid >> 6);
+        uint32_t code[ROWS];
+#pragma unroll
+        for (int32_t j = 0; j < ROWS; j++)
+            code[j] = code_at(ly0 + STEP * j, lx);
+        const uint32_t ei = tid < EXTRA ? tid : 0u;
+        const int32_t ely = (int32_t)(ei / (NORM_W - 64));
+        const int32_t elx = 64 + (int32_t)(ei - (uint32_t)ely * (NORM_W - 64));
+        const uint32_t ecode = code_at(ely, elx);
         const float cap = cap_of(ly0, lx);
-        for (int32_t ly = ly0; ly < NORM_H; ly += 4)
-            load(ly, lx, cap);
-        /* The remaining columns, one sensel a lane. */
-        for (uint32_t i = tid; i < NORM_H * (NORM_W - 64); i += 256) {
-            const int32_t ly = (int32_t)(i / (NORM_W - 64));
-            const int32_t lx2 = 64 + (int32_t)(i - (uint32_t)ly * (NORM_W - 64));
-            load(ly, lx2, cap_of(ly, lx2));
-        }
+#pragma unroll
+        for (int32_t j = 0; j < ROWS; j++)
+            if (ly0 + STEP * j < NORM_H)
+                store(ly0 + STEP * j, lx, code[j], cap);
+        if (tid < EXTRA)
+            store(ely, elx, ecode, cap_of(ely, elx));
     }
     __syncthreads();
 
@@ -1808,13 +1836,13 @@ __device__ inline void develop_tile_as(const DevelopDesc &d, const DevK &k, floa
     };
     {
         /* Pair columns 0..31: lane tid keeps column tid % 32 over rows
-         * tid / 32 + 8j (one Bayer phase); columns 32 and 33 go to lanes
-         * 64..99, so no wave takes more than 3 cells (as 612 cells over 256
-         * lanes need). */
+         * tid / 32 + (lanes / 32) j (one Bayer phase); columns 32 and 33 go
+         * to lanes 64..99, whose wave has one row fewer than the first. */
         static_assert(GRN_W / 2 == 34 && GRN_H == 18, "green-pass cells of a 64x16 tile");
+        static_assert(GRN_H % (DEV_LANES / 32) == 2, "the first wave's rows are the ones with a row more");
         const int32_t lp = (int32_t)(tid & 31), ly0 = (int32_t)(tid >> 5);
 #pragma unroll 1
-        for (int32_t ly = ly0; ly < GRN_H; ly += 8)
+        for (int32_t ly = ly0; ly < GRN_H; ly += DEV_LANES / 32)
             green_cells(ly, lp);
         if (tid >= 64 && tid < 64 + 2 * GRN_H) {
             const int32_t i = (int32_t)tid - 64;
@@ -1829,7 +1857,7 @@ __device__ inline void develop_tile_as(const DevelopDesc &d, const DevK &k, floa
          * once per cell instead of once per use (a cell is a neighbour of up
          * to eight pixels). Cells outside the picture hold whatever the LDS
          * held; no estimate reads them. */
-        for (uint32_t i = tid; i < GRN_H * (GRN_W / 2); i += 256) {
+        for (uint32_t i = tid; i < GRN_H * (GRN_W / 2); i += DEV_LANES) {
             const uint32_t ly = i / (GRN_W / 2), lp = i - ly * (GRN_W / 2);
             const float4 c = lds.rb[ly][lp];
             lds.u.rcp[ly][lp] = make_float4(refined_rcp(c.y), refined_rcp(c.w + c.y),
@@ -1841,9 +1869,11 @@ __device__ inline void develop_tile_as(const DevelopDesc &d, const DevK &k, floa
 
     /* Chroma pass, white balance and output matrix: each lane takes the
      * red/blue pixel and the green pixel of one pair. Lane tid takes pair
-     * column tid % 32 of rows tid / 32 + 8j: the column, its cells and
-     * (rows 8 apart) the Bayer phase stay the same for the lane. */
-    static_assert(DTW / 2 == 32 && DTH % 8 == 0, "chroma lanes cover 8 rows of 32 pairs");
+     * column tid % 32 of rows tid / 32 + (lanes / 32) j: the column, its
+     * cells and (an even number of rows apart) the Bayer phase stay the same
+     * for the lane. */
+    constexpr uint32_t CROWS = DEV_LANES / 32;
+    static_assert(DTW / 2 == 32 && DTH % CROWS == 0, "chroma lanes cover whole rows of 32 pairs");
     constexpr uint32_t RW = GRN_W / 2; /* cells a row */
     const uint32_t lp = tid % (DTW / 2), ly0 = tid / (DTW / 2);
     const uint32_t xb = (uint32_t)x0 + 2 * lp;
@@ -1866,12 +1896,12 @@ __device__ inline void develop_tile_as(const DevelopDesc &d, const DevK &k, floa
     const float2 *gcell = &lds.gr[0][0];
     const float4 *rcpcell = &lds.u.rcp[0][0];
 #pragma unroll
-    for (uint32_t j = 0; j < DTH / 8; j++) {
-        const uint32_t y = (uint32_t)y0 + ly0 + 8 * j;
+    for (uint32_t j = 0; j < DTH / CROWS; j++) {
+        const uint32_t y = (uint32_t)y0 + ly0 + CROWS * j;
         if (y >= h || xb >= w)
             continue;
         /* Row y is cell row ly0 + 8j + 1. */
-        pc.r0 = (ly0 + 8 * j + 1) * RW;
+        pc.r0 = (ly0 + CROWS * j + 1) * RW;
         pc.rm = y > 0 ? pc.r0 - RW : pc.r0 + RW;
         pc.rp = y < h - 1 ? pc.r0 + RW : pc.r0 - RW;
         float res[2][3];
@@ -1906,7 +1936,7 @@ __device__ __forceinline__ uint32_t xcd_tile()
     return (xcd < extra ? xcd * (per + 1) : extra * (per + 1) + (xcd - extra) * per) + b / 8;
 }
 
-__global__ __launch_bounds__(256) void k_develop(DevelopDesc d, float *out_r, float *out_g,
+__global__ __launch_bounds__(DEV_LANES) void k_develop(DevelopDesc d, float *out_r, float *out_g,
                                                  float *out_b)
 {
     __shared__ DevelopLds lds;
@@ -1923,7 +1953,7 @@ __global__ __launch_bounds__(256) void k_develop(DevelopDesc d, float *out_r, fl
  * into scratch[k] = r, g, b planes of 64x64, for the learned path's check. */
 constexpr uint32_t SAMPLE_PARTS = 64 / DTH;
 static_assert(DTW == 64 && 64 % DTH == 0, "a 64x64 sample is whole develop tiles");
-__global__ __launch_bounds__(256) void k_develop_sampled(DevelopDesc d, const uint32_t *tiles,
+__global__ __launch_bounds__(DEV_LANES) void k_develop_sampled(DevelopDesc d, const uint32_t *tiles,
                                                          float *scratch)
 {
     __shared__ DevelopLds lds;
@@ -2191,7 +2221,7 @@ __global__ __launch_bounds__(64) void k_batch_decode(const BatchFrame *__restric
 
 /* Develop every full-decode frame of a batch: blockIdx.z picks the frame;
  * workgroups past a smaller frame's extent return at once. */
-__global__ __launch_bounds__(256) void k_batch_develop(const BatchDevelop *jobs)
+__global__ __launch_bounds__(DEV_LANES) void k_batch_develop(const BatchDevelop *jobs)
 {
     __shared__ DevelopLds lds;
     const BatchDevelop &j = jobs[blockIdx.z];
@@ -2294,7 +2324,7 @@ hipError_t launch_develop(const DevelopDesc &d, float *r, float *g, float *b, hi
     if (d.width == 0 || d.height == 0)
         return hipSuccess;
     dim3 grid((d.width + DTW - 1) / DTW, (d.height + DTH - 1) / DTH);
-    hipLaunchKernelGGL(k_develop, grid, dim3(256), 0, s, d, r, g, b);
+    hipLaunchKernelGGL(k_develop, grid, dim3(DEV_LANES), 0, s, d, r, g, b);
     return hipGetLastError();
 }
 
@@ -2319,7 +2349,7 @@ hipError_t launch_batch(const BatchFrame *d_frames, uint32_t n_frames, uint32_t 
     }
     if (n_jobs) {
         dim3 g((max_w + DTW - 1) / DTW, (max_h + DTH - 1) / DTH, n_jobs);
-        hipLaunchKernelGGL(k_batch_develop, g, dim3(256), 0, s, d_jobs);
+        hipLaunchKernelGGL(k_batch_develop, g, dim3(DEV_LANES), 0, s, d_jobs);
     }
     return hipGetLastError();
 }
@@ -2333,7 +2363,7 @@ hipError_t launch_develop_sampled(const DevelopDesc &d, const uint32_t *tiles, u
 {
     if (n_tiles == 0)
         return hipSuccess;
-    hipLaunchKernelGGL(k_develop_sampled, dim3(SAMPLE_PARTS * n_tiles), dim3(256), 0, s, d, tiles,
+    hipLaunchKernelGGL(k_develop_sampled, dim3(SAMPLE_PARTS * n_tiles), dim3(DEV_LANES), 0, s, d, tiles,
                        scratch);
     return hipGetLastError();
 }
