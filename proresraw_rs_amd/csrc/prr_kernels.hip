@@ -1502,7 +1502,8 @@ __device__ __forceinline__ DevK dev_consts(const DevelopDesc &d)
      * [2^-86, 2^22], denominators in [2^-43, 2^12], quotients in [2^-98,
      * 2^65]. */
     k.in_range = k.n1 + k.n2 >= 0x1p-40f && d.one_over_range > 0.0f &&
-                 d.one_over_range <= 0x1p-8f && d.black >= -65536 && d.black <= 65536;
+                 d.one_over_range <= 0x1p-8f && d.black >= -65536 && d.black <= 65536 &&
+                 d.clip[0] == d.clip[0] && d.clip[1] == d.clip[1];
     return k;
 }
 
@@ -1525,6 +1526,23 @@ __device__ __forceinline__ float norm_of(const DevelopDesc &d, const uint32_t *c
     return d.one_over_range * (float)(lin - d.black);
 }
 
+/* min(v, c) for a normalized value v. On an in-range frame v is finite and
+ * the clips are not NaN, and v_med3(v, -inf, c) returns min(v, c) in one
+ * instruction, where fminf of a value read from LDS first quiets it with a
+ * second one. The -inf is opaque to the compiler, which would otherwise
+ * fold the median back into that fminf. */
+template <bool IN_RANGE>
+__device__ __forceinline__ float clip_norm(float v, float c)
+{
+    if constexpr (IN_RANGE) {
+        float lo = -__builtin_inff();
+        asm("" : "+s"(lo));
+        return __builtin_amdgcn_fmed3f(v, lo, c);
+    } else {
+        return fminf(v, c);
+    }
+}
+
 /* Green pass (green_row_impl, demosaic.rs:481-541) of one red/blue sensel
  * (gx, gy): `norm(y, x)` reads normalized values and `dev(y, x)` their
  * deviations as neighbours at picture coordinates (rows and columns outside
@@ -1538,20 +1556,20 @@ __device__ __forceinline__ float4 green_rb(const DevK &k, int32_t gx, int32_t gy
 {
     const float n0 = k.n0, n1 = k.n1, n2 = k.n2, bs = k.bs;
     const float centre = norm(gy, gx);
-    const float c = fminf(centre, clip);
+    const float c = clip_norm<IN_RANGE>(centre, clip);
     const float sc = dev(gy, gx);
     float hh[4] = {norm(gy, gx - 1), norm(gy, gx + 1), norm(gy, gx - 2), norm(gy, gx + 2)};
     float vv[4] = {norm(gy - 1, gx), norm(gy + 1, gx), norm(gy - 2, gx), norm(gy + 2, gx)};
     float sh[4] = {dev(gy, gx - 1), dev(gy, gx + 1), dev(gy, gx - 2), dev(gy, gx + 2)};
     float sv[4] = {dev(gy - 1, gx), dev(gy + 1, gx), dev(gy - 2, gx), dev(gy + 2, gx)};
-    hh[0] = fminf(hh[0], 1.0f);
-    hh[1] = fminf(hh[1], 1.0f);
-    hh[2] = fminf(hh[2], clip);
-    hh[3] = fminf(hh[3], clip);
-    vv[0] = fminf(vv[0], 1.0f);
-    vv[1] = fminf(vv[1], 1.0f);
-    vv[2] = fminf(vv[2], clip);
-    vv[3] = fminf(vv[3], clip);
+    hh[0] = clip_norm<IN_RANGE>(hh[0], 1.0f);
+    hh[1] = clip_norm<IN_RANGE>(hh[1], 1.0f);
+    hh[2] = clip_norm<IN_RANGE>(hh[2], clip);
+    hh[3] = clip_norm<IN_RANGE>(hh[3], clip);
+    vv[0] = clip_norm<IN_RANGE>(vv[0], 1.0f);
+    vv[1] = clip_norm<IN_RANGE>(vv[1], 1.0f);
+    vv[2] = clip_norm<IN_RANGE>(vv[2], clip);
+    vv[3] = clip_norm<IN_RANGE>(vv[3], clip);
     if (edge_x) {
         mirror4(hh, (uint32_t)gx, k.w, c);
         mirror4(sh, (uint32_t)gx, k.w, sc);
@@ -1767,7 +1785,7 @@ __device__ inline void develop_tile_as(const DevelopDesc &d, const DevK &k, floa
     auto store = [&](int32_t ly, int32_t lx, uint32_t code, float cap) {
         const float v = norm_of(d, lds.curve, code);
         lds.u.in.norm[ly][lx] = v;
-        lds.u.in.dev[ly][lx] = deviation<IN_RANGE>(fminf(v, cap), k.bs, k);
+        lds.u.in.dev[ly][lx] = deviation<IN_RANGE>(clip_norm<IN_RANGE>(v, cap), k.bs, k);
     };
     /* The clip of a neighbour by its Bayer phase (a red/blue sensel at its
      * row's clip, a green one at 1). */
