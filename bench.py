@@ -50,6 +50,9 @@ def parse_args():
     ap.add_argument("--batch-frames", type=int, default=64)
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--learned-frames", type=int, default=32)
+    ap.add_argument("--learned-contexts", type=int, default=3,
+                    help="learned-demosaic contexts (each on its own stream) the batch's frames "
+                         "are spread over")
     ap.add_argument("--learned-distinct", type=int, default=32,
                     help="distinct packets (half of each frame kind) cycled in the learned leg "
                          "(default: every frame its own scene)")
@@ -601,16 +604,16 @@ def learned_bench(args, plain_value):
     params, lo, hi = L.load_weights(args.net_weights) if args.net_weights else L.load_weights()
     net = L.Net(params)
     net.set_range(lo, hi)
+    group = L.NetGroup(params, lo, hi, n=max(1, args.learned_contexts))
     lib, ctx = N.lib(), P.context()
     s = torch.cuda.Stream()
     hs = ctypes.c_void_p(s.cuda_stream)
     err = N.PrrError()
+    jobs = [(slot.header, slot.samples, prm, planes, slot.status) for slot, prm, planes in frames]
 
     def learned_step():
         b.run(s)
-        for slot, prm, planes in frames:
-            net.develop(slot.header, slot.samples, prm, P.Stage.Rec2020, planes, status=slot.status,
-                        check_tiles=8, min_db=45.0, stream=s)
+        group.develop(jobs, P.Stage.Rec2020, check_tiles=8, min_db=45.0, stream=s)
 
     def plain_step():
         b.run(s)
@@ -669,7 +672,7 @@ def learned_bench(args, plain_value):
             "coded_bits_per_sensel": round(coded_bits / sensels, 3),
             "decode_bitrate": decode_bound("k_batch_decode", "learned32", coded_bits, tiles,
                                            batch_ms),
-            "distinct_packets": nd,
+            "distinct_packets": nd, "contexts": max(1, args.learned_contexts),
             "workload": "32 frames alternating 8192x4320 v1 a4 q15 and 5760x3240 v0 a3 q45, "
                         "batched decode+linearize, learned demosaic+WB+Rec2020 with fallback"}
 

@@ -254,6 +254,43 @@ class Net:
         return (out.astype(np.uint32) << 16).view(np.float32).reshape(h2, w2, C)
 
 
+class NetGroup:
+    """Learned develop of a batch's frames over several contexts with the
+    same weights, each on a stream of its own, so that one frame's network
+    runs beside the next frame's check, packing and composition (kernels
+    that leave most of the matrix cores, or most of the CUs, idle). Each
+    frame goes to the context with the fewest sensels queued so far (the
+    first on a tie); its result is that of Net.develop (the same kernels on
+    the same inputs)."""
+
+    def __init__(self, params, lo: float, hi: float, n: int = 2):
+        torch = _torch()
+        if n < 1:
+            raise ValueError("at least one context")
+        self.nets = [Net(params) for _ in range(n)]
+        for net in self.nets:
+            net.set_range(lo, hi)
+        self.streams = [torch.cuda.Stream() for _ in range(n)]
+
+    def develop(self, frames, stage: Stage, check_tiles: int = 8, min_db: float = 45.0,
+                stream=None):
+        """frames: (header, samples, params, planes, status) tuples. Work
+        queued on `stream` before the call comes first for every frame, and
+        `stream` waits for all of them (capturable in a graph)."""
+        torch = _torch()
+        s = stream if stream is not None else torch.cuda.current_stream()
+        for side in self.streams:
+            side.wait_stream(s)
+        queued = [0] * len(self.nets)
+        for header, samples, params, planes, status in frames:
+            k = queued.index(min(queued))
+            queued[k] += header.width * header.height
+            self.nets[k].develop(header, samples, params, stage, planes, status=status,
+                                 check_tiles=check_tiles, min_db=min_db, stream=self.streams[k])
+        for side in self.streams:
+            s.wait_stream(side)
+
+
 def random_crops(rng, frames: List[TrainFrame], steps: int, batch: int, cells: int,
                  hard=None, hard_frac: float = 0.0) -> np.ndarray:
     """(steps, batch, 3) crops of 2*cells sensels: frame, top, left (even).

@@ -460,3 +460,78 @@ def test_paired_residual_layers_equal_single_layers(env, pattern):
         assert not net.last_check()[0]
         out[pairs] = np.stack([p.cpu().numpy() for p in planes])
     assert np.array_equal(out[True].view(np.uint32), out[False].view(np.uint32))
+
+
+def test_frames_over_two_contexts_equal_one_context(env):
+    """A batch of frames of two sizes spread over two contexts on their own
+    streams (NetGroup) gives each frame the planes one context gives it,
+    bit for bit, eagerly and in replays of a captured graph; one frame falls
+    back to plain develop (its sampled-tile floor unreachable) in both."""
+    torch, L, _ = env
+    from proresraw_rs_amd import synth as S
+    from proresraw_rs_amd.api import DemosaicParams, Stage
+    params, lo, hi = L.load_weights()
+    frames = []
+    for i, (w, h) in enumerate([(384, 256), (256, 192), (384, 256), (256, 192), (320, 224)]):
+        sc = S.scene(w, h, i % 4, seed=900 + i, version=i % 2, alignment=2,
+                     qscale=(20 + 5 * i, 20 + 5 * i), white_level=4095)
+        frames.append(L.frame_pair(S.encode_scene(sc)))
+
+    def planes_of(f):
+        n = f.header.width * f.header.height
+        return [torch.full((n,), -1.0, dtype=torch.float32, device="cuda") for _ in range(3)]
+
+    single = L.Net(params)
+    single.set_range(lo, hi)
+    want = []
+    for i, f in enumerate(frames):
+        pl = planes_of(f)
+        prm = DemosaicParams.new(f.header, f.quantizer)
+        single.develop(f.header, f.samples, prm, Stage.Rec2020, pl, check_tiles=4,
+                       min_db=500.0 if i == 2 else 45.0)
+        torch.cuda.synchronize()
+        want.append(np.stack([p.cpu().numpy() for p in pl]))
+
+    group = L.NetGroup(params, lo, hi, n=2)
+    outs = [planes_of(f) for f in frames]
+    s = torch.cuda.Stream()
+
+    def run():
+        for i, f in enumerate(frames):
+            prm = DemosaicParams.new(f.header, f.quantizer)
+            group.develop([(f.header, f.samples, prm, outs[i], None)], Stage.Rec2020,
+                          check_tiles=4, min_db=500.0 if i == 2 else 45.0, stream=s)
+
+    def run_batch():
+        # One call per floor: frame 2 alone at the unreachable floor, the
+        # others together.
+        jobs = [(f.header, f.samples, DemosaicParams.new(f.header, f.quantizer), outs[i], None)
+                for i, f in enumerate(frames) if i != 2]
+        group.develop(jobs, Stage.Rec2020, check_tiles=4, min_db=45.0, stream=s)
+        f = frames[2]
+        group.develop([(f.header, f.samples, DemosaicParams.new(f.header, f.quantizer), outs[2],
+                        None)], Stage.Rec2020, check_tiles=4, min_db=500.0, stream=s)
+
+    def check(what):
+        for i in range(len(frames)):
+            got = np.stack([p.cpu().numpy() for p in outs[i]])
+            assert np.array_equal(got.view(np.uint32), want[i].view(np.uint32)), (what, i)
+
+    with torch.cuda.stream(s):
+        run()
+        s.synchronize()
+        check("eager, one frame a call")
+        run_batch()
+        s.synchronize()
+        check("eager, frames spread over the contexts")
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=s):
+            run_batch()
+        for rep in range(3):
+            for o in outs:
+                for p in o:
+                    p.fill_(-1.0)
+            s.synchronize()
+            g.replay()
+            s.synchronize()
+            check(f"graph replay {rep}")
