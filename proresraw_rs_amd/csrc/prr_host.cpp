@@ -669,7 +669,7 @@ struct prr_batch {
     std::vector<uint32_t> chunks; /* (frame << 16) | chunk of the size-table scan */
     DevBuf d_frames, d_jobs, d_queue, d_head, d_hist, d_offsets, d_chunks, d_chunk_bytes;
     uint32_t n_items = 0;
-    uint32_t max_w = 0, max_h = 0;
+    uint32_t dev_tiles = 0; /* develop tiles of the jobs so far */
     uint32_t spw = 64, chunk = 1, waves_per_cu = 0;
     /* LDS a batch workgroup may use: 32 KiB keeps every alignment's group at
      * or below 18 KiB, so 8 persistent workgroups fit a CU (measured best on
@@ -1309,9 +1309,10 @@ extern "C" int prr_batch_add(prr_batch *b, const prr_header *hdr, const prr_fram
         j.r = fr->d_r;
         j.g = fr->d_g;
         j.b = fr->d_b;
+        j.tiles_x = ((uint32_t)hdr->width + DEVELOP_TILE_W - 1) / DEVELOP_TILE_W;
+        j.first_tile = b->dev_tiles;
+        b->dev_tiles += j.tiles_x * (((uint32_t)hdr->height + DEVELOP_TILE_H - 1) / DEVELOP_TILE_H);
         b->jobs.push_back(j);
-        b->max_w = std::max<uint32_t>(b->max_w, hdr->width);
-        b->max_h = std::max<uint32_t>(b->max_h, hdr->height);
     }
     b->uploaded = false;
     return PRR_OK;
@@ -1332,7 +1333,7 @@ extern "C" int prr_batch_clear(prr_batch *b, prr_error *err)
     b->offsets_total = 0;
     b->chunks.clear();
     b->n_items = 0;
-    b->max_w = b->max_h = 0;
+    b->dev_tiles = 0;
     b->lds = 0;
     b->uploaded = false;
     return PRR_OK;
@@ -1407,7 +1408,7 @@ extern "C" int prr_batch_run(prr_batch *b, void *stream, prr_error *err)
                          (uint32_t *)b->d_hist.ptr, (const uint32_t *)b->d_chunks.ptr,
                          (uint32_t)b->chunks.size(), (uint32_t *)b->d_chunk_bytes.ptr,
                          (const BatchDevelop *)b->d_jobs.ptr,
-                         (uint32_t)b->jobs.size(), b->max_w, b->max_h, b->grid, b->chunk, b->lds, s),
+                         (uint32_t)b->jobs.size(), b->dev_tiles, b->grid, b->chunk, b->lds, s),
             "batch kernels");
     return PRR_OK;
 }

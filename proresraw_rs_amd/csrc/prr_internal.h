@@ -86,9 +86,17 @@ struct BatchFrame {
     prr_device_status *status;
 };
 
+/* Develop tiles (pixels): every develop launch covers its frame with
+ * tiles of this size. */
+constexpr uint32_t DEVELOP_TILE_W = 64, DEVELOP_TILE_H = 16;
+
+/* A develop of a batch: its frame's tiles are tiles first_tile ..
+ * first_tile + tiles_x * ceil(height / DEVELOP_TILE_H) - 1 of the batch's
+ * develop launch, row by row. */
 struct BatchDevelop {
     DevelopDesc d;
     float *r, *g, *b;
+    uint32_t first_tile, tiles_x;
 };
 
 /* Launch configuration of the full and reduced decode kernels. */
@@ -145,7 +153,7 @@ hipError_t launch_develop(const DevelopDesc &d, float *r, float *g, float *b, hi
 hipError_t launch_batch(const BatchFrame *d_frames, uint32_t n_frames, uint32_t n_items,
                         uint32_t *d_queue, uint32_t *d_head, uint32_t *d_hist,
                         const uint32_t *d_chunks, uint32_t n_chunks, uint32_t *d_chunk_bytes,
-                        const BatchDevelop *d_jobs, uint32_t n_jobs, uint32_t max_w, uint32_t max_h, uint32_t grid,
+                        const BatchDevelop *d_jobs, uint32_t n_jobs, uint32_t dev_tiles, uint32_t grid,
                         uint32_t chunk, size_t lds, hipStream_t s);
 size_t decode_lds_bytes(uint32_t spw, uint32_t per_block, uint32_t amax);
 /* LDS of a full decode workgroup (any spw). */

@@ -1419,6 +1419,8 @@ __device__ inline float estimate_pre(const float *xx, const float *gg, const flo
  * ran 10 % slower; 64 x 8 tiles recompute more halo than occupancy repays. */
 enum { DTW = 64, DTH = 16, NORM_W = DTW + 6, NORM_H = DTH + 6, GRN_W = DTW + 4, GRN_H = DTH + 2 };
 
+static_assert(DTW == DEVELOP_TILE_W && DTH == DEVELOP_TILE_H, "the host's tile size");
+
 /* Lanes of a develop workgroup: four waves, six workgroups a CU. 512 lanes
  * (eight waves, four workgroups a CU: more waves a SIMD, half the work a
  * lane) ran 8 % slower. */
@@ -2237,18 +2239,28 @@ __global__ __launch_bounds__(64) void k_batch_decode(const BatchFrame *__restric
     }
 }
 
-/* Develop every full-decode frame of a batch: blockIdx.z picks the frame;
- * workgroups past a smaller frame's extent return at once. */
-__global__ __launch_bounds__(DEV_LANES) void k_batch_develop(const BatchDevelop *jobs)
+/* Develop every full-decode frame of a batch: one workgroup per develop
+ * tile of the batch's frames (their tiles numbered frame after frame), in
+ * the XCD-wise order of k_develop, so that no workgroup is launched for
+ * nothing and each XCD's run of tiles shares its halo rows in its L2. The
+ * frame of tile t is the last job whose first tile is at most t (jobs are
+ * in tile order). */
+__global__ __launch_bounds__(DEV_LANES) void k_batch_develop(const BatchDevelop *jobs,
+                                                             uint32_t n_jobs)
 {
     __shared__ DevelopLds lds;
-    const BatchDevelop &j = jobs[blockIdx.z];
-    /* Plain order: the grid is sized for the largest frame, and an XCD-wise
-     * run of rows would leave the XCDs holding a small frame's lower rows
-     * with nothing to do. */
-    if (blockIdx.x * DTW >= j.d.width || blockIdx.y * DTH >= j.d.height)
-        return;
-    develop_tile(j.d, j.r, j.g, j.b, blockIdx.x, blockIdx.y, lds, 0, 0, j.d.width);
+    const uint32_t t = xcd_tile();
+    uint32_t lo = 0, hi = n_jobs;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (jobs[mid].first_tile <= t)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    const BatchDevelop &j = jobs[lo];
+    const uint32_t k = t - j.first_tile;
+    develop_tile(j.d, j.r, j.g, j.b, k % j.tiles_x, k / j.tiles_x, lds, 0, 0, j.d.width);
 }
 
 /* ------------------------------------------------------------------ */
@@ -2350,7 +2362,7 @@ hipError_t launch_batch(const BatchFrame *d_frames, uint32_t n_frames, uint32_t 
                         uint32_t *d_queue, uint32_t *d_head, uint32_t *d_hist,
                         const uint32_t *d_chunks, uint32_t n_chunks, uint32_t *d_chunk_bytes,
                         const BatchDevelop *d_jobs,
-                        uint32_t n_jobs, uint32_t max_w, uint32_t max_h, uint32_t grid,
+                        uint32_t n_jobs, uint32_t dev_tiles, uint32_t grid,
                         uint32_t chunk, size_t lds, hipStream_t s)
 {
     if (n_frames == 0)
@@ -2365,10 +2377,8 @@ hipError_t launch_batch(const BatchFrame *d_frames, uint32_t n_frames, uint32_t 
         hipLaunchKernelGGL(k_batch_decode, dim3(grid), dim3(64), lds, s, d_frames, d_queue,
                            n_items, d_head, chunk);
     }
-    if (n_jobs) {
-        dim3 g((max_w + DTW - 1) / DTW, (max_h + DTH - 1) / DTH, n_jobs);
-        hipLaunchKernelGGL(k_batch_develop, g, dim3(DEV_LANES), 0, s, d_jobs);
-    }
+    if (n_jobs && dev_tiles)
+        hipLaunchKernelGGL(k_batch_develop, dim3(dev_tiles), dim3(DEV_LANES), 0, s, d_jobs, n_jobs);
     return hipGetLastError();
 }
 
