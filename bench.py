@@ -50,7 +50,7 @@ def parse_args():
     ap.add_argument("--batch-frames", type=int, default=64)
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--learned-frames", type=int, default=32)
-    ap.add_argument("--learned-contexts", type=int, default=3,
+    ap.add_argument("--learned-contexts", type=int, default=4,
                     help="learned-demosaic contexts (each on its own stream) the batch's frames "
                          "are spread over")
     ap.add_argument("--learned-distinct", type=int, default=32,

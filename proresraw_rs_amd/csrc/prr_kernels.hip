@@ -1765,7 +1765,6 @@ __device__ inline void develop_tile_as(const DevelopDesc &d, const DevK &k, floa
     const int32_t x0 = (int32_t)(bx * DTW), y0 = (int32_t)(by * DTH);
     if (tid < 9)
         lds.curve[tid] = tid < 8 ? d.curve[tid] : 65536u;
-    __syncthreads();
 
     /* Normalized values, rows and columns clamped to the picture, and each
      * one's deviation as a green-pass neighbour (outside the picture, by the
@@ -1803,7 +1802,8 @@ __device__ inline void develop_tile_as(const DevelopDesc &d, const DevK &k, floa
          * Bayer phase are fixed; the remaining columns, one sensel a lane.
          * Every code value a lane needs is requested before the first is
          * used (rows past the tile read a clamped row and are not kept), so
-         * a lane waits for memory once. */
+         * a lane waits for memory once, and before the barrier that makes
+         * the curve points visible, so the loads overlap it. */
         constexpr int32_t STEP = DEV_LANES / 64, ROWS = (NORM_H + STEP - 1) / STEP;
         constexpr uint32_t EXTRA = NORM_H * (NORM_W - 64);
         static_assert(NORM_W >= 64 && EXTRA <= DEV_LANES, "the halo fits one pass of the lanes");
@@ -1817,6 +1817,7 @@ __device__ inline void develop_tile_as(const DevelopDesc &d, const DevK &k, floa
         const int32_t elx = 64 + (int32_t)(ei - (uint32_t)ely * (NORM_W - 64));
         const uint32_t ecode = code_at(ely, elx);
         const float cap = cap_of(ly0, lx);
+        __syncthreads();
 #pragma unroll
         for (int32_t j = 0; j < ROWS; j++)
             if (ly0 + STEP * j < NORM_H)
