@@ -125,6 +125,20 @@ def measured_valu(kernel, workload):
     return float(kv["SQ_INSTS_VALU"]), float(kv["valu_per_simd_cycle"])
 
 
+def measured_issue(kernel, workload):
+    """From the counter summary: VALU instructions per SIMD-cycle, the share
+    of a SIMD's quad-cycles that issue a VALU instruction (one, or two from
+    two waves) and the share of those that issue two, where recorded."""
+    kv = profile_fields(UTIL_FILE, kernel, workload)
+    if kv is None:
+        return {}
+    out = {"valu_insts_per_simd_cycle": float(kv["valu_per_simd_cycle"])}
+    for k in ("valu_busy", "dual_issue"):
+        if k in kv:
+            out[k] = float(kv[k])
+    return out
+
+
 def decode_bound(kernel, workload, coded_bits, tiles, kernel_ms):
     """The decode against its per-CU bit-rate bound: the kernel's own VALU
     instruction stream (wave64 instructions per dispatch, from the counter
@@ -350,17 +364,17 @@ def roofline_of(r):
                                       "decode_ms_alone"], 4),
            "bytes_per_launch": int(nbytes), "bytes_per_unit": "14 B/visible sensel"
            if develop_dominant else "packet + 2 B/coded sensel + 2 B/visible sensel"}
-    # The kernel's limiter is its VALU issue, not HBM: instructions per
-    # SIMD-cycle from the counter summary (a stream of wave64 FP32
-    # instructions tops out at 0.25).
+    # The kernels' limiter is their VALU issue, not HBM: from the counter
+    # summary, instructions per SIMD-cycle and the share of the SIMDs'
+    # quad-cycles that issue VALU work (alone: valu_busy; gfx950 issues one
+    # wave64 VALU instruction a quad-cycle, or two from two waves).
     if rate is not None:
-        out.update({"limiter": "valu_issue", "valu_insts_per_simd_cycle": rate,
-                    "valu_issue_frac_of_0.25": round(rate / 0.25, 3),
+        out.update({"limiter": "valu_issue", **measured_issue(kernel, key),
                     "valu_source": os.path.relpath(UTIL_FILE, ROOT)})
     other = "k_decode" if develop_dominant else "k_develop"
     out["other_kernel"] = {"kernel": other, "kernel_ms": round(r["decode_ms" if develop_dominant
                                                                  else "develop_ms"], 4),
-                           "traffic": measured_traffic(other, key)}
+                           "traffic": measured_traffic(other, key), **measured_issue(other, key)}
     out["decode_bitrate"] = decode_bound("k_decode", key, r["coded_bits"], r["nb_tiles"],
                                          r["decode_ms_alone"])
     out["decode_bitrate_frac"] = out["decode_bitrate"].get("frac")
