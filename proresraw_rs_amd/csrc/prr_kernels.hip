@@ -1798,8 +1798,9 @@ __device__ inline void develop_tile_as(const DevelopDesc &d, const DevK &k, floa
     };
     {
         /* The first 64 columns: lane tid keeps column tid % 64 and takes
-         * rows tid / 64 + 4j, so its column clamp and (rows 4 apart) its
-         * Bayer phase are fixed; the remaining columns, one sensel a lane.
+         * rows tid / 64 + STEP j (STEP = lanes / 64, even), so its column
+         * clamp and Bayer phase are fixed; the remaining columns, one
+         * sensel a lane.
          * Every code value a lane needs is requested before the first is
          * used (rows past the tile read a clamped row and are not kept), so
          * a lane waits for memory once, and before the barrier that makes
