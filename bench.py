@@ -42,6 +42,8 @@ def parse_args():
                          "0.3 ms, so a step of one frame would time the pipeline's fill)")
     ap.add_argument("--width", type=int, default=5760)
     ap.add_argument("--height", type=int, default=3240)
+    ap.add_argument("--pipeline-depth", type=int, default=4,
+                    help="frames in flight in the plain legs' frame pipeline")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
@@ -189,8 +191,8 @@ def device_packet(packet):
 
 
 def pipelined(packet, d_packet, hdr, info, planes, stage, idct, frames, warmup,
-              time_kernels=True):
-    """The pipelined stream of frames (prr_pipeline, four buffer sets):
+              time_kernels=True, depth=4):
+    """The pipelined stream of frames (prr_pipeline, `depth` buffer sets):
     `warmup` untimed frames, then `frames` frames submitted back to back; the
     timed region is bracketed by device synchronization, so it holds exactly
     `frames` whole frames. Returns
@@ -198,7 +200,7 @@ def pipelined(packet, d_packet, hdr, info, planes, stage, idct, frames, warmup,
     import torch
 
     from proresraw_rs_amd.pipeline import Pipeline
-    pipe = Pipeline(depth=4)
+    pipe = Pipeline(depth=depth)
     status = torch.zeros(4, dtype=torch.int32, device="cuda")
 
     def submit():
@@ -325,7 +327,8 @@ def plain_leg(args, packet, workload):
         serial_s = time.perf_counter() - t0
 
     elapsed, pipe_dec_ms, pipe_dev_ms = pipelined(packet, d_packet, hdr, info, planes,
-                                                  P.Stage.Rec2020, P.Idct.Apple, frames, warm)
+                                                  P.Stage.Rec2020, P.Idct.Apple, frames, warm,
+                                                  depth=args.pipeline_depth)
     table_bytes = 2 * info.nb_tiles
     coded_bits = 8 * (len(packet) - info.header_end - table_bytes)
     return {
@@ -402,7 +405,8 @@ def c1_leg(args):
     steps = max(args.steps * args.frames_per_step // 2, 10)
     for idct in (P.Idct.Apple, P.Idct.Ffmpeg):
         el, dec, dv = pipelined(packet, d_packet, hdr, info, planes, P.Stage.Rec2020, idct,
-                                steps, args.warmup * args.frames_per_step)
+                                steps, args.warmup * args.frames_per_step,
+                                depth=args.pipeline_depth)
         out[idct.name.lower()] = {"value": round(npix * steps / el / 1e9, 3),
                                   "ms_per_frame": round(1e3 * el / steps, 4),
                                   "decode_ms": round(dec, 4), "develop_ms": round(dv, 4)}
@@ -467,7 +471,7 @@ def main():
                    "coded_bits_per_sensel": round(r["coded_bits"] / r["npix"], 3),
                    "frames_per_step": args.frames_per_step, "frames_timed": r["frames"],
                    "ms_per_frame": round(r["ms_per_frame"], 4),
-                   "pipeline_depth": 4,
+                   "pipeline_depth": args.pipeline_depth,
                    "serial_value": round(r["serial_value"], 3),
                    "serial_ms_per_frame": round(r["serial_ms"], 4),
                    "serial_graph": r["serial_graph"]},
